@@ -1,0 +1,294 @@
+// Multilinear KZG (MultilinearKzgBackend, backend/src/multilinear_kzg.rs:20-55)
+// over a device-resident prepared SRS.
+//
+//   commit(f)       = MSM(layer[N - nv], f)                (cpu/multilinear_kzg.rs:40-59)
+//   multi_commit    = independent commits, order and duplicates kept; all MSMs
+//                     are queued back to back and synchronised once
+//                     (the reference bridge syncs after each, bridge_kzg.cpp:555-575)
+//   open(f, x)      = for j < nv: q[b] = f[2b+1] - f[2b], f[b] = f[2b] + x_j q[b],
+//                     pi_j = MSM(layer[N - nv + 1 + j], q)  (cpu/multilinear_kzg.rs:79-137)
+// Prepared layers hold window-shifted copies of every point (see msm.hip).
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include "kernels_common.hpp"
+#include "mle.hpp"
+#include "msm.hpp"
+
+namespace hp {
+namespace kzg {
+
+struct Params {
+  size_t num_vars = 0;
+  std::vector<msm::Layer> layers;  // N + 1
+  std::vector<bool> ready;
+  uint32_t window_override = 0;    // 0 = plan by size
+};
+
+namespace {
+
+__global__ void k_fold_quotient(const Fr* src, Fr* dst, Fr* q, const Fr* x, size_t half) {
+  const Fr c = load_fr(x, 0);
+  for (size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x; b < half;
+       b += (size_t)gridDim.x * blockDim.x) {
+    const Fr lo = load_fr(src, 2 * b), hi = load_fr(src, 2 * b + 1);
+    const Fr d = hi - lo;
+    store_fr(q, b, d);
+    store_fr(dst, b, lo + c * d);
+  }
+}
+
+unsigned grid_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, 65535)));
+}
+
+}  // namespace
+}  // namespace kzg
+}  // namespace hp
+
+struct HpMi355xKzgParams {
+  HpMi355xContext* ctx;
+  std::shared_ptr<hp::kzg::Params> impl;
+};
+
+using namespace hp;
+
+namespace {
+
+kzg::Params* bind(HpMi355xContext* ctx, rt::Context& c, const HpMi355xKzgParams* params) {
+  if (params == nullptr || params->ctx != ctx || !c.resources.count(params)) return nullptr;
+  return params->impl.get();
+}
+
+void finish_points(rt::Context& c, const G1Xyzz* dev, size_t count, uint8_t* out97) {
+  std::vector<G1Xyzz> host(count);
+  rt::download_bytes(c, dev, count * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
+  for (size_t i = 0; i < count; ++i) msm::to_packet(host[i], out97 + rt::kG1Bytes * i);
+}
+
+}  // namespace
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_create(HpMi355xContext* ctx,
+                                                                     size_t num_vars,
+                                                                     HpMi355xKzgParams** out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr || num_vars > 30) return rt::invalid_parameters();
+    auto p = std::make_shared<kzg::Params>();
+    p->num_vars = num_vars;
+    p->layers.resize(num_vars + 1);
+    p->ready.assign(num_vars + 1, false);
+    if (const char* env = std::getenv("HP_MI355X_MSM_WINDOW")) {
+      const int v = std::atoi(env);
+      if (v >= 4 && v <= 20) p->window_override = static_cast<uint32_t>(v);
+    }
+    auto* handle = new HpMi355xKzgParams{ctx, p};
+    c.resources[handle] = p;
+    *out = handle;
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_upload_layer(
+    HpMi355xContext* ctx, HpMi355xKzgParams* params, size_t layer_index, const uint8_t* points,
+    size_t point_count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || points == nullptr || layer_index > p->num_vars) return rt::invalid_parameters();
+    if (point_count != (size_t(1) << (p->num_vars - layer_index))) return rt::invalid_parameters();
+    msm::Layer layer;
+    if (!msm::upload_layer(c, layer, points, point_count, p->window_override))
+      return rt::invalid_parameters();
+    c.sync();
+    p->layers[layer_index] = std::move(layer);
+    p->ready[layer_index] = true;
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_generate(HpMi355xContext* ctx,
+                                                                       HpMi355xKzgParams* params,
+                                                                       const uint8_t* t) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p) return rt::invalid_parameters();
+    const size_t N = p->num_vars;
+    if (N > 0 && (t == nullptr || !rt::canonical_fr_array(t, N))) return rt::invalid_parameters();
+    rt::BufferRef tr = c.allocate(std::max<size_t>(N, 1) * rt::kFrBytes);
+    if (N) rt::upload_fr_values(c, static_cast<Fr*>(tr->ptr), t, N);
+    for (size_t i = 0; i <= N; ++i) {
+      const size_t len = N - i, n = size_t(1) << len;
+      rt::BufferRef scal = c.allocate(n * rt::kFrBytes);
+      if (len == 0) {
+        uint8_t one[32] = {1};
+        rt::upload_fr_values(c, static_cast<Fr*>(scal->ptr), one, 1);
+      } else {
+        mle::build_eq(c, static_cast<const Fr*>(tr->ptr) + i, len, static_cast<Fr*>(scal->ptr));
+      }
+      rt::BufferRef pts = c.allocate(n * sizeof(G1Affine));
+      msm::generate_points(c, static_cast<const Fr*>(scal->ptr), n,
+                           static_cast<G1Affine*>(pts->ptr));
+      msm::Layer layer;
+      msm::prepare_layer_from_device(c, layer, static_cast<const G1Affine*>(pts->ptr), n,
+                                     p->window_override);
+      p->layers[i] = std::move(layer);
+      p->ready[i] = true;
+    }
+    c.sync();
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer(
+    HpMi355xContext* ctx, const HpMi355xKzgParams* params, size_t layer_index, uint8_t* points,
+    size_t point_count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || points == nullptr || layer_index > p->num_vars) return rt::invalid_parameters();
+    if (point_count != (size_t(1) << (p->num_vars - layer_index))) return rt::invalid_parameters();
+    if (!p->ready[layer_index]) return rt::invalid_state();
+    msm::download_layer(c, p->layers[layer_index], points);
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_destroy(HpMi355xContext* ctx,
+                                                                      HpMi355xKzgParams* params) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (!bind(ctx, c, params)) return rt::invalid_parameters();
+    c.resources.erase(params);
+    delete params;
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_commit(HpMi355xContext* ctx,
+                                                              const HpMi355xKzgParams* params,
+                                                              HpMi355xDeviceAddress polynomial,
+                                                              size_t num_vars,
+                                                              uint8_t* out_commitment) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || out_commitment == nullptr) return rt::invalid_parameters();
+    if (num_vars > p->num_vars) return rt::invalid_parameters();
+    rt::BufferRef f = c.lookup(polynomial, num_vars);
+    if (!f) return rt::invalid_parameters();
+    const size_t layer = p->num_vars - num_vars;
+    if (!p->ready[layer]) return rt::invalid_state();
+    rt::BufferRef res = c.allocate(sizeof(G1Xyzz));
+    msm::enqueue(c, p->layers[layer], static_cast<const Fr*>(f->ptr),
+                 static_cast<G1Xyzz*>(res->ptr));
+    finish_points(c, static_cast<const G1Xyzz*>(res->ptr), 1, out_commitment);
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_multi_commit(
+    HpMi355xContext* ctx, const HpMi355xKzgParams* params, const HpMi355xDeviceAddress* polynomials,
+    const size_t* num_vars, size_t count, uint8_t* out_commitments) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p) return rt::invalid_parameters();
+    if (count == 0) return rt::ok();
+    if (polynomials == nullptr || num_vars == nullptr || out_commitments == nullptr)
+      return rt::invalid_parameters();
+    // validate the whole batch before any side effect
+    std::vector<rt::BufferRef> tables(count);
+    for (size_t i = 0; i < count; ++i) {
+      if (num_vars[i] > p->num_vars) return rt::invalid_parameters();
+      tables[i] = c.lookup(polynomials[i], num_vars[i]);
+      if (!tables[i]) return rt::invalid_parameters();
+    }
+    for (size_t i = 0; i < count; ++i)
+      if (!p->ready[p->num_vars - num_vars[i]]) return rt::invalid_state();
+    rt::BufferRef res = c.allocate(count * sizeof(G1Xyzz));
+    G1Xyzz* r = static_cast<G1Xyzz*>(res->ptr);
+    for (size_t i = 0; i < count; ++i)
+      msm::enqueue(c, p->layers[p->num_vars - num_vars[i]],
+                   static_cast<const Fr*>(tables[i]->ptr), r + i);
+    finish_points(c, r, count, out_commitments);
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
+    HpMi355xContext* ctx, const HpMi355xKzgParams* params, HpMi355xDeviceAddress polynomial,
+    size_t num_vars, const uint8_t* point, uint8_t* out_quotients, uint8_t* out_evaluation) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || out_evaluation == nullptr) return rt::invalid_parameters();
+    if (num_vars > p->num_vars) return rt::invalid_parameters();
+    if (num_vars > 0 && (point == nullptr || out_quotients == nullptr))
+      return rt::invalid_parameters();
+    rt::BufferRef f = c.lookup(polynomial, num_vars);
+    if (!f) return rt::invalid_parameters();
+    if (num_vars && !rt::canonical_fr_array(point, num_vars)) return rt::invalid_parameters();
+    const size_t N = p->num_vars;
+    for (size_t j = 0; j < num_vars; ++j)
+      if (!p->ready[N - num_vars + 1 + j]) return rt::invalid_state();
+    if (num_vars == 0) {
+      rt::download_fr(c, static_cast<const Fr*>(f->ptr), 1, out_evaluation);
+      return rt::ok();
+    }
+    const size_t n = size_t(1) << num_vars;
+    rt::BufferRef dpoint = c.allocate(num_vars * rt::kFrBytes);
+    rt::upload_fr_values(c, static_cast<Fr*>(dpoint->ptr), point, num_vars);
+    rt::BufferRef work = c.allocate((n / 2 + n / 4 + n / 2 + 1) * rt::kFrBytes);
+    Fr* fa = static_cast<Fr*>(work->ptr);
+    Fr* fb = fa + n / 2;
+    Fr* q = fb + n / 4 + 1;
+    rt::BufferRef res = c.allocate(num_vars * sizeof(G1Xyzz));
+    G1Xyzz* r = static_cast<G1Xyzz*>(res->ptr);
+    const Fr* src = static_cast<const Fr*>(f->ptr);
+    for (size_t j = 0; j < num_vars; ++j) {
+      const size_t half = n >> (j + 1);
+      Fr* dst = (j & 1) ? fb : fa;
+      hipLaunchKernelGGL(kzg::k_fold_quotient, dim3(kzg::grid_for(half)), dim3(256), 0, c.stream,
+                         src, dst, q, static_cast<const Fr*>(dpoint->ptr) + j, half);
+      rt::check_launch();
+      msm::enqueue(c, p->layers[N - num_vars + 1 + j], q, r + j);
+      src = dst;
+    }
+    finish_points(c, r, num_vars, out_quotients);
+    rt::download_fr(c, src, 1, out_evaluation);
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_msm_profile(HpMi355xContext* ctx,
+                                                   const HpMi355xKzgParams* params,
+                                                   HpMi355xDeviceAddress polynomial,
+                                                   size_t num_vars, float* stage_ms,
+                                                   size_t stage_count, uint32_t* plan_info) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || stage_ms == nullptr || stage_count < 7 || plan_info == nullptr)
+      return rt::invalid_parameters();
+    if (num_vars > p->num_vars) return rt::invalid_parameters();
+    rt::BufferRef f = c.lookup(polynomial, num_vars);
+    if (!f) return rt::invalid_parameters();
+    const size_t layer = p->num_vars - num_vars;
+    if (!p->ready[layer]) return rt::invalid_state();
+    msm::StageEvents ev;
+    for (auto& e : ev.marks) rt::hip_check(hipEventCreate(&e));
+    rt::BufferRef res = c.allocate(sizeof(G1Xyzz) + sizeof(uint32_t));
+    ev.entries_out = reinterpret_cast<uint32_t*>(static_cast<G1Xyzz*>(res->ptr) + 1);
+    msm::enqueue(c, p->layers[layer], static_cast<const Fr*>(f->ptr),
+                 static_cast<G1Xyzz*>(res->ptr), &ev);
+    c.sync();
+    for (int i = 0; i < 6; ++i)
+      rt::hip_check(hipEventElapsedTime(&stage_ms[i], ev.marks[i], ev.marks[i + 1]));
+    rt::hip_check(hipEventElapsedTime(&stage_ms[6], ev.marks[0], ev.marks[6]));
+    for (auto& e : ev.marks) (void)hipEventDestroy(e);
+    uint32_t entries = 0;
+    rt::download_bytes(c, ev.entries_out, sizeof(uint32_t), reinterpret_cast<uint8_t*>(&entries));
+    const msm::Plan& plan = p->layers[layer].plan;
+    plan_info[0] = plan.c;
+    plan_info[1] = plan.windows;
+    plan_info[2] = plan.buckets;
+    plan_info[3] = entries;
+    return rt::ok();
+  });
+}

@@ -1,0 +1,563 @@
+// MLE operations of MleBackend (backend/src/multilinear_polynomial.rs:33-121)
+// as streaming HIP kernels over HBM-resident Montgomery Fr tables.
+//
+// Semantics follow the reference CPU backend
+// (backend/src/cpu/multilinear_polynomial.rs:29-254, backend/src/common/eq.rs);
+// validation order and status kinds follow the reference bridge
+// (backend/src/ascend910b3/native/bridge_mle.cpp:173-733).  Table order is
+// low-bit-first: index bit i <-> variable i <-> point[i].
+#include <algorithm>
+#include <vector>
+
+#include "kernels_common.hpp"
+#include "mle.hpp"
+
+namespace hp {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+unsigned grid(size_t n) {
+  size_t b = (n + kThreads - 1) / kThreads;
+  b = std::min<size_t>(std::max<size_t>(b, 1), 65535);
+  return static_cast<unsigned>(b);
+}
+
+#define GRID_STRIDE(i, n)                                                        \
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); \
+       i += (size_t)gridDim.x * blockDim.x)
+
+// out[j] = sum_k coeff[k] * tables[k][j]
+__global__ void k_linear_combination(const Fr* const* tables, const Fr* coeffs, size_t count,
+                                     Fr* out, size_t n) {
+  GRID_STRIDE(j, n) {
+    Fr acc = Fr::zero();
+    for (size_t k = 0; k < count; ++k) acc = acc + load_fr(coeffs, k) * load_fr(tables[k], j);
+    store_fr(out, j, acc);
+  }
+}
+
+}  // namespace
+
+// ---- fold: dst[b] = src[2b] + r (src[2b+1] - src[2b]) ---------------------
+__global__ void k_fold(const Fr* src, Fr* dst, const Fr* r, size_t half) {
+  const Fr c = load_fr(r, 0);
+  GRID_STRIDE(b, half) {
+    const Fr lo = load_fr(src, 2 * b);
+    const Fr hi = load_fr(src, 2 * b + 1);
+    store_fr(dst, b, lo + c * (hi - lo));
+  }
+}
+
+// Fold the remaining `levels` variables of a table of 2^levels <= 1024 entries
+// inside one block (LDS), writing the single result to out[0].
+__global__ void k_fold_tail(const Fr* src, const Fr* point, size_t levels, Fr* out) {
+  __shared__ Fr buf[1024];
+  const size_t n = size_t(1) << levels;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) buf[i] = load_fr(src, i);
+  __syncthreads();
+  size_t len = n;
+  for (size_t l = 0; l < levels; ++l) {
+    const Fr c = load_fr(point, l);
+    const size_t half = len / 2;
+    Fr v[4];
+    size_t cnt = 0;
+    for (size_t b = threadIdx.x; b < half; b += blockDim.x) {
+      const Fr lo = buf[2 * b], hi = buf[2 * b + 1];
+      v[cnt++] = lo + c * (hi - lo);
+    }
+    __syncthreads();
+    cnt = 0;
+    for (size_t b = threadIdx.x; b < half; b += blockDim.x) buf[b] = v[cnt++];
+    __syncthreads();
+    len = half;
+  }
+  if (threadIdx.x == 0) store_fr(out, 0, buf[0]);
+}
+
+namespace {
+
+// eq table of one chunk of k <= 10 coordinates r[0..k):
+// t[x] = prod_{i<k} (bit_i(x) ? r_i : 1 - r_i), built in LDS by prepending one
+// coordinate per level from the highest to the lowest (eq.rs:67-95 order).
+__global__ void k_eq_chunk(const Fr* r, size_t k, Fr* table) {
+  __shared__ Fr buf[1024];
+  if (threadIdx.x == 0) buf[0] = Fr::one();
+  __syncthreads();
+  size_t len = 1;
+  for (size_t step = 0; step < k; ++step) {
+    const Fr ri = load_fr(r, k - 1 - step);
+    Fr even[2], odd[2];
+    size_t cnt = 0;
+    for (size_t b = threadIdx.x; b < len; b += blockDim.x) {
+      const Fr v = buf[b];
+      const Fr t = ri * v;
+      even[cnt] = v - t;
+      odd[cnt] = t;
+      ++cnt;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (size_t b = threadIdx.x; b < len; b += blockDim.x) {
+      buf[2 * b] = even[cnt];
+      buf[2 * b + 1] = odd[cnt];
+      ++cnt;
+    }
+    __syncthreads();
+    len *= 2;
+  }
+  for (size_t i = threadIdx.x; i < len; i += blockDim.x) store_fr(table, i, buf[i]);
+}
+
+struct EqChunks {
+  const Fr* table[4];
+  uint32_t shift[4];
+  uint32_t bits[4];
+  uint32_t count;
+};
+
+// out[x] = prod_c chunk_c[(x >> shift_c) & mask_c]
+__global__ void k_eq_outer(EqChunks chunks, Fr* out, size_t n) {
+  GRID_STRIDE(x, n) {
+    Fr v = load_fr(chunks.table[0], x & ((size_t(1) << chunks.bits[0]) - 1));
+    for (uint32_t c = 1; c < chunks.count; ++c)
+      v = v * load_fr(chunks.table[c], (x >> chunks.shift[c]) & ((size_t(1) << chunks.bits[c]) - 1));
+    store_fr(out, x, v);
+  }
+}
+
+__global__ void k_zero_u32(unsigned* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// Per-thread Montgomery batch inversion of prod_k g_k over a strided set of
+// entries; out = prod_k f_k / prod_k g_k.  A zero denominator raises *zero.
+__global__ void k_frac(const Fr* const* nums, const Fr* const* dens, size_t count, Fr* out,
+                       size_t n, unsigned* zero) {
+  const size_t threads = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  // pass 1: out[j] := prefix product of denominators (exclusive), acc = total
+  Fr acc = Fr::one();
+  bool saw_zero = false;
+  for (size_t j = tid; j < n; j += threads) {
+    Fr d = load_fr(dens[0], j);
+    for (size_t k = 1; k < count; ++k) d = d * load_fr(dens[k], j);
+    saw_zero |= d.is_zero();
+    store_fr(out, j, acc);
+    acc = acc * d;
+  }
+  if (saw_zero) atomicOr(zero, 1u);
+  if (tid >= n) return;
+  Fr inv = inverse(acc);
+  // pass 2 (backwards): inv_j = inv * prefix_j; inv *= d_j
+  size_t last = tid + ((n - 1 - tid) / threads) * threads;
+  for (size_t j = last;; j -= threads) {
+    Fr d = load_fr(dens[0], j);
+    for (size_t k = 1; k < count; ++k) d = d * load_fr(dens[k], j);
+    Fr num = load_fr(nums[0], j);
+    for (size_t k = 1; k < count; ++k) num = num * load_fr(nums[k], j);
+    const Fr dinv = inv * load_fr(out, j);
+    inv = inv * d;
+    store_fr(out, j, num * dinv);
+    if (j < threads) break;
+  }
+}
+
+// Product tree level: dst[i] = src[2i] * src[2i+1].
+__global__ void k_pair_products(const Fr* src, Fr* dst, size_t count) {
+  GRID_STRIDE(i, count) { store_fr(dst, i, load_fr(src, 2 * i) * load_fr(src, 2 * i + 1)); }
+}
+
+// Remaining product-tree levels once a level has <= 1024 entries, in one block.
+// `level` points at the current level (len entries); results follow it in place.
+__global__ void k_product_tail(Fr* level, size_t len) {
+  while (len > 1) {
+    const size_t half = len / 2;
+    for (size_t i = threadIdx.x; i < half; i += blockDim.x)
+      store_fr(level + len, i, load_fr(level, 2 * i) * load_fr(level, 2 * i + 1));
+    __syncthreads();
+    level += len;
+    len = half;
+  }
+  if (threadIdx.x == 0) store_fr(level, 1, Fr::zero());  // prod(1,...,1) := 0
+}
+
+// num_c[j] = f_c[j] + beta (c 2^n + j) + gamma ; den_c[j] = g_c[j] + beta perm_c[j] + gamma
+__global__ void k_nums_denoms(const Fr* const* fxs, const Fr* const* gxs, const Fr* const* perms,
+                              const Fr* beta_gamma, Fr* const* nums, Fr* const* dens,
+                              size_t count, size_t n) {
+  const Fr beta = load_fr(beta_gamma, 0);
+  const Fr gamma = load_fr(beta_gamma, 1);
+  GRID_STRIDE(j, n * count) {
+    const size_t c = j / n, row = j % n;
+    const Fr id = from_u64<FrParams>((uint64_t)(c * n + row));
+    store_fr(nums[c], row, load_fr(fxs[c], row) + beta * id + gamma);
+    store_fr(dens[c], row, load_fr(gxs[c], row) + beta * load_fr(perms[c], row) + gamma);
+  }
+}
+
+// p1 = [frac even | prod even], p2 = [frac odd | prod odd]
+__global__ void k_product_factors(const Fr* frac, const Fr* prod, Fr* p1, Fr* p2, size_t half) {
+  GRID_STRIDE(i, 2 * half) {
+    const Fr* src = i < half ? frac : prod;
+    const size_t k = i < half ? i : i - half;
+    store_fr(p1, i, load_fr(src, 2 * k));
+    store_fr(p2, i, load_fr(src, 2 * k + 1));
+  }
+}
+
+}  // namespace
+
+namespace mle {
+
+void fold(rt::Context& c, const Fr* src, Fr* dst, const Fr* r, size_t half) {
+  hipLaunchKernelGGL(k_fold, dim3(grid(half)), dim3(kThreads), 0, c.stream, src, dst, r, half);
+  rt::check_launch();
+}
+
+// Evaluate a table of 2^nv entries at a device-resident point; result in out[0].
+// Folds ping-pong between two scratch tables (a fold cannot run in place on
+// the device: thread b would overwrite entry b while thread b/2 still reads it).
+void evaluate(rt::Context& c, const Fr* table, size_t nv, const Fr* point, Fr* out) {
+  if (nv == 0) {
+    rt::hip_check(hipMemcpyAsync(out, table, rt::kFrBytes, hipMemcpyDeviceToDevice, c.stream));
+    return;
+  }
+  const Fr* src = table;
+  size_t level = 0;
+  rt::BufferRef scratch;
+  if (nv > 10) {
+    const size_t a = size_t(1) << (nv - 1), b = size_t(1) << (nv - 2);
+    scratch = c.allocate((a + b) * rt::kFrBytes);
+    Fr* bufs[2] = {static_cast<Fr*>(scratch->ptr), static_cast<Fr*>(scratch->ptr) + a};
+    while (nv - level > 10) {
+      const size_t half = size_t(1) << (nv - level - 1);
+      Fr* dst = bufs[level & 1];
+      fold(c, src, dst, point + level, half);
+      src = dst;
+      ++level;
+    }
+  }
+  hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(256), 0, c.stream, src, point + level, nv - level,
+                     out);
+  rt::check_launch();
+}
+
+void build_eq(rt::Context& c, const Fr* r, size_t nv, Fr* out) {
+  // split the coordinates into <= 4 chunks of <= 10 bits, low bits first
+  const size_t chunks = (nv + 9) / 10;
+  EqChunks eq{};
+  eq.count = static_cast<uint32_t>(chunks);
+  size_t off = 0, total = 0;
+  size_t bits[4];
+  for (size_t k = 0; k < chunks; ++k) {
+    bits[k] = (nv - off + (chunks - k) - 1) / (chunks - k);  // balanced split
+    total += size_t(1) << bits[k];
+    off += bits[k];
+  }
+  rt::BufferRef tables = c.allocate(total * rt::kFrBytes);
+  Fr* t = static_cast<Fr*>(tables->ptr);
+  off = 0;
+  for (size_t k = 0; k < chunks; ++k) {
+    hipLaunchKernelGGL(k_eq_chunk, dim3(1), dim3(256), 0, c.stream, r + off, bits[k], t);
+    rt::check_launch();
+    eq.table[k] = t;
+    eq.shift[k] = static_cast<uint32_t>(off);
+    eq.bits[k] = static_cast<uint32_t>(bits[k]);
+    t += size_t(1) << bits[k];
+    off += bits[k];
+  }
+  const size_t n = size_t(1) << nv;
+  hipLaunchKernelGGL(k_eq_outer, dim3(grid(n)), dim3(kThreads), 0, c.stream, eq, out, n);
+  rt::check_launch();
+}
+
+}  // namespace mle
+}  // namespace hp
+
+// ============================ C ABI =========================================
+using namespace hp;
+
+namespace {
+
+bool dims_ok(size_t nv) { return nv < rt::kMaxNumVars; }
+
+// Upload a small array of canonical Fr values (already validated).
+rt::BufferRef upload_small(rt::Context& c, const uint8_t* bytes, size_t count) {
+  rt::BufferRef b = c.allocate(std::max<size_t>(count, 1) * rt::kFrBytes);
+  if (count) rt::upload_fr_values(c, static_cast<Fr*>(b->ptr), bytes, count);
+  return b;
+}
+
+rt::BufferRef upload_pointers(rt::Context& c, const std::vector<const void*>& ptrs) {
+  rt::BufferRef b = c.allocate(std::max<size_t>(ptrs.size(), 1) * sizeof(void*));
+  if (!ptrs.empty()) {
+    uint8_t* stage = c.staging(ptrs.size() * sizeof(void*));
+    std::memcpy(stage, ptrs.data(), ptrs.size() * sizeof(void*));
+    rt::hip_check(hipMemcpyAsync(b->ptr, stage, ptrs.size() * sizeof(void*),
+                                 hipMemcpyHostToDevice, c.stream));
+    c.staging_used();
+  }
+  return b;
+}
+
+}  // namespace
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_upload(HpMi355xContext* ctx, size_t num_vars,
+                                                  const uint8_t* evaluations,
+                                                  size_t evaluation_count,
+                                                  HpMi355xDeviceAddress* out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr || !dims_ok(num_vars)) return rt::invalid_parameters();
+    if (evaluation_count != (size_t(1) << num_vars) || evaluations == nullptr)
+      return rt::invalid_parameters();
+    if (!rt::canonical_fr_array(evaluations, evaluation_count)) return rt::invalid_parameters();
+    rt::BufferRef buf = rt::upload_fr_table(c, evaluations, evaluation_count, num_vars);
+    c.sync();  // host buffer is borrowed only until return
+    *out = c.publish(std::move(buf));
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_download(HpMi355xContext* ctx,
+                                                    HpMi355xDeviceAddress polynomial,
+                                                    size_t num_vars, uint8_t* out_evaluations,
+                                                    size_t evaluation_count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out_evaluations == nullptr || !dims_ok(num_vars)) return rt::invalid_parameters();
+    if (evaluation_count != (size_t(1) << num_vars)) return rt::invalid_parameters();
+    rt::BufferRef buf = c.lookup(polynomial, num_vars);
+    if (!buf) return rt::invalid_parameters();
+    rt::download_fr(c, static_cast<const Fr*>(buf->ptr), evaluation_count, out_evaluations);
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_linear_combination(
+    HpMi355xContext* ctx, size_t num_vars, const HpMi355xDeviceAddress* polynomials,
+    const uint8_t* coefficients, size_t count, HpMi355xDeviceAddress* out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr || polynomials == nullptr || coefficients == nullptr || count == 0)
+      return rt::invalid_parameters();
+    if (!dims_ok(num_vars)) return rt::invalid_parameters();
+    std::vector<rt::BufferRef> inputs;
+    for (size_t k = 0; k < count; ++k) {
+      rt::BufferRef b = c.lookup(polynomials[k], num_vars);
+      if (!b) return rt::invalid_parameters();
+      inputs.push_back(b);
+    }
+    if (!rt::canonical_fr_array(coefficients, count)) return rt::invalid_parameters();
+    // zero coefficients contribute nothing (cpu/multilinear_polynomial.rs:72-74)
+    std::vector<const void*> ptrs;
+    std::vector<uint8_t> coeffs;
+    for (size_t k = 0; k < count; ++k) {
+      const uint8_t* cb = coefficients + rt::kFrBytes * k;
+      if (std::all_of(cb, cb + rt::kFrBytes, [](uint8_t v) { return v == 0; })) continue;
+      ptrs.push_back(inputs[k]->ptr);
+      coeffs.insert(coeffs.end(), cb, cb + rt::kFrBytes);
+    }
+    const size_t n = size_t(1) << num_vars;
+    rt::BufferRef result = c.allocate(n * rt::kFrBytes, num_vars);
+    rt::BufferRef dptrs = upload_pointers(c, ptrs);
+    rt::BufferRef dcoef = upload_small(c, coeffs.data(), ptrs.size());
+    hipLaunchKernelGGL(k_linear_combination, dim3(grid(n)), dim3(kThreads), 0, c.stream,
+                       static_cast<const Fr* const*>(dptrs->ptr),
+                       static_cast<const Fr*>(dcoef->ptr), ptrs.size(),
+                       static_cast<Fr*>(result->ptr), n);
+    rt::check_launch();
+    *out = c.publish(std::move(result));
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate(HpMi355xContext* ctx,
+                                                    HpMi355xDeviceAddress polynomial,
+                                                    size_t num_vars, const uint8_t* point,
+                                                    uint8_t* out_evaluation) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out_evaluation == nullptr) return rt::invalid_parameters();
+    if (point == nullptr && num_vars != 0) return rt::invalid_parameters();
+    if (!dims_ok(num_vars)) return rt::invalid_parameters();
+    rt::BufferRef table = c.lookup(polynomial, num_vars);
+    if (!table) return rt::invalid_parameters();
+    if (num_vars && !rt::canonical_fr_array(point, num_vars)) return rt::invalid_parameters();
+    rt::BufferRef dpoint = upload_small(c, point, num_vars);
+    rt::BufferRef result = c.allocate(rt::kFrBytes);
+    mle::evaluate(c, static_cast<const Fr*>(table->ptr), num_vars,
+                  static_cast<const Fr*>(dpoint->ptr), static_cast<Fr*>(result->ptr));
+    rt::download_fr(c, static_cast<const Fr*>(result->ptr), 1, out_evaluation);
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_build_eq_x_r(HpMi355xContext* ctx, const uint8_t* r,
+                                                        size_t num_vars,
+                                                        HpMi355xDeviceAddress* out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr || r == nullptr) return rt::invalid_parameters();
+    if (num_vars == 0 || !dims_ok(num_vars)) return rt::invalid_parameters();
+    if (!rt::canonical_fr_array(r, num_vars)) return rt::invalid_parameters();
+    rt::BufferRef dr = upload_small(c, r, num_vars);
+    rt::BufferRef result = c.allocate((size_t(1) << num_vars) * rt::kFrBytes, num_vars);
+    mle::build_eq(c, static_cast<const Fr*>(dr->ptr), num_vars, static_cast<Fr*>(result->ptr));
+    *out = c.publish(std::move(result));
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_frac_poly(
+    HpMi355xContext* ctx, size_t num_vars, const HpMi355xDeviceAddress* numerators,
+    const HpMi355xDeviceAddress* denominators, size_t count, HpMi355xDeviceAddress* out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr || numerators == nullptr || denominators == nullptr || count == 0)
+      return rt::invalid_parameters();
+    if (!dims_ok(num_vars)) return rt::invalid_parameters();
+    std::vector<rt::BufferRef> keep;
+    std::vector<const void*> nums, dens;
+    for (size_t k = 0; k < count; ++k) {
+      rt::BufferRef f = c.lookup(numerators[k], num_vars);
+      rt::BufferRef g = c.lookup(denominators[k], num_vars);
+      if (!f || !g) return rt::invalid_parameters();
+      nums.push_back(f->ptr);
+      dens.push_back(g->ptr);
+      keep.push_back(f);
+      keep.push_back(g);
+    }
+    const size_t n = size_t(1) << num_vars;
+    rt::BufferRef result = c.allocate(n * rt::kFrBytes, num_vars);
+    rt::BufferRef dn = upload_pointers(c, nums);
+    rt::BufferRef dd = upload_pointers(c, dens);
+    rt::BufferRef flag = c.allocate(sizeof(unsigned));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.stream, static_cast<unsigned*>(flag->ptr));
+    rt::check_launch();
+    // ~16 entries per thread amortise the one Fermat inversion per thread
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(n / 16 + 1, 65536));
+    hipLaunchKernelGGL(k_frac, dim3(rt::blocks_for(threads, 64)), dim3(64), 0, c.stream,
+                       static_cast<const Fr* const*>(dn->ptr),
+                       static_cast<const Fr* const*>(dd->ptr), count,
+                       static_cast<Fr*>(result->ptr), n, static_cast<unsigned*>(flag->ptr));
+    rt::check_launch();
+    unsigned zero = 0;
+    rt::download_bytes(c, flag->ptr, sizeof(unsigned), reinterpret_cast<uint8_t*>(&zero));
+    if (zero) return rt::invalid_parameters();  // result buffer is released unpublished
+    *out = c.publish(std::move(result));
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_product_poly(HpMi355xContext* ctx,
+                                                                HpMi355xDeviceAddress frac,
+                                                                size_t num_vars,
+                                                                HpMi355xDeviceAddress* out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr) return rt::invalid_parameters();
+    if (num_vars == 0 || !dims_ok(num_vars)) return rt::invalid_parameters();
+    rt::BufferRef f = c.lookup(frac, num_vars);
+    if (!f) return rt::invalid_parameters();
+    const size_t n = size_t(1) << num_vars;
+    rt::BufferRef result = c.allocate(n * rt::kFrBytes, num_vars);
+    Fr* o = static_cast<Fr*>(result->ptr);
+    // level 1 from frac, then level by level until <= 1024 entries remain
+    size_t len = n / 2;
+    hipLaunchKernelGGL(k_pair_products, dim3(grid(len)), dim3(kThreads), 0, c.stream,
+                       static_cast<const Fr*>(f->ptr), o, len);
+    rt::check_launch();
+    Fr* level = o;
+    while (len > 1024) {
+      hipLaunchKernelGGL(k_pair_products, dim3(grid(len / 2)), dim3(kThreads), 0, c.stream, level,
+                         level + len, len / 2);
+      rt::check_launch();
+      level += len;
+      len /= 2;
+    }
+    hipLaunchKernelGGL(k_product_tail, dim3(1), dim3(256), 0, c.stream, level, len);
+    rt::check_launch();
+    *out = c.publish(std::move(result));
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_nums_and_denoms(
+    HpMi355xContext* ctx, size_t num_vars, const uint8_t* beta, const uint8_t* gamma,
+    const HpMi355xDeviceAddress* fxs, const HpMi355xDeviceAddress* gxs,
+    const HpMi355xDeviceAddress* permutations, size_t count,
+    HpMi355xDeviceAddress* out_numerators, HpMi355xDeviceAddress* out_denominators) {
+  if (out_numerators && out_denominators)
+    for (size_t i = 0; i < count; ++i) out_numerators[i] = out_denominators[i] = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (beta == nullptr || gamma == nullptr || fxs == nullptr || gxs == nullptr ||
+        permutations == nullptr || out_numerators == nullptr || out_denominators == nullptr ||
+        count == 0)
+      return rt::invalid_parameters();
+    if (!dims_ok(num_vars)) return rt::invalid_parameters();
+    std::vector<rt::BufferRef> keep;
+    std::vector<const void*> f, g, p;
+    for (size_t k = 0; k < count; ++k) {
+      rt::BufferRef a = c.lookup(fxs[k], num_vars), b = c.lookup(gxs[k], num_vars),
+                    d = c.lookup(permutations[k], num_vars);
+      if (!a || !b || !d) return rt::invalid_parameters();
+      f.push_back(a->ptr);
+      g.push_back(b->ptr);
+      p.push_back(d->ptr);
+      keep.insert(keep.end(), {a, b, d});
+    }
+    if (!rt::canonical_fr(beta) || !rt::canonical_fr(gamma)) return rt::invalid_parameters();
+    if (count > (size_t(1) << 20)) return rt::invalid_parameters();
+    const size_t n = size_t(1) << num_vars;
+    std::vector<rt::BufferRef> nums, dens;
+    std::vector<const void*> np, dp;
+    for (size_t k = 0; k < count; ++k) {
+      nums.push_back(c.allocate(n * rt::kFrBytes, num_vars));
+      dens.push_back(c.allocate(n * rt::kFrBytes, num_vars));
+      np.push_back(nums.back()->ptr);
+      dp.push_back(dens.back()->ptr);
+    }
+    uint8_t bg[64];
+    std::memcpy(bg, beta, 32);
+    std::memcpy(bg + 32, gamma, 32);
+    rt::BufferRef dbg = upload_small(c, bg, 2);
+    rt::BufferRef df = upload_pointers(c, f), dgp = upload_pointers(c, g),
+                  dpp = upload_pointers(c, p), dn = upload_pointers(c, np),
+                  dd = upload_pointers(c, dp);
+    hipLaunchKernelGGL(k_nums_denoms, dim3(grid(n * count)), dim3(kThreads), 0, c.stream,
+                       static_cast<const Fr* const*>(df->ptr),
+                       static_cast<const Fr* const*>(dgp->ptr),
+                       static_cast<const Fr* const*>(dpp->ptr), static_cast<const Fr*>(dbg->ptr),
+                       static_cast<Fr* const*>(dn->ptr), static_cast<Fr* const*>(dd->ptr), count,
+                       n);
+    rt::check_launch();
+    for (size_t k = 0; k < count; ++k) {
+      out_numerators[k] = c.publish(nums[k]);
+      out_denominators[k] = c.publish(dens[k]);
+    }
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_product_factors(
+    HpMi355xContext* ctx, HpMi355xDeviceAddress frac, HpMi355xDeviceAddress product,
+    size_t num_vars, HpMi355xDeviceAddress* out_p1, HpMi355xDeviceAddress* out_p2) {
+  if (out_p1) *out_p1 = nullptr;
+  if (out_p2) *out_p2 = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out_p1 == nullptr || out_p2 == nullptr) return rt::invalid_parameters();
+    if (num_vars == 0 || !dims_ok(num_vars)) return rt::invalid_parameters();
+    rt::BufferRef f = c.lookup(frac, num_vars), p = c.lookup(product, num_vars);
+    if (!f || !p) return rt::invalid_parameters();
+    const size_t n = size_t(1) << num_vars;
+    rt::BufferRef a = c.allocate(n * rt::kFrBytes, num_vars);
+    rt::BufferRef b = c.allocate(n * rt::kFrBytes, num_vars);
+    hipLaunchKernelGGL(k_product_factors, dim3(grid(n)), dim3(kThreads), 0, c.stream,
+                       static_cast<const Fr*>(f->ptr), static_cast<const Fr*>(p->ptr),
+                       static_cast<Fr*>(a->ptr), static_cast<Fr*>(b->ptr), n / 2);
+    rt::check_launch();
+    *out_p1 = c.publish(std::move(a));
+    *out_p2 = c.publish(std::move(b));
+    return rt::ok();
+  });
+}

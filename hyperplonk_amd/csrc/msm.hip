@@ -1,0 +1,689 @@
+// Pippenger G1 MSM on gfx950 against a prepared, window-shifted SRS layer.
+//
+// Stages (all on the context stream; the host only finishes the last
+// affine normalisation):
+//   1. digits    one thread per scalar: Montgomery -> canonical, signed c-bit
+//                digits for W windows; entry (bucket |d|-1, base index w*n+i,
+//                sign) per non-zero digit.
+//   2. sort      counting sort by bucket: histogram (atomics), exclusive scan,
+//                scatter of base references into bucket-contiguous order.
+//   3. tasks     each bucket is cut into tasks of <= task_cap entries.
+//   4. accumulate one wavefront per task: lanes stride over the entries with
+//                XYZZ += affine (8M + 2S), then a shuffle tree reduction.
+//   5. buckets   per-bucket sum of its task partials.
+//   6. reduce    sum_b (b + 1) S_b: blocks of 256 buckets run an LDS suffix
+//                scan (U_t = sum_{t' >= t} S_t') and a tree sum of U, giving
+//                (A, R) per block; higher levels combine (A, R) pairs with the
+//                same scan, scaled by the block span; one pair remains.
+// The group law is exact in every case, so the result does not depend on the
+// order of accumulation.  Work per 2^20 scalars at c = 16: W = 16 windows,
+// ~16.8M mixed additions, 2^15 buckets.
+//
+// The reference computes the same value with ark-ec's msm_unchecked
+// (backend/src/cpu/multilinear_kzg.rs:55,131); the reference accelerator
+// bridge uses unsigned windows with c = ell - ell/3 + 2
+// (backend/src/ascend910b3/native/msm_plan.cpp:119-123).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "kernels_common.hpp"
+#include "mle.hpp"
+#include "msm.hpp"
+
+namespace hp {
+namespace msm {
+namespace {
+
+constexpr uint32_t kSkip = 0xffffffffu;
+
+struct Task {
+  uint32_t start, len, bucket, pad;
+};
+
+__device__ __forceinline__ uint32_t scalar_bits(const uint32_t* s, uint32_t pos, uint32_t c) {
+  const uint32_t k = pos >> 5, sh = pos & 31;
+  uint64_t w = s[k];
+  if (k + 1 < 8) w |= (uint64_t)s[k + 1] << 32;
+  return (uint32_t)(w >> sh) & ((1u << c) - 1u);
+}
+
+__global__ void k_digits(const Fr* scalars, size_t n, uint32_t c, uint32_t windows,
+                         uint32_t* keys, uint32_t* vals) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const Fr s = from_mont(load_fr(scalars, i));
+    uint32_t carry = 0;
+    const uint32_t half = 1u << (c - 1), full = 1u << c;
+    for (uint32_t w = 0; w < windows; ++w) {
+      uint32_t raw = scalar_bits(s.v, w * c, c) + carry;
+      uint32_t key = kSkip, val = 0;
+      if (raw > half) {
+        carry = 1;
+        const uint32_t mag = full - raw;  // digit = raw - 2^c < 0
+        key = mag - 1;
+        val = (uint32_t)(w * n + i) | 0x80000000u;
+      } else {
+        carry = 0;
+        if (raw != 0) {
+          key = raw - 1;
+          val = (uint32_t)(w * n + i);
+        }
+      }
+      keys[w * n + i] = key;
+      vals[w * n + i] = val;
+    }
+  }
+}
+
+__global__ void k_histogram(const uint32_t* keys, size_t m, uint32_t* counts) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t k = keys[i];
+    if (k != kSkip) atomicAdd(&counts[k], 1u);
+  }
+}
+
+__global__ void k_scatter(const uint32_t* keys, const uint32_t* vals, size_t m, uint32_t* cursor,
+                          uint32_t* sorted) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t k = keys[i];
+    if (k != kSkip) sorted[atomicAdd(&cursor[k], 1u)] = vals[i];
+  }
+}
+
+// ---- exclusive scan of uint32 (up to 4M elements) -------------------------
+constexpr unsigned kScanThreads = 256, kScanPer = 4, kScanTile = kScanThreads * kScanPer;
+
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t s[kScanThreads];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (unsigned d = 1; d < kScanThreads; d <<= 1) {
+    uint32_t add = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += add;
+    __syncthreads();
+  }
+  const uint32_t incl = s[threadIdx.x];
+  *total = s[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+// out[i] = exclusive prefix within the tile; tile_sums[tile] = tile total.
+__global__ void k_scan_tiles(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tile_sums) {
+  const size_t base = blockIdx.x * (size_t)kScanTile + threadIdx.x * kScanPer;
+  uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+  for (unsigned j = 0; j < kScanPer; ++j) {
+    v[j] = base + j < n ? in[base + j] : 0;
+    sum += v[j];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, &total);
+#pragma unroll
+  for (unsigned j = 0; j < kScanPer; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of tile sums (<= kScanTile entries) in one block; writes the
+// grand total to tile_sums[tiles].
+__global__ void k_scan_top(uint32_t* tile_sums, size_t tiles) {
+  const size_t base = threadIdx.x * kScanPer;
+  uint32_t v[kScanPer], sum = 0;
+#pragma unroll
+  for (unsigned j = 0; j < kScanPer; ++j) {
+    v[j] = base + j < tiles ? tile_sums[base + j] : 0;
+    sum += v[j];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(sum, &total);
+#pragma unroll
+  for (unsigned j = 0; j < kScanPer; ++j) {
+    if (base + j < tiles) tile_sums[base + j] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == 0) tile_sums[tiles] = total;
+}
+
+__global__ void k_scan_add(uint32_t* out, size_t n, const uint32_t* tile_sums, uint32_t* total_out,
+                           size_t tiles) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] += tile_sums[i / kScanTile];
+  if (i == 0 && total_out) *total_out = tile_sums[tiles];
+}
+
+// ---- tasks ----------------------------------------------------------------
+__global__ void k_task_counts(const uint32_t* counts, uint32_t buckets, uint32_t cap,
+                              uint32_t* ntask) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
+       b += gridDim.x * blockDim.x)
+    ntask[b] = (counts[b] + cap - 1) / cap;
+}
+
+__global__ void k_build_tasks(const uint32_t* counts, const uint32_t* offsets,
+                              const uint32_t* task_off, uint32_t buckets, uint32_t cap,
+                              Task* tasks) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
+       b += gridDim.x * blockDim.x) {
+    const uint32_t cnt = counts[b], start = offsets[b];
+    uint32_t t = task_off[b];
+    for (uint32_t done = 0; done < cnt; done += cap, ++t)
+      tasks[t] = {start + done, min(cap, cnt - done), b, 0u};
+  }
+}
+
+__device__ __forceinline__ G1Xyzz shfl_down_xyzz(const G1Xyzz& v, unsigned off) {
+  G1Xyzz r;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    r.X.v[i] = __shfl_down(v.X.v[i], off, 64);
+    r.Y.v[i] = __shfl_down(v.Y.v[i], off, 64);
+    r.ZZ.v[i] = __shfl_down(v.ZZ.v[i], off, 64);
+    r.ZZZ.v[i] = __shfl_down(v.ZZZ.v[i], off, 64);
+  }
+  return r;
+}
+
+// One wavefront per task.
+__global__ void __launch_bounds__(256) k_accumulate(const Task* tasks, const uint32_t* ntasks,
+                                                    const uint32_t* sorted,
+                                                    const G1Affine* bases, G1Xyzz* partials) {
+  const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (task >= *ntasks) return;  // uniform per wavefront
+  const Task t = tasks[task];
+  G1Xyzz acc = G1Xyzz::identity();
+  for (uint32_t e = lane; e < t.len; e += 64) {
+    const uint32_t v = sorted[t.start + e];
+    G1Affine p = load_affine(bases, v & 0x7fffffffu);
+    if (v >> 31) p.y = neg(p.y);
+    acc = add_mixed(acc, p);
+  }
+  for (unsigned off = 32; off > 0; off >>= 1) {
+    const G1Xyzz o = shfl_down_xyzz(acc, off);
+    if (lane < off) acc = add(acc, o);
+  }
+  if (lane == 0) store_xyzz(partials, task, acc);
+}
+
+__global__ void k_bucket_sums(const uint32_t* task_off, const uint32_t* ntask,
+                              const G1Xyzz* partials, uint32_t buckets, G1Xyzz* sums) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
+       b += gridDim.x * blockDim.x) {
+    const uint32_t first = task_off[b], cnt = ntask[b];
+    G1Xyzz s = G1Xyzz::identity();
+    if (cnt > 0) s = load_xyzz(partials, first);
+    for (uint32_t k = 1; k < cnt; ++k) s = add(s, load_xyzz(partials, first + k));
+    store_xyzz(sums, b, s);
+  }
+}
+
+// Level 1: blocks of 256 buckets.  U_t = sum_{t' >= t} S_t' (suffix scan),
+// A = sum_t U_t = sum_t (t + 1) S_t, R = U_0.
+constexpr unsigned kRed1 = 256;
+__global__ void __launch_bounds__(kRed1) k_reduce_buckets(const G1Xyzz* sums, uint32_t buckets,
+                                                          G1Xyzz* pair_a, G1Xyzz* pair_r) {
+  __shared__ G1Xyzz u[kRed1];
+  const uint32_t t = threadIdx.x;
+  const uint32_t b = blockIdx.x * kRed1 + t;
+  G1Xyzz v = b < buckets ? load_xyzz(sums, b) : G1Xyzz::identity();
+  u[t] = v;
+  __syncthreads();
+  for (unsigned d = 1; d < kRed1; d <<= 1) {
+    if (t + d < kRed1) v = add(v, u[t + d]);
+    __syncthreads();
+    u[t] = v;
+    __syncthreads();
+  }
+  if (t == 0) store_xyzz(pair_r, blockIdx.x, v);
+  // tree sum of U
+  for (unsigned d = kRed1 / 2; d > 0; d >>= 1) {
+    if (t < d) {
+      v = add(v, u[t + d]);
+      u[t] = v;
+    }
+    __syncthreads();
+  }
+  if (t == 0) store_xyzz(pair_a, blockIdx.x, v);
+}
+
+// Higher levels: blocks of 128 pairs (A_k, R_k) whose buckets span `span`
+// (= 2^span_log2) each: A' = sum A_k + span * sum_k k R_k (k relative to the
+// block), R' = sum R_k.
+constexpr unsigned kRed2 = 128;
+__global__ void __launch_bounds__(kRed2) k_reduce_pairs(const G1Xyzz* in_a, const G1Xyzz* in_r,
+                                                        uint32_t count, uint32_t span_log2,
+                                                        G1Xyzz* out_a, G1Xyzz* out_r) {
+  __shared__ G1Xyzz u[kRed2];
+  __shared__ G1Xyzz a[kRed2];
+  const uint32_t t = threadIdx.x;
+  const uint32_t k = blockIdx.x * kRed2 + t;
+  G1Xyzz v = k < count ? load_xyzz(in_r, k) : G1Xyzz::identity();
+  G1Xyzz w = k < count ? load_xyzz(in_a, k) : G1Xyzz::identity();
+  u[t] = v;
+  a[t] = w;
+  __syncthreads();
+  for (unsigned d = 1; d < kRed2; d <<= 1) {
+    if (t + d < kRed2) v = add(v, u[t + d]);
+    __syncthreads();
+    u[t] = v;
+    __syncthreads();
+  }
+  if (t == 0) {
+    store_xyzz(out_r, blockIdx.x, v);
+    v = G1Xyzz::identity();  // sum_{t >= 1} U_t = sum_t t R_t
+    u[0] = v;
+  }
+  __syncthreads();
+  for (unsigned d = kRed2 / 2; d > 0; d >>= 1) {
+    if (t < d) {
+      v = add(v, u[t + d]);
+      u[t] = v;
+      w = add(w, a[t + d]);
+      a[t] = w;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (uint32_t s = 0; s < span_log2; ++s) v = dbl(v);
+    store_xyzz(out_a, blockIdx.x, add(w, v));
+  }
+}
+
+// ---- SRS preparation --------------------------------------------------------
+// next[i] = 2^c prev[i], normalised with one inversion per thread
+// (Montgomery batch inversion over the thread's strided points).
+__global__ void k_shift_window(const G1Affine* prev, G1Affine* next, size_t n, uint32_t c,
+                               G1Xyzz* scratch, Fq* prefix) {
+  const size_t threads = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  Fq acc = Fq::one();
+  for (size_t i = tid; i < n; i += threads) {
+    const G1Affine p = load_affine(prev, i);
+    G1Xyzz q = p.is_identity() ? G1Xyzz::identity() : dbl_affine(p);
+    for (uint32_t s = 1; s < c; ++s) q = dbl(q);
+    store_xyzz(scratch, i, q);
+    store_fq(&prefix[i], acc);
+    if (!q.is_identity()) acc = acc * (q.ZZ * q.ZZZ);
+  }
+  Fq inv = inverse(acc);
+  const size_t last = tid + ((n - 1 - tid) / threads) * threads;
+  for (size_t i = last;; i -= threads) {
+    const G1Xyzz q = load_xyzz(scratch, i);
+    if (q.is_identity()) {
+      store_affine(next, i, G1Affine::identity());
+    } else {
+      const Fq d_inv = inv * load_fq(&prefix[i]);  // 1 / (ZZ ZZZ)
+      inv = inv * (q.ZZ * q.ZZZ);
+      store_affine(next, i, {q.X * (d_inv * q.ZZZ), q.Y * (d_inv * q.ZZ)});
+    }
+    if (i < threads) break;
+  }
+}
+
+// Canonical packets (x LE48 | y LE48 as 24 u32, identity = zeros) -> Montgomery
+// affine, flagging off-curve finite points.
+__global__ void k_points_to_mont(G1Affine* pts, size_t n, unsigned* bad) {
+  Fq four = Fq::zero();
+  four.v[0] = 4;
+  four = to_mont(four);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    G1Affine p = load_affine(pts, i);
+    if (p.is_identity()) continue;
+    p.x = to_mont(p.x);
+    p.y = to_mont(p.y);
+    if (sqr(p.y) != sqr(p.x) * p.x + four) atomicOr(bad, 1u);
+    store_affine(pts, i, p);
+  }
+}
+
+__global__ void k_points_from_mont(const G1Affine* pts, G1Affine* out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    G1Affine p = load_affine(pts, i);
+    if (!p.is_identity()) {
+      p.x = from_mont(p.x);
+      p.y = from_mont(p.y);
+    }
+    store_affine(out, i, p);
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// ---- fixed-base generation (SRS) ------------------------------------------
+// table[w * 256 + d] = d * 2^(8 w) * g (affine), built by one thread per entry.
+__global__ void k_fixed_table(G1Affine g, G1Affine* table) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 32 * 256) return;
+  const uint32_t w = idx / 256, d = idx % 256;
+  G1Xyzz base = G1Xyzz::from_affine(g);
+  for (uint32_t s = 0; s < 8 * w; ++s) base = dbl(base);
+  G1Xyzz acc = G1Xyzz::identity();
+  for (int b = 7; b >= 0; --b) {
+    acc = dbl(acc);
+    if ((d >> b) & 1) acc = add(acc, base);
+  }
+  store_affine(table, idx, to_affine(acc));
+}
+
+// out[i] = s_i * g using the byte table; batch-normalised per thread.
+__global__ void k_fixed_base(const Fr* scalars, size_t n, const G1Affine* table, G1Affine* out,
+                             G1Xyzz* scratch, Fq* prefix) {
+  const size_t threads = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  Fq acc = Fq::one();
+  for (size_t i = tid; i < n; i += threads) {
+    const Fr s = from_mont(load_fr(scalars, i));
+    G1Xyzz q = G1Xyzz::identity();
+    for (uint32_t w = 0; w < 32; ++w) {
+      const uint32_t d = (s.v[w >> 2] >> (8 * (w & 3))) & 0xffu;
+      if (d) q = add_mixed(q, load_affine(table, w * 256 + d));
+    }
+    store_xyzz(scratch, i, q);
+    store_fq(&prefix[i], acc);
+    if (!q.is_identity()) acc = acc * (q.ZZ * q.ZZZ);
+  }
+  Fq inv = inverse(acc);
+  const size_t last = tid + ((n - 1 - tid) / threads) * threads;
+  for (size_t i = last;; i -= threads) {
+    const G1Xyzz q = load_xyzz(scratch, i);
+    if (q.is_identity()) {
+      store_affine(out, i, G1Affine::identity());
+    } else {
+      const Fq d_inv = inv * load_fq(&prefix[i]);
+      inv = inv * (q.ZZ * q.ZZZ);
+      store_affine(out, i, {q.X * (d_inv * q.ZZZ), q.Y * (d_inv * q.ZZ)});
+    }
+    if (i < threads) break;
+  }
+}
+
+unsigned grid_for(size_t n, unsigned threads = 256, size_t cap = 65535) {
+  size_t b = (n + threads - 1) / threads;
+  return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, cap)));
+}
+
+// Exclusive scan of n uint32 values (n <= 4M); optional grand total.
+void exclusive_scan(rt::Context& c, const uint32_t* in, uint32_t* out, size_t n,
+                    uint32_t* total_out, rt::BufferRef& scratch) {
+  const size_t tiles = (n + kScanTile - 1) / kScanTile;
+  scratch = c.allocate((tiles + 1) * sizeof(uint32_t));
+  uint32_t* sums = static_cast<uint32_t*>(scratch->ptr);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, c.stream, in, out,
+                     n, sums);
+  rt::check_launch();
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, c.stream, sums, tiles);
+  rt::check_launch();
+  hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c.stream, out, n,
+                     sums, total_out, tiles);
+  rt::check_launch();
+}
+
+void zero_u32(rt::Context& c, uint32_t* p, size_t n) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n)), dim3(256), 0, c.stream, p, n);
+  rt::check_launch();
+}
+
+}  // namespace
+
+Plan choose_plan(size_t n, uint32_t override_c) {
+  Plan p;
+  uint32_t c = override_c;
+  if (c == 0) {
+    // ~512 entries per bucket: B = 2^(c-1) ~ n W / 512.
+    c = 16;
+    for (int it = 0; it < 3; ++it) {
+      const uint32_t w = (256 + c - 1) / c;
+      const double lg = std::log2(std::max<double>(1.0, (double)n * w));
+      int cc = (int)std::lround(lg) - 8;
+      c = (uint32_t)std::min(20, std::max(4, cc));
+    }
+  }
+  p.c = c;
+  p.windows = (256 + c - 1) / c;
+  p.buckets = 1u << (c - 1);
+  p.task_cap = 64 * 16;
+  return p;
+}
+
+void prepare_layer_from_device(rt::Context& c, Layer& layer, const G1Affine* points, size_t n,
+                               uint32_t override_c) {
+  layer.n = n;
+  layer.plan = choose_plan(n, override_c);
+  const uint32_t W = layer.plan.windows;
+  layer.bases = c.allocate(W * n * sizeof(G1Affine));
+  G1Affine* bases = static_cast<G1Affine*>(layer.bases->ptr);
+  rt::hip_check(hipMemcpyAsync(bases, points, n * sizeof(G1Affine), hipMemcpyDeviceToDevice,
+                               c.stream));
+  if (W > 1) {
+    const size_t threads = std::max<size_t>(1, std::min<size_t>(n, 65536));
+    rt::BufferRef scratch = c.allocate(n * sizeof(G1Xyzz));
+    rt::BufferRef prefix = c.allocate(n * sizeof(Fq));
+    for (uint32_t w = 1; w < W; ++w) {
+      hipLaunchKernelGGL(k_shift_window, dim3(grid_for(threads, 64)), dim3(64), 0, c.stream,
+                         bases + (w - 1) * n, bases + w * n, n, layer.plan.c,
+                         static_cast<G1Xyzz*>(scratch->ptr), static_cast<Fq*>(prefix->ptr));
+      rt::check_launch();
+    }
+  }
+}
+
+bool upload_layer(rt::Context& c, Layer& layer, const uint8_t* packets, size_t n,
+                  uint32_t override_c) {
+  // host validation: tag, identity coordinates, canonical coordinates
+  for (size_t i = 0; i < n; ++i) {
+    const uint8_t* p = packets + rt::kG1Bytes * i;
+    if (p[0] == 1) {
+      for (size_t k = 1; k < rt::kG1Bytes; ++k)
+        if (p[k]) return false;
+      continue;
+    }
+    if (p[0] != 0) return false;
+    if (!rt::canonical_fq(p + 1) || !rt::canonical_fq(p + 49)) return false;
+    bool all_zero = true;
+    for (size_t k = 1; k < rt::kG1Bytes && all_zero; ++k) all_zero = p[k] == 0;
+    if (all_zero) return false;  // finite-tag (0, 0)
+  }
+  rt::BufferRef raw = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Affine));
+  // pack to 96-byte coordinates (identity -> zeros) through pinned staging
+  const size_t chunk = (size_t(32) << 20) / sizeof(G1Affine);
+  for (size_t done = 0; done < n; done += chunk) {
+    const size_t cnt = std::min(chunk, n - done);
+    uint8_t* stage = c.staging(cnt * sizeof(G1Affine));
+    for (size_t i = 0; i < cnt; ++i) {
+      const uint8_t* p = packets + rt::kG1Bytes * (done + i);
+      uint8_t* q = stage + sizeof(G1Affine) * i;
+      if (p[0] == 1)
+        std::memset(q, 0, sizeof(G1Affine));
+      else
+        std::memcpy(q, p + 1, 96);
+    }
+    rt::hip_check(hipMemcpyAsync(static_cast<G1Affine*>(raw->ptr) + done, stage,
+                                 cnt * sizeof(G1Affine), hipMemcpyHostToDevice, c.stream));
+    c.staging_used();
+  }
+  rt::BufferRef flag = c.allocate(sizeof(unsigned));
+  zero_u32(c, static_cast<uint32_t*>(flag->ptr), 1);
+  hipLaunchKernelGGL(k_points_to_mont, dim3(grid_for(n)), dim3(256), 0, c.stream,
+                     static_cast<G1Affine*>(raw->ptr), n, static_cast<unsigned*>(flag->ptr));
+  rt::check_launch();
+  unsigned bad = 0;
+  rt::download_bytes(c, flag->ptr, sizeof(unsigned), reinterpret_cast<uint8_t*>(&bad));
+  if (bad) return false;
+  prepare_layer_from_device(c, layer, static_cast<const G1Affine*>(raw->ptr), n, override_c);
+  return true;
+}
+
+void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* result,
+             StageEvents* ev) {
+  auto mark = [&](int i) {
+    if (ev) rt::hip_check(hipEventRecord(ev->marks[i], c.stream));
+  };
+  const Plan& p = layer.plan;
+  const size_t n = layer.n;
+  const size_t m = n * p.windows;
+  const uint32_t B = p.buckets;
+  rt::BufferRef kv = c.allocate(2 * m * sizeof(uint32_t));
+  uint32_t* keys = static_cast<uint32_t*>(kv->ptr);
+  uint32_t* vals = keys + m;
+  rt::BufferRef sorted_buf = c.allocate(std::max<size_t>(m, 1) * sizeof(uint32_t));
+  uint32_t* sorted = static_cast<uint32_t*>(sorted_buf->ptr);
+  // per-bucket arrays: counts | offsets | cursor | ntask | task_off | totals(2)
+  rt::BufferRef meta = c.allocate((5 * (size_t)B + 4) * sizeof(uint32_t));
+  uint32_t* counts = static_cast<uint32_t*>(meta->ptr);
+  uint32_t* offsets = counts + B;
+  uint32_t* cursor = offsets + B;
+  uint32_t* ntask = cursor + B;
+  uint32_t* task_off = ntask + B;
+  uint32_t* totals = task_off + B;  // [0] entries, [1] tasks
+
+  mark(0);
+  hipLaunchKernelGGL(k_digits, dim3(grid_for(n)), dim3(256), 0, c.stream, scalars, n, p.c,
+                     p.windows, keys, vals);
+  rt::check_launch();
+  mark(1);
+  zero_u32(c, counts, B);
+  hipLaunchKernelGGL(k_histogram, dim3(grid_for(m, 256, 16384)), dim3(256), 0, c.stream, keys, m,
+                     counts);
+  rt::check_launch();
+  rt::BufferRef scan_a, scan_b, scan_c;
+  exclusive_scan(c, counts, offsets, B, totals, scan_a);
+  rt::hip_check(hipMemcpyAsync(cursor, offsets, B * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                               c.stream));
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(m, 256, 16384)), dim3(256), 0, c.stream, keys, vals,
+                     m, cursor, sorted);
+  rt::check_launch();
+  mark(2);
+  hipLaunchKernelGGL(k_task_counts, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, B,
+                     p.task_cap, ntask);
+  rt::check_launch();
+  exclusive_scan(c, ntask, task_off, B, totals + 1, scan_b);
+  const size_t max_tasks = m / p.task_cap + B + 1;
+  rt::BufferRef tasks_buf = c.allocate(max_tasks * sizeof(Task));
+  Task* tasks = static_cast<Task*>(tasks_buf->ptr);
+  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, offsets,
+                     task_off, B, p.task_cap, tasks);
+  rt::check_launch();
+  rt::BufferRef partials_buf = c.allocate(max_tasks * sizeof(G1Xyzz));
+  G1Xyzz* partials = static_cast<G1Xyzz*>(partials_buf->ptr);
+  mark(3);
+  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((max_tasks + 3) / 4)), dim3(256), 0, c.stream,
+                     tasks, totals + 1, sorted, static_cast<const G1Affine*>(layer.bases->ptr),
+                     partials);
+  rt::check_launch();
+  rt::BufferRef sums_buf = c.allocate((size_t)B * sizeof(G1Xyzz));
+  G1Xyzz* sums = static_cast<G1Xyzz*>(sums_buf->ptr);
+  mark(4);
+  hipLaunchKernelGGL(k_bucket_sums, dim3(grid_for(B, 64)), dim3(64), 0, c.stream, task_off, ntask,
+                     partials, B, sums);
+  rt::check_launch();
+  // weighted bucket reduction
+  size_t groups = (B + kRed1 - 1) / kRed1;
+  rt::BufferRef pairs_buf = c.allocate(2 * 2 * groups * sizeof(G1Xyzz));
+  G1Xyzz* pa = static_cast<G1Xyzz*>(pairs_buf->ptr);
+  G1Xyzz* pr = pa + groups;
+  G1Xyzz* qa = pr + groups;
+  G1Xyzz* qr = qa + groups;
+  mark(5);
+  hipLaunchKernelGGL(k_reduce_buckets, dim3((unsigned)groups), dim3(kRed1), 0, c.stream, sums, B,
+                     pa, pr);
+  rt::check_launch();
+  uint32_t span_log2 = 8;  // buckets per level-1 group
+  while (groups > 1) {
+    const size_t next = (groups + kRed2 - 1) / kRed2;
+    G1Xyzz* oa = next == 1 ? result : qa;
+    hipLaunchKernelGGL(k_reduce_pairs, dim3((unsigned)next), dim3(kRed2), 0, c.stream, pa, pr,
+                       (uint32_t)groups, span_log2, oa, qr);
+    rt::check_launch();
+    if (next == 1) break;
+    std::swap(pa, qa);
+    std::swap(pr, qr);
+    groups = next;
+    span_log2 += 7;
+  }
+  if (groups == 1)
+    rt::hip_check(hipMemcpyAsync(result, pa, sizeof(G1Xyzz), hipMemcpyDeviceToDevice, c.stream));
+  if (ev) {
+    mark(6);
+    if (ev->entries_out)
+      rt::hip_check(hipMemcpyAsync(ev->entries_out, totals, sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, c.stream));
+  }
+}
+
+void affine_to_packet(const G1Affine& a, uint8_t* out) {
+  std::memset(out, 0, rt::kG1Bytes);
+  if (a.is_identity()) {
+    out[0] = 1;
+    return;
+  }
+  const Fq x = from_mont(a.x), y = from_mont(a.y);
+  std::memcpy(out + 1, x.v, 48);
+  std::memcpy(out + 49, y.v, 48);
+}
+
+void to_packet(const G1Xyzz& p, uint8_t* out) { affine_to_packet(to_affine(p), out); }
+
+void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets) {
+  const size_t n = layer.n;
+  rt::BufferRef tmp = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Affine));
+  hipLaunchKernelGGL(k_points_from_mont, dim3(grid_for(n)), dim3(256), 0, c.stream,
+                     static_cast<const G1Affine*>(layer.bases->ptr),
+                     static_cast<G1Affine*>(tmp->ptr), n);
+  rt::check_launch();
+  std::vector<uint8_t> raw(n * sizeof(G1Affine));
+  rt::download_bytes(c, tmp->ptr, raw.size(), raw.data());
+  for (size_t i = 0; i < n; ++i) {
+    uint8_t* out = packets + rt::kG1Bytes * i;
+    const uint8_t* q = raw.data() + sizeof(G1Affine) * i;
+    bool zero = std::all_of(q, q + 96, [](uint8_t v) { return v == 0; });
+    out[0] = zero ? 1 : 0;
+    if (zero)
+      std::memset(out + 1, 0, 96);
+    else
+      std::memcpy(out + 1, q, 96);
+  }
+}
+
+void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out) {
+  // standard generator (canonical) -> Montgomery on the host
+  static const uint32_t gx[12] = {0xdb22c6bbu, 0xfb3af00au, 0xf97a1aefu, 0x6c55e83fu,
+                                  0x171bac58u, 0xa14e3a3fu, 0x9774b905u, 0xc3688c4fu,
+                                  0x4fa9ac0fu, 0x2695638cu, 0x3197d794u, 0x17f1d3a7u};
+  static const uint32_t gy[12] = {0x46c5e7e1u, 0x0caa2329u, 0xa2888ae4u, 0xd03cc744u,
+                                  0x2c04b3edu, 0x00db18cbu, 0xd5d00af6u, 0xfcf5e095u,
+                                  0x741d8ae4u, 0xa09e30edu, 0xe3aaa0f1u, 0x08b3f481u};
+  G1Affine g;
+  std::memcpy(g.x.v, gx, 48);
+  std::memcpy(g.y.v, gy, 48);
+  g.x = to_mont(g.x);
+  g.y = to_mont(g.y);
+  rt::BufferRef table = c.allocate(32 * 256 * sizeof(G1Affine));
+  hipLaunchKernelGGL(k_fixed_table, dim3(32), dim3(256), 0, c.stream, g,
+                     static_cast<G1Affine*>(table->ptr));
+  rt::check_launch();
+  const size_t threads = std::max<size_t>(1, std::min<size_t>(n, 65536));
+  rt::BufferRef scratch = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Xyzz));
+  rt::BufferRef prefix = c.allocate(std::max<size_t>(n, 1) * sizeof(Fq));
+  hipLaunchKernelGGL(k_fixed_base, dim3(grid_for(threads, 64)), dim3(64), 0, c.stream, scalars, n,
+                     static_cast<const G1Affine*>(table->ptr), out,
+                     static_cast<G1Xyzz*>(scratch->ptr), static_cast<Fq*>(prefix->ptr));
+  rt::check_launch();
+}
+
+}  // namespace msm
+}  // namespace hp

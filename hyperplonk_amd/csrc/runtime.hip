@@ -1,0 +1,287 @@
+// Runtime: contexts, caching allocator, pinned staging, handle registry and
+// the runtime entry points of the C ABI (include/hp_mi355x.h).
+//
+// Mirrors the lifetime contract of the reference runtime
+// (backend/src/ascend910b3/native/bridge_runtime.h:1-41): contexts are host
+// metadata, buffers are released to a stream-ordered reuse queue rather than
+// freed while work may still read them, and context_destroy drains the stream
+// before returning memory to the driver.
+#include <algorithm>
+#include <mutex>
+#include <unordered_set>
+
+#include "kernels_common.hpp"
+#include "runtime.hpp"
+
+namespace hp {
+namespace rt {
+
+namespace {
+std::mutex g_contexts_mutex;
+std::unordered_set<HpMi355xContext*> g_contexts;
+
+template <int N>
+bool canonical_limbs(const uint8_t* bytes, const uint32_t* mod) {
+  // compare from the most significant limb down
+  for (int i = N - 1; i >= 0; --i) {
+    uint32_t w;
+    std::memcpy(&w, bytes + 4 * i, 4);
+    if (w < mod[i]) return true;
+    if (w > mod[i]) return false;
+  }
+  return false;  // equal to the modulus
+}
+}  // namespace
+
+bool canonical_fr(const uint8_t* bytes) {
+  return canonical_limbs<8>(bytes, FrParams::MOD);
+}
+bool canonical_fq(const uint8_t* bytes) {
+  return canonical_limbs<12>(bytes, FqParams::MOD);
+}
+bool canonical_fr_array(const uint8_t* bytes, size_t count) {
+  for (size_t i = 0; i < count; ++i)
+    if (!canonical_fr(bytes + kFrBytes * i)) return false;
+  return true;
+}
+
+size_t Allocator::round_up(size_t bytes) {
+  if (bytes <= 256) return 256;
+  if (bytes <= (size_t(1) << 20)) {
+    size_t r = 256;
+    while (r < bytes) r <<= 1;
+    return r;
+  }
+  const size_t granule = size_t(2) << 20;
+  return (bytes + granule - 1) / granule * granule;
+}
+
+void* Allocator::acquire(size_t bytes) {
+  const size_t size = round_up(bytes);
+  auto it = free_.find(size);
+  if (it != free_.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    cached_ -= size;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, size);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim();
+    e = hipMalloc(&p, size);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail_oom();
+    }
+  }
+  return p;
+}
+
+void Allocator::release(void* ptr, size_t bytes) {
+  if (!ptr) return;
+  const size_t size = round_up(bytes);
+  free_[size].push_back(ptr);
+  cached_ += size;
+}
+
+void Allocator::trim() {
+  (void)hipStreamSynchronize(stream_);
+  for (auto& kv : free_)
+    for (void* p : kv.second) (void)hipFree(p);
+  free_.clear();
+  cached_ = 0;
+}
+
+Allocator::~Allocator() { trim(); }
+
+Buffer::~Buffer() {
+  if (ctx && ptr && ctx->alloc) ctx->alloc->release(ptr, bytes);
+}
+
+BufferRef Context::allocate(size_t bytes, size_t nv) {
+  auto b = std::make_shared<Buffer>();
+  b->ctx = this;
+  b->bytes = bytes;
+  b->num_vars = nv;
+  b->ptr = alloc->acquire(bytes == 0 ? 1 : bytes);
+  return b;
+}
+
+void* Context::publish(BufferRef buffer) {
+  void* p = buffer->ptr;
+  registry[p] = std::move(buffer);
+  return p;
+}
+
+BufferRef Context::lookup(void* handle, size_t nv) const {
+  if (handle == nullptr) return nullptr;
+  auto it = registry.find(handle);
+  if (it == registry.end()) return nullptr;
+  if (it->second->num_vars != nv) return nullptr;
+  return it->second;
+}
+
+uint8_t* Context::staging(size_t bytes) {
+  if (pinned_free) hip_check(hipEventSynchronize(pinned_free));
+  if (bytes > pinned_bytes) {
+    if (pinned) hip_check(hipHostFree(pinned));
+    pinned = nullptr;
+    pinned_bytes = 0;
+    size_t want = std::max(bytes, size_t(1) << 20);
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&pinned), want, hipHostMallocDefault));
+    pinned_bytes = want;
+  }
+  return pinned;
+}
+
+void Context::staging_used() {
+  if (!pinned_free) hip_check(hipEventCreateWithFlags(&pinned_free, hipEventDisableTiming));
+  hip_check(hipEventRecord(pinned_free, stream));
+}
+
+void Context::sync() { hip_check(hipStreamSynchronize(stream)); }
+
+// ---- boundary conversions ----------------------------------------------
+namespace {
+__global__ void k_fr_to_mont(Fr* data, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    store_fr(data, i, to_mont(load_fr(data, i)));
+}
+
+__global__ void k_fr_from_mont(const Fr* src, Fr* dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    store_fr(dst, i, from_mont(load_fr(src, i)));
+}
+
+unsigned grid_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b == 0) b = 1;
+  return static_cast<unsigned>(b);
+}
+
+constexpr size_t kStagingChunk = size_t(64) << 20;
+}  // namespace
+
+void upload_fr_values(Context& c, Fr* dst, const uint8_t* host, size_t count) {
+  size_t done = 0;
+  while (done < count) {
+    const size_t n = std::min(count - done, kStagingChunk / kFrBytes);
+    uint8_t* stage = c.staging(n * kFrBytes);
+    std::memcpy(stage, host + done * kFrBytes, n * kFrBytes);
+    hip_check(hipMemcpyAsync(dst + done, stage, n * kFrBytes, hipMemcpyHostToDevice, c.stream));
+    c.staging_used();
+    done += n;
+  }
+  if (count) {
+    hipLaunchKernelGGL(k_fr_to_mont, dim3(grid_for(count)), dim3(256), 0, c.stream, dst, count);
+    check_launch();
+  }
+}
+
+BufferRef upload_fr_table(Context& c, const uint8_t* host, size_t count, size_t num_vars) {
+  BufferRef buf = c.allocate(count * kFrBytes, num_vars);
+  upload_fr_values(c, static_cast<Fr*>(buf->ptr), host, count);
+  return buf;
+}
+
+void download_bytes(Context& c, const void* src, size_t bytes, uint8_t* host) {
+  size_t done = 0;
+  while (done < bytes) {
+    const size_t n = std::min(bytes - done, kStagingChunk);
+    uint8_t* stage = c.staging(n);
+    hip_check(hipMemcpyAsync(stage, static_cast<const uint8_t*>(src) + done, n,
+                             hipMemcpyDeviceToHost, c.stream));
+    c.staging_used();
+    hip_check(hipStreamSynchronize(c.stream));
+    std::memcpy(host + done, stage, n);
+    done += n;
+  }
+}
+
+void download_fr(Context& c, const Fr* src, size_t count, uint8_t* host) {
+  if (count == 0) return;
+  BufferRef tmp = c.allocate(count * kFrBytes);
+  hipLaunchKernelGGL(k_fr_from_mont, dim3(grid_for(count)), dim3(256), 0, c.stream, src,
+                     static_cast<Fr*>(tmp->ptr), count);
+  check_launch();
+  download_bytes(c, tmp->ptr, count * kFrBytes, host);
+}
+
+bool context_live(HpMi355xContext* ctx) {
+  std::lock_guard<std::mutex> lock(g_contexts_mutex);
+  return g_contexts.count(ctx) != 0;
+}
+
+}  // namespace rt
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" HpMi355xStatus hp_mi355x_v1_context_create(int32_t device_id, HpMi355xContext** out) {
+  if (out == nullptr) return rt::invalid_parameters();
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return rt::invalid_state();
+  }
+  if (device_id < 0 || device_id >= count) return rt::invalid_parameters();
+  if (hipSetDevice(device_id) != hipSuccess) return rt::invalid_state();
+  HpMi355xContext* ctx = nullptr;
+  try {
+    ctx = new HpMi355xContext();
+    ctx->impl.device = device_id;
+    rt::hip_check(hipStreamCreateWithFlags(&ctx->impl.stream, hipStreamNonBlocking));
+    ctx->impl.alloc = std::make_unique<rt::Allocator>(ctx->impl.stream);
+  } catch (const rt::Failure& f) {
+    if (ctx) {
+      if (ctx->impl.stream) (void)hipStreamDestroy(ctx->impl.stream);
+      delete ctx;
+    }
+    return f.status;
+  } catch (...) {
+    delete ctx;
+    return rt::invalid_state(-2);
+  }
+  {
+    std::lock_guard<std::mutex> lock(rt::g_contexts_mutex);
+    rt::g_contexts.insert(ctx);
+  }
+  *out = ctx;
+  return rt::ok();
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext* ctx) {
+  if (ctx == nullptr) return rt::invalid_parameters();
+  {
+    std::lock_guard<std::mutex> lock(rt::g_contexts_mutex);
+    if (!rt::g_contexts.erase(ctx)) return rt::invalid_parameters();
+  }
+  rt::Context& c = ctx->impl;
+  (void)hipSetDevice(c.device);
+  const hipError_t drained = hipStreamSynchronize(c.stream);
+  c.resources.clear();
+  c.registry.clear();
+  c.alloc.reset();  // frees every cached block (after the drain)
+  if (c.pinned) (void)hipHostFree(c.pinned);
+  if (c.pinned_free) (void)hipEventDestroy(c.pinned_free);
+  (void)hipStreamDestroy(c.stream);
+  delete ctx;
+  if (drained != hipSuccess) return rt::invalid_state(static_cast<int32_t>(drained));
+  return rt::ok();
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_buffer_retire(HpMi355xContext* ctx,
+                                                     HpMi355xDeviceAddress address) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    auto it = c.registry.find(address);
+    if (address == nullptr || it == c.registry.end()) return rt::invalid_parameters();
+    c.registry.erase(it);  // states holding references keep the memory alive
+    return rt::ok();
+  });
+}

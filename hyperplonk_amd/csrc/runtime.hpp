@@ -1,0 +1,162 @@
+// Context, device memory and status plumbing shared by every entry point.
+//
+// One context = one device + one HIP stream, driven by one host thread at a
+// time (the single-thread contract of the reference bridge,
+// backend/src/ascend910b3/native/bridge.h:16-21).  Device memory comes from a
+// per-context caching allocator: freed blocks go to size-keyed free lists and
+// are reused in stream order, so a handle's memory is recycled only after the
+// work already queued on the stream that touched it (all work of a context is
+// on its stream).  MLE handles are the base addresses of registered buffers;
+// prover states hold shared references so a retired handle stays alive for as
+// long as a state still reads it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hp_mi355x.h"
+
+struct HpMi355xContext;
+
+namespace hp {
+namespace rt {
+
+constexpr size_t kFrBytes = 32;
+constexpr size_t kG1Bytes = 97;
+constexpr size_t kMaxNumVars = 40;
+
+inline HpMi355xStatus ok() { return {HP_MI355X_STATUS_SUCCESS, 0}; }
+inline HpMi355xStatus invalid_parameters(int32_t code = 0) {
+  return {HP_MI355X_STATUS_INVALID_PARAMETERS, code};
+}
+inline HpMi355xStatus invalid_state(int32_t code = 0) {
+  return {HP_MI355X_STATUS_INVALID_STATE, code};
+}
+inline bool is_ok(const HpMi355xStatus& s) { return s.kind == 0 && s.native_code == 0; }
+
+// Thrown inside the library only; converted to a status at the ABI.
+// `poison` marks device-side failures, after which the context is unusable;
+// validation failures and exhausted memory leave the context intact.
+struct Failure {
+  HpMi355xStatus status;
+  bool poison;
+};
+
+[[noreturn]] inline void fail_state(int32_t code) { throw Failure{invalid_state(code), true}; }
+[[noreturn]] inline void fail_params() { throw Failure{invalid_parameters(), false}; }
+[[noreturn]] inline void fail_oom() {
+  throw Failure{invalid_state(static_cast<int32_t>(hipErrorOutOfMemory)), false};
+}
+
+inline void hip_check(hipError_t e) {
+  if (e != hipSuccess) fail_state(static_cast<int32_t>(e));
+}
+
+// Canonical little-endian field element check (value < modulus).
+bool canonical_fr(const uint8_t* bytes);
+bool canonical_fq(const uint8_t* bytes);
+bool canonical_fr_array(const uint8_t* bytes, size_t count);
+
+class Allocator {
+ public:
+  explicit Allocator(hipStream_t stream) : stream_(stream) {}
+  ~Allocator();
+  void* acquire(size_t bytes);
+  void release(void* ptr, size_t bytes);
+  void trim();  // synchronises, then frees every cached block
+  size_t cached_bytes() const { return cached_; }
+
+ private:
+  static size_t round_up(size_t bytes);
+  hipStream_t stream_;
+  std::map<size_t, std::vector<void*>> free_;
+  size_t cached_ = 0;
+};
+
+struct Context;
+
+// A device allocation owned by a context.  Releasing the last reference
+// returns the block to the context allocator.
+struct Buffer {
+  Context* ctx = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  size_t num_vars = 0;  // for MLE tables
+  ~Buffer();
+};
+using BufferRef = std::shared_ptr<Buffer>;
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Allocator> alloc;
+  std::unordered_map<void*, BufferRef> registry;  // live MLE handles
+  // Live prover states and KZG parameters (released before memory on destroy).
+  std::unordered_map<const void*, std::shared_ptr<void>> resources;
+  bool poisoned = false;
+  int32_t first_error = 0;
+  // Pinned host staging for uploads and small results.
+  uint8_t* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  hipEvent_t pinned_free = nullptr;  // last copy that read/wrote `pinned`
+
+  // Allocate a new device buffer (not registered).
+  BufferRef allocate(size_t bytes, size_t num_vars = 0);
+  // Register an MLE buffer as a handle.
+  void* publish(BufferRef buffer);
+  // Look up a live MLE handle with the expected dimension.
+  BufferRef lookup(void* handle, size_t num_vars) const;
+  // Pinned host bytes of at least `bytes`, after previous users completed.
+  uint8_t* staging(size_t bytes);
+  void staging_used();  // record that queued work reads/writes staging
+  void sync();
+  void poison(int32_t code) {
+    if (!poisoned) first_error = code ? code : -1;
+    poisoned = true;
+  }
+};
+
+bool context_live(HpMi355xContext* ctx);
+
+// Launch-error check after kernel launches.
+inline void check_launch() { hip_check(hipGetLastError()); }
+
+// Grid helpers.
+inline unsigned blocks_for(size_t n, unsigned threads) {
+  size_t b = (n + threads - 1) / threads;
+  if (b == 0) b = 1;
+  return static_cast<unsigned>(b);
+}
+
+}  // namespace rt
+}  // namespace hp
+
+struct HpMi355xContext {
+  hp::rt::Context impl;
+};
+
+// Run `body` as an ABI call: validate the context, select its device, map
+// library failures to statuses and poison the context on device errors.
+template <class F>
+HpMi355xStatus hp_guarded(HpMi355xContext* ctx, F&& body) noexcept {
+  if (ctx == nullptr || !hp::rt::context_live(ctx)) return hp::rt::invalid_parameters();
+  hp::rt::Context& c = ctx->impl;
+  if (c.poisoned) return hp::rt::invalid_state(c.first_error);
+  try {
+    if (hipSetDevice(c.device) != hipSuccess) return hp::rt::invalid_state();
+    return body(c);
+  } catch (const hp::rt::Failure& f) {
+    if (f.poison) c.poison(f.status.native_code);
+    return f.status;
+  } catch (...) {
+    c.poison(-2);
+    return hp::rt::invalid_state(-2);
+  }
+}

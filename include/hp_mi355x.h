@@ -1,0 +1,177 @@
+/* C ABI of the MI355X HyperPlonk prover backend (hp_mi355x_v1_*).
+ *
+ * It is the drop-in replacement for the reference's accelerator bridge
+ * (backend/src/ascend910b3/native/bridge.h:29-62, bound from Rust in
+ * backend/src/ascend910b3/ffi.rs:34-173): the same 20 entry points with the
+ * same argument meaning, ownership, validation order and status kinds, so the
+ * Rust trait glue of that backend (MleBackend, MultilinearKzgBackend,
+ * SumCheckProver) binds to these symbols unchanged apart from the prefix.
+ *
+ * Host exchange formats (not device layouts): Fr = canonical 32-byte
+ * little-endian integer; G1 = 97 bytes: tag (0 finite, 1 identity with zero
+ * coordinates) | x LE48 | y LE48.  Byte widths: evaluations 32, commitments
+ * 97, point/r 32*num_vars, SRS layers 97*point_count, open quotients
+ * 97*num_vars, rounds 32*evaluation_count.  Zero-count arrays may be null,
+ * except product_offsets, which always holds product_count+1 entries
+ * starting at 0.  Host buffers are borrowed only until return; host outputs
+ * are fully written on success.  Resource outputs are unique and non-null on
+ * success; on failure output slots are cleared.  Inputs are immutable.  A
+ * context is used by one host thread at a time.  MLE handles are device base
+ * addresses of Montgomery-form tables owned by the context; retire hands a
+ * handle back, and its memory is reused only after every prover state that
+ * still reads it is destroyed.  Zero denominators (including 0/0) are
+ * InvalidParameters.  The first sumcheck round requires a null challenge,
+ * later rounds a non-null one; ordering errors and exhaustion return
+ * InvalidState without advancing the state.  A failed device execution
+ * poisons the state (and the context for context-wide failures); later calls
+ * on it return InvalidState.  No C++ exception crosses this interface.
+ */
+#ifndef HP_MI355X_H
+#define HP_MI355X_H
+#include <stddef.h>
+#include <stdint.h>
+
+#define HP_MI355X_STATUS_SUCCESS 0
+#define HP_MI355X_STATUS_INVALID_PARAMETERS 1
+#define HP_MI355X_STATUS_INVALID_STATE 2
+#define HP_MI355X_STATUS_NOT_IMPLEMENTED 3
+
+/* bridge.h:34 Ascend910B3Status: {kind, native_code}; only {0, 0} is success. */
+typedef struct HpMi355xStatus {
+  int32_t kind;
+  int32_t native_code;
+} HpMi355xStatus;
+
+typedef struct HpMi355xContext HpMi355xContext;
+typedef struct HpMi355xKzgParams HpMi355xKzgParams;
+typedef struct HpMi355xSumCheckState HpMi355xSumCheckState;
+typedef void *HpMi355xDeviceAddress;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime (bridge.h:43-45) ------------------------------------------ */
+/* replaces hp_ascend910b3_v1_context_create (bridge.h:43) */
+HpMi355xStatus hp_mi355x_v1_context_create(int32_t device_id, HpMi355xContext **out);
+/* replaces hp_ascend910b3_v1_context_destroy (bridge.h:44) */
+HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext *ctx);
+/* replaces hp_ascend910b3_v1_buffer_retire (bridge.h:45) */
+HpMi355xStatus hp_mi355x_v1_buffer_retire(HpMi355xContext *ctx, HpMi355xDeviceAddress address);
+
+/* ---- MLE (bridge.h:46-53; MleBackend, backend/src/multilinear_polynomial.rs:33-121) */
+/* mle_from_evaluations: replaces hp_ascend910b3_v1_mle_upload (bridge.h:46) */
+HpMi355xStatus hp_mi355x_v1_mle_upload(HpMi355xContext *ctx, size_t num_vars,
+                                       const uint8_t *evaluations, size_t evaluation_count,
+                                       HpMi355xDeviceAddress *out);
+/* linear_combination: replaces bridge.h:47 */
+HpMi355xStatus hp_mi355x_v1_mle_linear_combination(HpMi355xContext *ctx, size_t num_vars,
+                                                   const HpMi355xDeviceAddress *polynomials,
+                                                   const uint8_t *coefficients, size_t count,
+                                                   HpMi355xDeviceAddress *out);
+/* evaluate_mle: replaces bridge.h:48 */
+HpMi355xStatus hp_mi355x_v1_mle_evaluate(HpMi355xContext *ctx, HpMi355xDeviceAddress polynomial,
+                                         size_t num_vars, const uint8_t *point,
+                                         uint8_t *out_evaluation);
+/* build_eq_x_r: replaces bridge.h:49 */
+HpMi355xStatus hp_mi355x_v1_mle_build_eq_x_r(HpMi355xContext *ctx, const uint8_t *r,
+                                             size_t num_vars, HpMi355xDeviceAddress *out);
+/* compute_frac_poly: replaces bridge.h:50 */
+HpMi355xStatus hp_mi355x_v1_mle_compute_frac_poly(HpMi355xContext *ctx, size_t num_vars,
+                                                  const HpMi355xDeviceAddress *numerators,
+                                                  const HpMi355xDeviceAddress *denominators,
+                                                  size_t count, HpMi355xDeviceAddress *out);
+/* compute_product_poly: replaces bridge.h:51 */
+HpMi355xStatus hp_mi355x_v1_mle_compute_product_poly(HpMi355xContext *ctx,
+                                                     HpMi355xDeviceAddress frac, size_t num_vars,
+                                                     HpMi355xDeviceAddress *out);
+/* compute_nums_and_denoms: replaces bridge.h:52 */
+HpMi355xStatus hp_mi355x_v1_mle_compute_nums_and_denoms(
+    HpMi355xContext *ctx, size_t num_vars, const uint8_t *beta, const uint8_t *gamma,
+    const HpMi355xDeviceAddress *fxs, const HpMi355xDeviceAddress *gxs,
+    const HpMi355xDeviceAddress *permutations, size_t count,
+    HpMi355xDeviceAddress *out_numerators, HpMi355xDeviceAddress *out_denominators);
+/* compute_product_factors: replaces bridge.h:53 */
+HpMi355xStatus hp_mi355x_v1_mle_compute_product_factors(HpMi355xContext *ctx,
+                                                        HpMi355xDeviceAddress frac,
+                                                        HpMi355xDeviceAddress product,
+                                                        size_t num_vars,
+                                                        HpMi355xDeviceAddress *out_p1,
+                                                        HpMi355xDeviceAddress *out_p2);
+
+/* ---- multilinear KZG (bridge.h:54-59; backend/src/multilinear_kzg.rs:20-55) */
+/* prepare_prover_param, step 1: replaces bridge.h:54 */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_create(HpMi355xContext *ctx, size_t num_vars,
+                                                          HpMi355xKzgParams **out);
+/* prepare_prover_param, step 2 (per layer): replaces bridge.h:55 */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_upload_layer(HpMi355xContext *ctx,
+                                                                HpMi355xKzgParams *params,
+                                                                size_t layer_index,
+                                                                const uint8_t *points,
+                                                                size_t point_count);
+/* replaces bridge.h:56 */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_destroy(HpMi355xContext *ctx,
+                                                           HpMi355xKzgParams *params);
+/* commit: replaces bridge.h:57 */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_commit(HpMi355xContext *ctx,
+                                                   const HpMi355xKzgParams *params,
+                                                   HpMi355xDeviceAddress polynomial,
+                                                   size_t num_vars, uint8_t *out_commitment);
+/* multi_commit: replaces bridge.h:58 */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_multi_commit(HpMi355xContext *ctx,
+                                                         const HpMi355xKzgParams *params,
+                                                         const HpMi355xDeviceAddress *polynomials,
+                                                         const size_t *num_vars, size_t count,
+                                                         uint8_t *out_commitments);
+/* open: replaces bridge.h:59 */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(HpMi355xContext *ctx,
+                                                 const HpMi355xKzgParams *params,
+                                                 HpMi355xDeviceAddress polynomial, size_t num_vars,
+                                                 const uint8_t *point, uint8_t *out_quotients,
+                                                 uint8_t *out_evaluation);
+
+/* ---- SumCheck (bridge.h:60-62; backend/src/sum_check.rs:17-38) --------- */
+/* prover_init: replaces bridge.h:60 */
+HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
+    HpMi355xContext *ctx, size_t num_vars, size_t max_degree,
+    const HpMi355xDeviceAddress *factors, size_t factor_count, const uint8_t *coefficients,
+    const size_t *product_offsets, size_t product_count, const size_t *factor_indices,
+    size_t index_count, HpMi355xSumCheckState **out);
+/* prove_round_and_update_state: replaces bridge.h:61 */
+HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(HpMi355xContext *ctx,
+                                                        HpMi355xSumCheckState *state,
+                                                        const uint8_t *challenge_or_null,
+                                                        uint8_t *out_evaluations,
+                                                        size_t evaluation_count);
+/* replaces bridge.h:62 */
+HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_destroy(HpMi355xContext *ctx,
+                                                          HpMi355xSumCheckState *state);
+
+/* ---- additive extensions (no counterpart in the reference bridge) ------ */
+/* Copy a resident table back as canonical LE32 values (tests, diagnostics). */
+HpMi355xStatus hp_mi355x_v1_mle_download(HpMi355xContext *ctx, HpMi355xDeviceAddress polynomial,
+                                         size_t num_vars, uint8_t *out_evaluations,
+                                         size_t evaluation_count);
+/* Fill every layer of params on the device from a trapdoor t (num_vars
+ * canonical Fr): layer i = { g * eq(t[i..], x) }, last layer = [g]; the
+ * construction of subroutines/src/pcs/multilinear_kzg/srs.rs:158-220 with
+ * g the standard generator. */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_generate(HpMi355xContext *ctx,
+                                                            HpMi355xKzgParams *params,
+                                                            const uint8_t *t);
+/* Copy one prepared layer back as 97-byte packets (tests, SRS files). */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer(
+    HpMi355xContext *ctx, const HpMi355xKzgParams *params, size_t layer_index, uint8_t *points,
+    size_t point_count);
+/* Run one commit-sized MSM (layer N - num_vars) with HIP events between its
+ * stages on the context stream.  stage_ms (>= 7 floats): digits, sort, tasks,
+ * accumulate, bucket sums, bucket reduction, whole device part; plan_info
+ * (4 u32): window bits c, windows W, buckets B, non-zero digits. */
+HpMi355xStatus hp_mi355x_v1_msm_profile(HpMi355xContext *ctx, const HpMi355xKzgParams *params,
+                                        HpMi355xDeviceAddress polynomial, size_t num_vars,
+                                        float *stage_ms, size_t stage_count,
+                                        uint32_t *plan_info);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -30,3 +30,18 @@ def oracle():
 
     o.lib()
     return o
+
+
+@pytest.fixture(scope="session")
+def backend():
+    """One Mi355xBackend (device 0) shared by the GPU tests."""
+    from hyperplonk_amd.backend import Mi355xBackend
+
+    return Mi355xBackend(0)
+
+
+@pytest.fixture(scope="session")
+def rng_factory():
+    from hyperplonk_amd.synth import SplitMix64Stream
+
+    return SplitMix64Stream

@@ -1,0 +1,167 @@
+"""G1 MSM / multilinear KZG on the GPU.
+
+Pinned three ways: the Python-integer closed-form fixtures
+(tests/golden/golden.json: expected (sum a_i s_i) G over bases a_i G, including
+zero / ones / identity / cancel / alternating / boundary / exception patterns),
+the device SRS against the oracle's SRS layers, and commitments / openings
+against the oracle's Pippenger and KZG open on the same bytes."""
+import pytest
+
+from hyperplonk_amd import InvalidParameters
+from hyperplonk_amd.fields import R_MODULUS as R
+from hyperplonk_amd.fields import fr_from_bytes
+from tests import pyfield as pf
+
+pytestmark = pytest.mark.gpu
+
+
+def H(v):
+    return int(v, 16)
+
+
+@pytest.mark.parametrize("window", [None, 4, 7])
+def test_golden_msm_cases(backend, golden, window, monkeypatch):
+    if window:
+        monkeypatch.setenv("HP_MI355X_MSM_WINDOW", str(window))
+    small = [pf.point_from_hex(p) for p in golden["g1_small_multiples"]]
+
+    def base(a):
+        return small[a] if a <= 300 else pf.aff_neg(small[R - a])
+
+    for case in golden["msm"]:
+        coeffs = [H(c) for c in case["coeffs"]]
+        scalars = [H(s) for s in case["scalars"]]
+        n = len(coeffs)
+        N = max(1, (max(n, 1) - 1).bit_length())
+        size = 1 << N
+        pts = [base(a) for a in coeffs] + [None] * (size - n)
+        layers = [b"".join(pf.packet(p) for p in pts)]
+        for i in range(1, N + 1):
+            layers.append(b"".join(pf.packet(small[1]) for _ in range(1 << (N - i))))
+        pp = backend.prepare_prover_param(N, layers)
+        m = backend.mle_from_evaluations(N, scalars + [0] * (size - n))
+        got = backend.commit(pp, m)
+        assert got == pf.point_from_hex(case["expected"]), (case["name"], window)
+
+
+def test_srs_generation_matches_oracle(backend, oracle, rng_factory):
+    rng = rng_factory(42)
+    N = 6
+    t = fr_from_bytes(rng.fr_bytes(N).tobytes())
+    pp = backend.generate_prover_param(N, t)
+    for i in range(N + 1):
+        assert backend.download_layer(pp, i) == oracle.srs_layer_packets(t[i:], threads=4), i
+
+
+@pytest.fixture(scope="module")
+def srs20(backend, rng_factory):
+    """A device-generated SRS with N = 20 (layer k has 2^(20 - k) points)."""
+    rng = rng_factory(2020)
+    N = 20
+    t = fr_from_bytes(rng.fr_bytes(N).tobytes())
+    return N, t, backend.generate_prover_param(N, t)
+
+
+@pytest.mark.parametrize("nv", [0, 1, 2, 5, 10, 12, 16])
+def test_commit_matches_oracle(backend, oracle, rng_factory, srs20, nv):
+    N, t, pp = srs20
+    rng = rng_factory(600 + nv)
+    tab = rng.fr_bytes(1 << nv)
+    m = backend.mle_from_evaluations(nv, tab)
+    layer = backend.download_layer(pp, N - nv)
+    want = oracle.msm_pippenger_packets(layer, tab.tobytes(), 1 << nv, 8)
+    assert backend.commit_packet(pp, m) == want
+
+
+def test_commit_2_20_closed_form(backend, rng_factory, srs20):
+    """At 2^20 the commitment of f against layer 0 equals g * f(t) (the SRS
+    trapdoor makes layer 0 the eq(t, .) multiples of g): a size-independent check."""
+    N, t, pp = srs20
+    rng = rng_factory(620)
+    tab = rng.fr_bytes(1 << N)
+    m = backend.mle_from_evaluations(N, tab)
+    value = backend.evaluate_mle(m, t)
+    assert backend.commit(pp, m) == pf.aff_mul(pf.G, value)
+
+
+def test_constant_and_special_commits(backend, srs20):
+    N, t, pp = srs20
+    # constant polynomial commits to g * 29 (pcs/multilinear_kzg/mod.rs:342-363 pattern)
+    c = backend.mle_from_evaluations(0, [29])
+    assert backend.commit(pp, c) == pf.aff_mul(pf.G, 29)
+    z = backend.mle_from_evaluations(8, [0] * 256)
+    assert backend.commit(pp, z) is None
+    ones = backend.mle_from_evaluations(4, [R - 1] * 16)
+    value = backend.evaluate_mle(ones, t[N - 4 :])
+    assert backend.commit(pp, ones) == pf.aff_mul(pf.G, value)
+    with pytest.raises(InvalidParameters):
+        backend.commit(pp, backend.mle_from_evaluations(N + 1, bytes(32 << (N + 1))))
+
+
+def test_multi_commit_uneven_matches_single(backend, rng_factory, srs20):
+    N, t, pp = srs20
+    rng = rng_factory(700)
+    nvs = [3, 10, 0, 7, 12, 3, 1, 9, 11, 5, 2, 8, 6]
+    mles = [backend.mle_from_evaluations(nv, rng.fr_bytes(1 << nv)) for nv in nvs]
+    mles.append(mles[1])  # duplicates are kept
+    batch = backend.multi_commit_packets(pp, mles)
+    assert batch == [backend.commit_packet(pp, m) for m in mles]
+    assert backend.multi_commit(pp, []) == []
+
+
+@pytest.mark.parametrize("nv", [0, 1, 4, 9, 14])
+def test_open_matches_oracle(backend, oracle, rng_factory, srs20, nv):
+    N, t, pp = srs20
+    rng = rng_factory(800 + nv)
+    tab = rng.fr_bytes(1 << nv)
+    m = backend.mle_from_evaluations(nv, tab)
+    point = fr_from_bytes(rng.fr_bytes(nv).tobytes()) if nv else []
+    proofs, value = backend.open_packets(pp, m, point)
+    ignored = N - nv + 1
+    layers = b"".join(backend.download_layer(pp, ignored + j) for j in range(nv))
+    want_proofs, want_value = oracle.kzg_open_packets(layers, fr_from_bytes(tab.tobytes()), point, threads=8)
+    assert value == want_value
+    assert proofs == want_proofs
+    with pytest.raises(InvalidParameters):
+        backend.open(pp, m, point + [1])
+
+
+def test_open_known_values(backend, srs20):
+    # pcs/multilinear_kzg/mod.rs:391-426: [2,3,5,7] opened at (0,0), (1,1), (2,3)
+    N, t, pp = srs20
+    f = backend.mle_from_evaluations(2, [2, 3, 5, 7])
+    for point, value in (([0, 0], 2), ([1, 1], 7), ([2, 3], 19)):
+        assert backend.open(pp, f, point)[1] == value
+
+
+def test_upload_layer_rejections(backend):
+    g = pf.G
+    good = pf.packet(g)
+    for bad in (
+        b"\x02" + bytes(96),  # bad tag
+        b"\x01" + b"\x01" + bytes(95),  # identity with coordinates
+        b"\x00" + bytes(96),  # finite-tag (0, 0)
+        pf.packet((1, 1)),  # off-curve
+        b"\x00" + (pf.Q).to_bytes(48, "little") + g[1].to_bytes(48, "little"),  # non-canonical x
+    ):
+        with pytest.raises(InvalidParameters):
+            backend.prepare_prover_param(1, [good + bad, good])
+    with pytest.raises(InvalidParameters):
+        backend.prepare_prover_param(1, [good, good])  # wrong layer size
+    # an on-curve point outside the subgroup is accepted (no subgroup re-check)
+    pp = backend.prepare_prover_param(1, [pf.packet((0, 2)) + good, good])
+    m = backend.mle_from_evaluations(1, [0, 3])
+    assert backend.commit(pp, m) == pf.aff_mul(g, 3)
+
+
+def test_commit_before_layer_upload_is_invalid_state(backend):
+    import ctypes
+
+    raw = ctypes.c_void_p()
+    st = backend.lib.hp_mi355x_v1_multilinear_kzg_params_create(backend.context, 2, ctypes.byref(raw))
+    assert st.kind == 0
+    m = backend.mle_from_evaluations(2, [1, 2, 3, 4])
+    out = ctypes.create_string_buffer(97)
+    st = backend.lib.hp_mi355x_v1_multilinear_kzg_commit(backend.context, raw.value, m.address, 2, out)
+    assert st.kind == 2
+    assert backend.lib.hp_mi355x_v1_multilinear_kzg_params_destroy(backend.context, raw.value).kind == 0

@@ -1,0 +1,234 @@
+"""SumCheck prover state on the GPU, round by round, byte-exact against the
+oracle; cases follow the reference's own tests (backend/src/cpu/sum_check.rs:224-253,
+backend/src/ascend910b3/native/tests/sumcheck_tensor_edge.cpp:164-187) plus the
+degree-5 jellyfish-style gate of the n = 20 workload at a smaller size."""
+import ctypes
+
+import pytest
+
+from hyperplonk_amd import InvalidParameters, InvalidState, VirtualPolynomial, abi
+from hyperplonk_amd.fields import R_MODULUS as R
+from hyperplonk_amd.fields import fr_from_bytes
+from tests import pyfield as pf
+
+pytestmark = pytest.mark.gpu
+
+
+def test_round_lifecycle_known_answer(backend):
+    # backend/src/cpu/sum_check.rs:225-253
+    f = backend.mle_from_evaluations(2, [2, 3, 5, 7])
+    vp = VirtualPolynomial.new_from_mle(f, 2)
+    vp.add_mle_list([f, f], 3)
+    st = backend.prover_init(vp)
+    with pytest.raises(InvalidState):
+        backend.prove_round_and_update_state(st, 2)
+    assert backend.prove_round_and_update_state(st, None) == [101, 194, 317]
+    with pytest.raises(InvalidState):
+        backend.prove_round_and_update_state(st, None)
+    assert backend.prove_round_and_update_state(st, 2) == [56, 261, 616]
+    with pytest.raises(InvalidState):
+        backend.prove_round_and_update_state(st, 2)
+    assert backend.mle_values(f) == [2, 3, 5, 7]
+
+
+EDGE_CASES = [
+    ("n1_d2", 1, 2, 0), ("n2_d2", 2, 2, 0), ("n3_multi", 3, 2, 1), ("n9_tail", 9, 2, 0),
+    ("n12_d2", 12, 2, 0), ("n12_multi", 12, 2, 1), ("n9_mixed", 9, 2, 2), ("n9_d3", 9, 3, 3),
+    ("n2_empty", 2, 2, 4),
+]
+
+
+def edge_inputs(n, kind, seed):
+    rng = pf.SplitMix64(seed)
+    tables = [[rng.field() for _ in range(1 << n)] for _ in range(2)]
+    if n > 1:
+        tables[0][0], tables[0][1] = 0, R - 1
+        tables[1][0], tables[1][1] = R - 1, 1
+    products = {
+        0: [(rng.nonboolean(), [2, 1])],
+        1: [(rng.nonboolean(), [2, 1]), (R - 1, [0, 2]), (rng.nonboolean(), [1, 1])],
+        2: [(3, [0]), (7, [2, 1])],
+        3: [(rng.nonboolean(), [0, 1, 2]), (R - 1, [1])],
+        4: [],
+    }[kind]
+    return tables, products, rng
+
+
+def raw_state(backend, n, degree, handles, products):
+    """Create a state through the C ABI with handle list `handles` (repeats allowed)."""
+    coeffs = b"".join(c.to_bytes(32, "little") for c, _ in products)
+    offsets, idx = [0], []
+    for _, ids in products:
+        idx.extend(ids)
+        offsets.append(len(idx))
+    out = ctypes.c_void_p()
+    st = backend.lib.hp_mi355x_v1_sumcheck_prover_state_create(
+        backend.context, n, degree, abi.ptr_array(handles), len(handles), coeffs or None,
+        abi.size_array(offsets), len(products), abi.size_array(idx), len(idx), ctypes.byref(out),
+    )
+    return st, out.value
+
+
+def round_raw(backend, state, challenge, count):
+    buf = ctypes.create_string_buffer(b"\xa7" * (32 * count), 32 * count)
+    ch = challenge.to_bytes(32, "little") if isinstance(challenge, int) else challenge
+    st = backend.lib.hp_mi355x_v1_sumcheck_prover_state_round(backend.context, state, ch, buf, count)
+    return st, buf.raw
+
+
+@pytest.mark.parametrize("name,n,degree,kind", EDGE_CASES)
+def test_edge_cases_round_by_round(backend, oracle, name, n, degree, kind):
+    tables, products, rng = edge_inputs(n, kind, 0x534354454E534F52 + 17 * n + kind)
+    a = backend.mle_from_evaluations(n, tables[0])
+    b = backend.mle_from_evaluations(n, tables[1])
+    st, state = raw_state(backend, n, degree, [a.address, b.address, a.address], products)
+    assert st.kind == 0, name
+    # the state owns its inputs: retire the originals before round zero
+    del a, b
+    ost = oracle.SumCheckState(n, degree, tables + [tables[0]], products)
+    ch = rng.nonboolean()
+    # premature challenge / wrong count / null output leave the state untouched
+    s, out = round_raw(backend, state, ch, degree + 1)
+    assert s.kind == abi.INVALID_STATE and out == b"\xa7" * len(out)
+    s, _ = round_raw(backend, state, None, degree)
+    assert s.kind == abi.INVALID_PARAMETERS
+    s = backend.lib.hp_mi355x_v1_sumcheck_prover_state_round(backend.context, state, None, None, degree + 1)
+    assert s.kind == abi.INVALID_PARAMETERS
+    for rnd in range(n):
+        c = None
+        if rnd:
+            s, _ = round_raw(backend, state, None, degree + 1)
+            assert s.kind == abi.INVALID_STATE
+            s, _ = round_raw(backend, state, b"\xff" * 32, degree + 1)
+            assert s.kind == abi.INVALID_PARAMETERS
+            c = rng.nonboolean()
+        s, out = round_raw(backend, state, c, degree + 1)
+        assert s.kind == 0, (name, rnd)
+        assert fr_from_bytes(out) == ost.round(c), (name, rnd)
+    s, _ = round_raw(backend, state, 5, degree + 1)
+    assert s.kind == abi.INVALID_STATE
+    assert backend.lib.hp_mi355x_v1_sumcheck_prover_state_destroy(backend.context, state).kind == 0
+
+
+def test_create_rejects_bad_metadata(backend):
+    a = backend.mle_from_evaluations(2, [1, 2, 3, 4])
+    h = [a.address]
+    for n, d, prods in ((0, 2, [(1, [0])]), (2, 0, [(1, [0])]), (2, 1, [(1, [0, 0])]), (2, 2, [(1, [1])])):
+        st, _ = raw_state(backend, n, d, h, prods)
+        assert st.kind == abi.INVALID_PARAMETERS
+    st, _ = raw_state(backend, 3, 2, h, [(1, [0])])  # dimension mismatch
+    assert st.kind == abi.INVALID_PARAMETERS
+    st, _ = raw_state(backend, 2, 2, h, [(R, [0])])  # non-canonical coefficient
+    assert st.kind == abi.INVALID_PARAMETERS
+
+
+def run_vp(backend, oracle, nv, degree, tables, products, seed, check_every=True):
+    mles = [backend.mle_from_evaluations(nv, t) for t in tables]
+    vp = VirtualPolynomial(nv)
+    for c, idx in products:
+        vp.add_mle_list([mles[i] for i in idx], c)
+    assert vp.aux_info.max_degree == degree
+    st = backend.prover_init(vp)
+    ints = [fr_from_bytes(t.tobytes()) for t in tables] if check_every else None
+    ost = oracle.SumCheckState(nv, degree, ints, products) if check_every else None
+    rng = pf.SplitMix64(seed)
+    c = None
+    msgs = []
+    for _ in range(nv):
+        got = backend.prove_round_and_update_state(st, c)
+        if check_every:
+            assert got == ost.round(c)
+        msgs.append((c, got))
+        c = rng.nonboolean()
+    return msgs, c, mles
+
+
+def jellyfish_products(rng, ids):
+    """Degree-5 gate over 5 wires and 9 selectors plus a scalar q_H w1^5 term."""
+    q1, q2, q3, q4, qm1, qm2, qe, qo, qc, w1, w2, w3, w4, w5 = ids
+    return [
+        (1, [q1, w1]), (1, [q2, w2]), (1, [q3, w3]), (1, [q4, w4]), (1, [qm1, w1, w2]),
+        (1, [qm2, w3, w4]), (1, [qe, w1, w2, w3, w4]), (R - 1, [qo, w5]), (1, [qc]),
+        (rng.nonboolean(), [w1, w1, w1, w1, w1]),
+    ]
+
+
+@pytest.mark.parametrize("nv", [6, 13])
+def test_degree2_and_degree5_match_oracle(backend, oracle, rng_factory, nv):
+    rng = rng_factory(300 + nv)
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(6)]
+    prods = [(fr_from_bytes(rng.fr_bytes(1).tobytes())[0], [2 * i, 2 * i + 1]) for i in range(3)]
+    run_vp(backend, oracle, nv, 2, tabs, prods, 11)
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(14)]
+    run_vp(backend, oracle, nv, 5, tabs, jellyfish_products(pf.SplitMix64(5), list(range(14))), 12)
+
+
+@pytest.mark.parametrize("degree", [7, 9, 34])
+def test_high_degree_products(backend, oracle, rng_factory, degree):
+    # degrees beyond the register-resident kernel (cf. sum_check/mod.rs:663-914 degrees {2,3,7,34})
+    nv = 5
+    rng = rng_factory(400 + degree)
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(3)]
+    prods = [(3, [i % 3 for i in range(degree)]), (5, [1, 2]), (R - 2, [0])]
+    run_vp(backend, oracle, nv, degree, tabs, prods, 13)
+
+
+@pytest.mark.parametrize("degree", [2, 5])
+def test_n20_invariants(backend, rng_factory, degree):
+    """Full-size (n = 20) check through size-independent properties: each
+    round's g(0) + g(1) equals the previous round polynomial at the challenge,
+    and the final polynomial at the last challenge equals the virtual
+    polynomial evaluated at the challenge point (per-MLE device evaluations)."""
+    nv = 20
+    rng = rng_factory(500 + degree)
+    k = 6 if degree == 2 else 14
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(k)]
+    if degree == 2:
+        prods = [(fr_from_bytes(rng.fr_bytes(1).tobytes())[0], [2 * i, 2 * i + 1]) for i in range(3)]
+    else:
+        prods = jellyfish_products(pf.SplitMix64(9), list(range(14)))
+    mles = [backend.mle_from_evaluations(nv, t) for t in tabs]
+    vp = VirtualPolynomial(nv)
+    for c, idx in prods:
+        vp.add_mle_list([mles[i] for i in idx], c)
+    st = backend.prover_init(vp)
+    srng = pf.SplitMix64(77)
+    prev, point = None, []
+    c = None
+    for rnd in range(nv):
+        got = backend.prove_round_and_update_state(st, c)
+        assert len(got) == degree + 1
+        if prev is not None:
+            assert (got[0] + got[1]) % R == interpolate(prev, c)
+        prev = got
+        c = srng.nonboolean()
+        point.append(c)
+    evals = [backend.evaluate_mle(m, point) for m in mles]
+    expect = 0
+    for coeff, idx in prods:
+        term = coeff
+        for i in idx:
+            term = term * evals[i] % R
+        expect += term
+    assert interpolate(prev, point[-1]) == expect % R
+
+
+def interpolate(values, x):
+    n = len(values)
+    total = 0
+    for j in range(n):
+        num, den = 1, 1
+        for m in range(n):
+            if m != j:
+                num = num * (x - m) % R
+                den = den * (j - m) % R
+        total += values[j] * num * pow(den, R - 2, R)
+    return total % R
+
+
+def test_invalid_challenge_rejected_by_host_mirror(backend):
+    f = backend.mle_from_evaluations(2, [2, 3, 5, 7])
+    st = backend.prover_init(VirtualPolynomial.new_from_mle(f, 1))
+    backend.prove_round_and_update_state(st, None)
+    with pytest.raises(InvalidParameters):
+        backend.prove_round_and_update_state(st, R)
