@@ -1,0 +1,186 @@
+// Host-side G1 arithmetic on 6 x 64-bit limbs for the MSM's last steps.
+//
+// The device keeps Fq in Montgomery form with R = 2^384 on 12 x 32-bit limbs;
+// the same little-endian bytes read as 6 x 64-bit limbs are the same
+// Montgomery representation (R = 2^384 either way), so device XYZZ results
+// are used here without conversion.  The host finishes what is inherently
+// sequential and short (combining the last bucket-reduction groups, the
+// final affine normalisation), where one CPU core with 64 x 64 -> 128-bit
+// products beats a single GPU wavefront by an order of magnitude.
+#pragma once
+#include <cstdint>
+#include <cstring>
+
+#include "curve.hpp"
+
+namespace hp {
+namespace host {
+
+struct Fq64 {
+  uint64_t v[6];
+};
+
+constexpr uint64_t kQ[6] = {0xb9feffffffffaaabull, 0x1eabfffeb153ffffull, 0x6730d2a0f6b0f624ull,
+                            0x64774b84f38512bfull, 0x4b1ba7b6434bacd7ull, 0x1a0111ea397fe69aull};
+constexpr uint64_t kQInv = 0x89f3fffcfffcfffdull;  // -q^{-1} mod 2^64
+
+inline bool is_zero(const Fq64& a) {
+  return (a.v[0] | a.v[1] | a.v[2] | a.v[3] | a.v[4] | a.v[5]) == 0;
+}
+
+inline void cond_sub_q(uint64_t t[6]) {
+  uint64_t d[6];
+  unsigned __int128 borrow = 0;
+  for (int i = 0; i < 6; ++i) {
+    unsigned __int128 x = (unsigned __int128)t[i] - kQ[i] - (uint64_t)borrow;
+    d[i] = (uint64_t)x;
+    borrow = (x >> 64) & 1;
+  }
+  if (!borrow) std::memcpy(t, d, sizeof d);
+}
+
+inline Fq64 add(const Fq64& a, const Fq64& b) {
+  Fq64 r;
+  unsigned __int128 c = 0;
+  for (int i = 0; i < 6; ++i) {
+    c += (unsigned __int128)a.v[i] + b.v[i];
+    r.v[i] = (uint64_t)c;
+    c >>= 64;
+  }
+  cond_sub_q(r.v);
+  return r;
+}
+
+inline Fq64 sub(const Fq64& a, const Fq64& b) {
+  Fq64 r;
+  unsigned __int128 borrow = 0;
+  for (int i = 0; i < 6; ++i) {
+    unsigned __int128 x = (unsigned __int128)a.v[i] - b.v[i] - (uint64_t)borrow;
+    r.v[i] = (uint64_t)x;
+    borrow = (x >> 64) & 1;
+  }
+  if (borrow) {
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 6; ++i) {
+      c += (unsigned __int128)r.v[i] + kQ[i];
+      r.v[i] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+  return r;
+}
+
+// CIOS Montgomery product (q < 2^382 leaves room for the lazy top word).
+inline Fq64 mul(const Fq64& a, const Fq64& b) {
+  uint64_t t[8] = {0};
+  for (int i = 0; i < 6; ++i) {
+    unsigned __int128 c = 0;
+    for (int j = 0; j < 6; ++j) {
+      c += (unsigned __int128)a.v[j] * b.v[i] + t[j];
+      t[j] = (uint64_t)c;
+      c >>= 64;
+    }
+    c += t[6];
+    t[6] = (uint64_t)c;
+    t[7] = (uint64_t)(c >> 64);
+    const uint64_t m = t[0] * kQInv;
+    c = (unsigned __int128)m * kQ[0] + t[0];
+    c >>= 64;
+    for (int j = 1; j < 6; ++j) {
+      c += (unsigned __int128)m * kQ[j] + t[j];
+      t[j - 1] = (uint64_t)c;
+      c >>= 64;
+    }
+    c += t[6];
+    t[5] = (uint64_t)c;
+    t[6] = t[7] + (uint64_t)(c >> 64);
+  }
+  Fq64 r;
+  std::memcpy(r.v, t, sizeof r.v);
+  cond_sub_q(r.v);
+  return r;
+}
+
+// a^(q - 2) (zero maps to zero).
+inline Fq64 inverse(const Fq64& a) {
+  uint64_t e[6];
+  unsigned __int128 borrow = 2;
+  for (int i = 0; i < 6; ++i) {
+    unsigned __int128 x = (unsigned __int128)kQ[i] - (uint64_t)borrow;
+    e[i] = (uint64_t)x;
+    borrow = (x >> 64) & 1;
+  }
+  // Montgomery one = 2^384 mod q
+  Fq64 acc{{0x760900000002fffdull, 0xebf4000bc40c0002ull, 0x5f48985753c758baull,
+            0x77ce585370525745ull, 0x5c071a97a256ec6dull, 0x15f65ec3fa80e493ull}};
+  for (int i = 5; i >= 0; --i)
+    for (int b = 63; b >= 0; --b) {
+      acc = mul(acc, acc);
+      if ((e[i] >> b) & 1) acc = mul(acc, a);
+    }
+  return acc;
+}
+
+inline Fq64 from_mont(const Fq64& a) {
+  Fq64 one{{1, 0, 0, 0, 0, 0}};
+  return mul(a, one);
+}
+
+struct Xyzz64 {
+  Fq64 X, Y, ZZ, ZZZ;
+  bool is_identity() const { return is_zero(ZZ); }
+};
+
+inline Xyzz64 from_device(const G1Xyzz& p) {
+  Xyzz64 r;
+  std::memcpy(&r, &p, sizeof r);
+  return r;
+}
+
+inline G1Xyzz to_device(const Xyzz64& p) {
+  G1Xyzz r;
+  std::memcpy(&r, &p, sizeof r);
+  return r;
+}
+
+inline Xyzz64 dbl(const Xyzz64& p) {
+  if (p.is_identity() || is_zero(p.Y)) return Xyzz64{{}, {}, {}, {}};
+  const Fq64 U = add(p.Y, p.Y), V = mul(U, U), W = mul(U, V), S = mul(p.X, V);
+  const Fq64 xx = mul(p.X, p.X), M = add(add(xx, xx), xx);
+  const Fq64 X3 = sub(sub(mul(M, M), S), S);
+  const Fq64 Y3 = sub(mul(M, sub(S, X3)), mul(W, p.Y));
+  return {X3, Y3, mul(V, p.ZZ), mul(W, p.ZZZ)};
+}
+
+inline Xyzz64 add(const Xyzz64& p, const Xyzz64& q) {
+  if (q.is_identity()) return p;
+  if (p.is_identity()) return q;
+  const Fq64 U1 = mul(p.X, q.ZZ), U2 = mul(q.X, p.ZZ);
+  const Fq64 S1 = mul(p.Y, q.ZZZ), S2 = mul(q.Y, p.ZZZ);
+  const Fq64 P = sub(U2, U1), R = sub(S2, S1);
+  if (is_zero(P)) {
+    if (is_zero(R)) return dbl(p);
+    return Xyzz64{{}, {}, {}, {}};
+  }
+  const Fq64 PP = mul(P, P), PPP = mul(P, PP), Q = mul(U1, PP);
+  const Fq64 X3 = sub(sub(sub(mul(R, R), PPP), Q), Q);
+  const Fq64 Y3 = sub(mul(R, sub(Q, X3)), mul(S1, PPP));
+  return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
+}
+
+// Canonical 97-byte packet of an XYZZ point (identity -> tag 1).
+inline void to_packet(const Xyzz64& p, uint8_t* out) {
+  std::memset(out, 0, 97);
+  if (p.is_identity()) {
+    out[0] = 1;
+    return;
+  }
+  const Fq64 inv = inverse(mul(p.ZZ, p.ZZZ));
+  const Fq64 x = from_mont(mul(p.X, mul(inv, p.ZZZ)));
+  const Fq64 y = from_mont(mul(p.Y, mul(inv, p.ZZ)));
+  std::memcpy(out + 1, x.v, 48);
+  std::memcpy(out + 49, y.v, 48);
+}
+
+}  // namespace host
+}  // namespace hp

@@ -62,10 +62,24 @@ kzg::Params* bind(HpMi355xContext* ctx, rt::Context& c, const HpMi355xKzgParams*
   return params->impl.get();
 }
 
-void finish_points(rt::Context& c, const G1Xyzz* dev, size_t count, uint8_t* out97) {
-  std::vector<G1Xyzz> host(count);
-  rt::download_bytes(c, dev, count * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
-  for (size_t i = 0; i < count; ++i) msm::to_packet(host[i], out97 + rt::kG1Bytes * i);
+// Device result slots of several queued MSMs, finished on the host.
+struct MsmBatch {
+  std::vector<const msm::Layer*> layers;
+  std::vector<size_t> offsets;
+  size_t slots = 0;
+  void add(const msm::Layer* l) {
+    layers.push_back(l);
+    offsets.push_back(slots);
+    slots += msm::result_slots(*l);
+  }
+};
+
+void finish_points(rt::Context& c, const MsmBatch& batch, const G1Xyzz* dev, uint8_t* out97) {
+  std::vector<G1Xyzz> host(batch.slots);
+  rt::download_bytes(c, dev, batch.slots * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
+  for (size_t i = 0; i < batch.layers.size(); ++i)
+    msm::to_packet(msm::finish(*batch.layers[i], host.data() + batch.offsets[i]),
+                   out97 + rt::kG1Bytes * i);
 }
 
 }  // namespace
@@ -177,10 +191,12 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_commit(HpMi355xContext* c
     if (!f) return rt::invalid_parameters();
     const size_t layer = p->num_vars - num_vars;
     if (!p->ready[layer]) return rt::invalid_state();
-    rt::BufferRef res = c.allocate(sizeof(G1Xyzz));
+    MsmBatch batch;
+    batch.add(&p->layers[layer]);
+    rt::BufferRef res = c.allocate(batch.slots * sizeof(G1Xyzz));
     msm::enqueue(c, p->layers[layer], static_cast<const Fr*>(f->ptr),
                  static_cast<G1Xyzz*>(res->ptr));
-    finish_points(c, static_cast<const G1Xyzz*>(res->ptr), 1, out_commitment);
+    finish_points(c, batch, static_cast<const G1Xyzz*>(res->ptr), out_commitment);
     return rt::ok();
   });
 }
@@ -203,12 +219,14 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_multi_commit(
     }
     for (size_t i = 0; i < count; ++i)
       if (!p->ready[p->num_vars - num_vars[i]]) return rt::invalid_state();
-    rt::BufferRef res = c.allocate(count * sizeof(G1Xyzz));
+    MsmBatch batch;
+    for (size_t i = 0; i < count; ++i) batch.add(&p->layers[p->num_vars - num_vars[i]]);
+    rt::BufferRef res = c.allocate(batch.slots * sizeof(G1Xyzz));
     G1Xyzz* r = static_cast<G1Xyzz*>(res->ptr);
     for (size_t i = 0; i < count; ++i)
-      msm::enqueue(c, p->layers[p->num_vars - num_vars[i]],
-                   static_cast<const Fr*>(tables[i]->ptr), r + i);
-    finish_points(c, r, count, out_commitments);
+      msm::enqueue(c, *batch.layers[i], static_cast<const Fr*>(tables[i]->ptr),
+                   r + batch.offsets[i]);
+    finish_points(c, batch, r, out_commitments);
     return rt::ok();
   });
 }
@@ -239,7 +257,9 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
     Fr* fa = static_cast<Fr*>(work->ptr);
     Fr* fb = fa + n / 2;
     Fr* q = fb + n / 4 + 1;
-    rt::BufferRef res = c.allocate(num_vars * sizeof(G1Xyzz));
+    MsmBatch batch;
+    for (size_t j = 0; j < num_vars; ++j) batch.add(&p->layers[N - num_vars + 1 + j]);
+    rt::BufferRef res = c.allocate(batch.slots * sizeof(G1Xyzz));
     G1Xyzz* r = static_cast<G1Xyzz*>(res->ptr);
     const Fr* src = static_cast<const Fr*>(f->ptr);
     for (size_t j = 0; j < num_vars; ++j) {
@@ -248,10 +268,10 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
       hipLaunchKernelGGL(kzg::k_fold_quotient, dim3(kzg::grid_for(half)), dim3(256), 0, c.stream,
                          src, dst, q, static_cast<const Fr*>(dpoint->ptr) + j, half);
       rt::check_launch();
-      msm::enqueue(c, p->layers[N - num_vars + 1 + j], q, r + j);
+      msm::enqueue(c, *batch.layers[j], q, r + batch.offsets[j]);
       src = dst;
     }
-    finish_points(c, r, num_vars, out_quotients);
+    finish_points(c, batch, r, out_quotients);
     rt::download_fr(c, src, 1, out_evaluation);
     return rt::ok();
   });
@@ -273,8 +293,9 @@ extern "C" HpMi355xStatus hp_mi355x_v1_msm_profile(HpMi355xContext* ctx,
     if (!p->ready[layer]) return rt::invalid_state();
     msm::StageEvents ev;
     for (auto& e : ev.marks) rt::hip_check(hipEventCreate(&e));
-    rt::BufferRef res = c.allocate(sizeof(G1Xyzz) + sizeof(uint32_t));
-    ev.entries_out = reinterpret_cast<uint32_t*>(static_cast<G1Xyzz*>(res->ptr) + 1);
+    const size_t slots = msm::result_slots(p->layers[layer]);
+    rt::BufferRef res = c.allocate(slots * sizeof(G1Xyzz) + sizeof(uint32_t));
+    ev.entries_out = reinterpret_cast<uint32_t*>(static_cast<G1Xyzz*>(res->ptr) + slots);
     msm::enqueue(c, p->layers[layer], static_cast<const Fr*>(f->ptr),
                  static_cast<G1Xyzz*>(res->ptr), &ev);
     c.sync();

@@ -1,20 +1,22 @@
 // Pippenger G1 MSM on gfx950 against a prepared, window-shifted SRS layer.
 //
-// Stages (all on the context stream; the host only finishes the last
-// affine normalisation):
-//   1. digits    one thread per scalar: Montgomery -> canonical, signed c-bit
-//                digits for W windows; entry (bucket |d|-1, base index w*n+i,
-//                sign) per non-zero digit.
-//   2. sort      counting sort by bucket: histogram (atomics), exclusive scan,
-//                scatter of base references into bucket-contiguous order.
-//   3. tasks     each bucket is cut into tasks of <= task_cap entries.
-//   4. accumulate one wavefront per task: lanes stride over the entries with
+// Stages (all on the context stream; the host finishes the last level):
+//   1. sort      counting sort of the signed c-bit digits by bucket, without
+//                materialising digits: blocks (bucket range r, scalar chunk e)
+//                recompute every digit of their chunk (Montgomery -> canonical,
+//                W windows), count those in their range with LDS atomics
+//                (8192 buckets per range), an exclusive scan turns the
+//                (bucket, chunk) counts into offsets, and a second pass
+//                scatters base references (w n + i | sign) into
+//                bucket-contiguous order.
+//   2. tasks     each bucket is cut into tasks of <= task_cap entries.
+//   3. accumulate one wavefront per task: lanes stride over the entries with
 //                XYZZ += affine (8M + 2S), then a shuffle tree reduction.
-//   5. buckets   per-bucket sum of its task partials.
-//   6. reduce    sum_b (b + 1) S_b: blocks of 256 buckets run an LDS suffix
+//   4. buckets   per-bucket sum of its task partials.
+//   5. reduce    sum_b (b + 1) S_b: blocks of 256 buckets run an LDS suffix
 //                scan (U_t = sum_{t' >= t} S_t') and a tree sum of U, giving
-//                (A, R) per block; higher levels combine (A, R) pairs with the
-//                same scan, scaled by the block span; one pair remains.
+//                (A, R) per block; the host adds the blocks with their
+//                weights (sum A_k + 256 sum k R_k) on 64-bit limbs.
 // The group law is exact in every case, so the result does not depend on the
 // order of accumulation.  Work per 2^20 scalars at c = 16: W = 16 windows,
 // ~16.8M mixed additions, 2^15 buckets.
@@ -29,6 +31,7 @@
 
 #include "kernels_common.hpp"
 #include "mle.hpp"
+#include "host_g1.hpp"
 #include "msm.hpp"
 
 namespace hp {
@@ -48,48 +51,88 @@ __device__ __forceinline__ uint32_t scalar_bits(const uint32_t* s, uint32_t pos,
   return (uint32_t)(w >> sh) & ((1u << c) - 1u);
 }
 
-__global__ void k_digits(const Fr* scalars, size_t n, uint32_t c, uint32_t windows,
-                         uint32_t* keys, uint32_t* vals) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
+// Signed c-bit digit of window w given the running carry; returns the bucket
+// key (|d| - 1, or kSkip for a zero digit) and sets *neg for a negative digit.
+__device__ __forceinline__ uint32_t window_digit(const uint32_t* s, uint32_t w, uint32_t c,
+                                                 uint32_t& carry, bool& neg) {
+  const uint32_t half = 1u << (c - 1), full = 1u << c;
+  const uint32_t raw = scalar_bits(s, w * c, c) + carry;
+  if (raw > half) {
+    carry = 1;
+    neg = true;
+    const uint32_t mag = full - raw;
+    return mag ? mag - 1 : kSkip;
+  }
+  carry = 0;
+  neg = false;
+  return raw ? raw - 1 : kSkip;
+}
+
+// Buckets are counted and scattered in ranges of kRangeBuckets so that the
+// per-block counters live in LDS (LDS atomics instead of global atomics).
+// Block (r, e) handles bucket range r and scalar chunk e; digits are
+// recomputed from the scalars in both passes, so no key/value arrays exist.
+constexpr uint32_t kRangeBuckets = 8192;
+constexpr unsigned kSortThreads = 1024;
+
+// counts[b * chunks + e] = number of digits of chunk e in bucket b.
+__global__ void __launch_bounds__(kSortThreads)
+    k_count_ranges(const Fr* scalars, size_t n, uint32_t c, uint32_t windows, uint32_t buckets,
+                   uint32_t chunks, uint32_t* counts) {
+  __shared__ uint32_t cnt[kRangeBuckets];
+  const uint32_t b0 = blockIdx.x * kRangeBuckets;
+  const uint32_t span = min(buckets - b0, kRangeBuckets);
+  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) cnt[b] = 0;
+  __syncthreads();
+  const size_t i0 = n * blockIdx.y / chunks, i1 = n * (blockIdx.y + 1) / chunks;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     const Fr s = from_mont(load_fr(scalars, i));
     uint32_t carry = 0;
-    const uint32_t half = 1u << (c - 1), full = 1u << c;
     for (uint32_t w = 0; w < windows; ++w) {
-      uint32_t raw = scalar_bits(s.v, w * c, c) + carry;
-      uint32_t key = kSkip, val = 0;
-      if (raw > half) {
-        carry = 1;
-        const uint32_t mag = full - raw;  // digit = raw - 2^c < 0
-        key = mag - 1;
-        val = (uint32_t)(w * n + i) | 0x80000000u;
-      } else {
-        carry = 0;
-        if (raw != 0) {
-          key = raw - 1;
-          val = (uint32_t)(w * n + i);
-        }
+      bool neg;
+      const uint32_t key = window_digit(s.v, w, c, carry, neg);
+      if (key - b0 < span) atomicAdd(&cnt[key - b0], 1u);  // kSkip never lands in range
+    }
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x)
+    counts[(size_t)(b0 + b) * chunks + blockIdx.y] = cnt[b];
+}
+
+// sorted[offsets[b * chunks + e] + rank] = (w * n + i) | sign << 31
+__global__ void __launch_bounds__(kSortThreads)
+    k_scatter_ranges(const Fr* scalars, size_t n, uint32_t c, uint32_t windows, uint32_t buckets,
+                     uint32_t chunks, const uint32_t* offsets, uint32_t* sorted) {
+  __shared__ uint32_t cur[kRangeBuckets];
+  const uint32_t b0 = blockIdx.x * kRangeBuckets;
+  const uint32_t span = min(buckets - b0, kRangeBuckets);
+  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x)
+    cur[b] = offsets[(size_t)(b0 + b) * chunks + blockIdx.y];
+  __syncthreads();
+  const size_t i0 = n * blockIdx.y / chunks, i1 = n * (blockIdx.y + 1) / chunks;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const Fr s = from_mont(load_fr(scalars, i));
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < windows; ++w) {
+      bool neg;
+      const uint32_t key = window_digit(s.v, w, c, carry, neg);
+      if (key - b0 < span) {
+        const uint32_t pos = atomicAdd(&cur[key - b0], 1u);
+        sorted[pos] = (uint32_t)(w * n + i) | (neg ? 0x80000000u : 0u);
       }
-      keys[w * n + i] = key;
-      vals[w * n + i] = val;
     }
   }
 }
 
-__global__ void k_histogram(const uint32_t* keys, size_t m, uint32_t* counts) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t k = keys[i];
-    if (k != kSkip) atomicAdd(&counts[k], 1u);
-  }
-}
-
-__global__ void k_scatter(const uint32_t* keys, const uint32_t* vals, size_t m, uint32_t* cursor,
-                          uint32_t* sorted) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t k = keys[i];
-    if (k != kSkip) sorted[atomicAdd(&cursor[k], 1u)] = vals[i];
+// Per-bucket start and count from the (bucket, chunk) exclusive offsets.
+__global__ void k_bucket_ranges(const uint32_t* offsets, uint32_t buckets, uint32_t chunks,
+                                const uint32_t* total, uint32_t* starts, uint32_t* counts) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
+       b += gridDim.x * blockDim.x) {
+    const uint32_t s = offsets[(size_t)b * chunks];
+    const uint32_t e = b + 1 < buckets ? offsets[(size_t)(b + 1) * chunks] : *total;
+    starts[b] = s;
+    counts[b] = e - s;
   }
 }
 
@@ -253,49 +296,6 @@ __global__ void __launch_bounds__(kRed1) k_reduce_buckets(const G1Xyzz* sums, ui
   if (t == 0) store_xyzz(pair_a, blockIdx.x, v);
 }
 
-// Higher levels: blocks of 128 pairs (A_k, R_k) whose buckets span `span`
-// (= 2^span_log2) each: A' = sum A_k + span * sum_k k R_k (k relative to the
-// block), R' = sum R_k.
-constexpr unsigned kRed2 = 128;
-__global__ void __launch_bounds__(kRed2) k_reduce_pairs(const G1Xyzz* in_a, const G1Xyzz* in_r,
-                                                        uint32_t count, uint32_t span_log2,
-                                                        G1Xyzz* out_a, G1Xyzz* out_r) {
-  __shared__ G1Xyzz u[kRed2];
-  __shared__ G1Xyzz a[kRed2];
-  const uint32_t t = threadIdx.x;
-  const uint32_t k = blockIdx.x * kRed2 + t;
-  G1Xyzz v = k < count ? load_xyzz(in_r, k) : G1Xyzz::identity();
-  G1Xyzz w = k < count ? load_xyzz(in_a, k) : G1Xyzz::identity();
-  u[t] = v;
-  a[t] = w;
-  __syncthreads();
-  for (unsigned d = 1; d < kRed2; d <<= 1) {
-    if (t + d < kRed2) v = add(v, u[t + d]);
-    __syncthreads();
-    u[t] = v;
-    __syncthreads();
-  }
-  if (t == 0) {
-    store_xyzz(out_r, blockIdx.x, v);
-    v = G1Xyzz::identity();  // sum_{t >= 1} U_t = sum_t t R_t
-    u[0] = v;
-  }
-  __syncthreads();
-  for (unsigned d = kRed2 / 2; d > 0; d >>= 1) {
-    if (t < d) {
-      v = add(v, u[t + d]);
-      u[t] = v;
-      w = add(w, a[t + d]);
-      a[t] = w;
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    for (uint32_t s = 0; s < span_log2; ++s) v = dbl(v);
-    store_xyzz(out_a, blockIdx.x, add(w, v));
-  }
-}
-
 // ---- SRS preparation --------------------------------------------------------
 // next[i] = 2^c prev[i], normalised with one inversion per thread
 // (Montgomery batch inversion over the thread's strided points).
@@ -417,17 +417,28 @@ unsigned grid_for(size_t n, unsigned threads = 256, size_t cap = 65535) {
   return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, cap)));
 }
 
-// Exclusive scan of n uint32 values (n <= 4M); optional grand total.
+// Exclusive scan of n uint32 values; optional grand total.  Tile sums are
+// scanned recursively when there are more than one tile's worth of tiles.
 void exclusive_scan(rt::Context& c, const uint32_t* in, uint32_t* out, size_t n,
-                    uint32_t* total_out, rt::BufferRef& scratch) {
+                    uint32_t* total_out, std::vector<rt::BufferRef>& keep) {
   const size_t tiles = (n + kScanTile - 1) / kScanTile;
-  scratch = c.allocate((tiles + 1) * sizeof(uint32_t));
-  uint32_t* sums = static_cast<uint32_t*>(scratch->ptr);
+  rt::BufferRef sums_buf = c.allocate((tiles + 1) * sizeof(uint32_t));
+  keep.push_back(sums_buf);
+  uint32_t* sums = static_cast<uint32_t*>(sums_buf->ptr);
   hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, c.stream, in, out,
                      n, sums);
   rt::check_launch();
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, c.stream, sums, tiles);
-  rt::check_launch();
+  if (tiles <= kScanTile) {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, c.stream, sums, tiles);
+    rt::check_launch();
+  } else {
+    // scan the tile sums in place (recursively), total to sums[tiles]
+    rt::BufferRef copy = c.allocate(tiles * sizeof(uint32_t));
+    keep.push_back(copy);
+    rt::hip_check(hipMemcpyAsync(copy->ptr, sums, tiles * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, c.stream));
+    exclusive_scan(c, static_cast<const uint32_t*>(copy->ptr), sums, tiles, sums + tiles, keep);
+  }
   hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c.stream, out, n,
                      sums, total_out, tiles);
   rt::check_launch();
@@ -528,6 +539,10 @@ bool upload_layer(rt::Context& c, Layer& layer, const uint8_t* packets, size_t n
   return true;
 }
 
+size_t result_slots(const Layer& layer) {
+  return 2 * ((layer.plan.buckets + kRed1 - 1) / kRed1);
+}
+
 void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* result,
              StageEvents* ev) {
   auto mark = [&](int i) {
@@ -537,45 +552,43 @@ void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* resu
   const size_t n = layer.n;
   const size_t m = n * p.windows;
   const uint32_t B = p.buckets;
-  rt::BufferRef kv = c.allocate(2 * m * sizeof(uint32_t));
-  uint32_t* keys = static_cast<uint32_t*>(kv->ptr);
-  uint32_t* vals = keys + m;
+  const uint32_t chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, n / 4096));
+  const uint32_t ranges = (B + kRangeBuckets - 1) / kRangeBuckets;
+  std::vector<rt::BufferRef> keep;
   rt::BufferRef sorted_buf = c.allocate(std::max<size_t>(m, 1) * sizeof(uint32_t));
   uint32_t* sorted = static_cast<uint32_t*>(sorted_buf->ptr);
-  // per-bucket arrays: counts | offsets | cursor | ntask | task_off | totals(2)
-  rt::BufferRef meta = c.allocate((5 * (size_t)B + 4) * sizeof(uint32_t));
-  uint32_t* counts = static_cast<uint32_t*>(meta->ptr);
-  uint32_t* offsets = counts + B;
-  uint32_t* cursor = offsets + B;
-  uint32_t* ntask = cursor + B;
+  // (bucket, chunk) counts and offsets | per-bucket start, count, ntask, task_off | totals
+  const size_t bc = (size_t)B * chunks;
+  rt::BufferRef meta = c.allocate((2 * bc + 4 * (size_t)B + 4) * sizeof(uint32_t));
+  uint32_t* cc_counts = static_cast<uint32_t*>(meta->ptr);
+  uint32_t* cc_offsets = cc_counts + bc;
+  uint32_t* starts = cc_offsets + bc;
+  uint32_t* counts = starts + B;
+  uint32_t* ntask = counts + B;
   uint32_t* task_off = ntask + B;
   uint32_t* totals = task_off + B;  // [0] entries, [1] tasks
 
   mark(0);
-  hipLaunchKernelGGL(k_digits, dim3(grid_for(n)), dim3(256), 0, c.stream, scalars, n, p.c,
-                     p.windows, keys, vals);
-  rt::check_launch();
   mark(1);
-  zero_u32(c, counts, B);
-  hipLaunchKernelGGL(k_histogram, dim3(grid_for(m, 256, 16384)), dim3(256), 0, c.stream, keys, m,
-                     counts);
+  hipLaunchKernelGGL(k_count_ranges, dim3(ranges, chunks), dim3(kSortThreads), 0, c.stream,
+                     scalars, n, p.c, p.windows, B, chunks, cc_counts);
   rt::check_launch();
-  rt::BufferRef scan_a, scan_b, scan_c;
-  exclusive_scan(c, counts, offsets, B, totals, scan_a);
-  rt::hip_check(hipMemcpyAsync(cursor, offsets, B * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                               c.stream));
-  hipLaunchKernelGGL(k_scatter, dim3(grid_for(m, 256, 16384)), dim3(256), 0, c.stream, keys, vals,
-                     m, cursor, sorted);
+  exclusive_scan(c, cc_counts, cc_offsets, bc, totals, keep);
+  hipLaunchKernelGGL(k_scatter_ranges, dim3(ranges, chunks), dim3(kSortThreads), 0, c.stream,
+                     scalars, n, p.c, p.windows, B, chunks, cc_offsets, sorted);
+  rt::check_launch();
+  hipLaunchKernelGGL(k_bucket_ranges, dim3(grid_for(B)), dim3(256), 0, c.stream, cc_offsets, B,
+                     chunks, totals, starts, counts);
   rt::check_launch();
   mark(2);
   hipLaunchKernelGGL(k_task_counts, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, B,
                      p.task_cap, ntask);
   rt::check_launch();
-  exclusive_scan(c, ntask, task_off, B, totals + 1, scan_b);
+  exclusive_scan(c, ntask, task_off, B, totals + 1, keep);
   const size_t max_tasks = m / p.task_cap + B + 1;
   rt::BufferRef tasks_buf = c.allocate(max_tasks * sizeof(Task));
   Task* tasks = static_cast<Task*>(tasks_buf->ptr);
-  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, offsets,
+  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, starts,
                      task_off, B, p.task_cap, tasks);
   rt::check_launch();
   rt::BufferRef partials_buf = c.allocate(max_tasks * sizeof(G1Xyzz));
@@ -591,38 +604,33 @@ void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* resu
   hipLaunchKernelGGL(k_bucket_sums, dim3(grid_for(B, 64)), dim3(64), 0, c.stream, task_off, ntask,
                      partials, B, sums);
   rt::check_launch();
-  // weighted bucket reduction
-  size_t groups = (B + kRed1 - 1) / kRed1;
-  rt::BufferRef pairs_buf = c.allocate(2 * 2 * groups * sizeof(G1Xyzz));
-  G1Xyzz* pa = static_cast<G1Xyzz*>(pairs_buf->ptr);
-  G1Xyzz* pr = pa + groups;
-  G1Xyzz* qa = pr + groups;
-  G1Xyzz* qr = qa + groups;
+  // level 1 of the weighted bucket reduction on the device: (A_k, R_k) per
+  // block of 256 buckets, written to result[0..G) and result[G..2G)
+  const size_t groups = (B + kRed1 - 1) / kRed1;
   mark(5);
   hipLaunchKernelGGL(k_reduce_buckets, dim3((unsigned)groups), dim3(kRed1), 0, c.stream, sums, B,
-                     pa, pr);
+                     result, result + groups);
   rt::check_launch();
-  uint32_t span_log2 = 8;  // buckets per level-1 group
-  while (groups > 1) {
-    const size_t next = (groups + kRed2 - 1) / kRed2;
-    G1Xyzz* oa = next == 1 ? result : qa;
-    hipLaunchKernelGGL(k_reduce_pairs, dim3((unsigned)next), dim3(kRed2), 0, c.stream, pa, pr,
-                       (uint32_t)groups, span_log2, oa, qr);
-    rt::check_launch();
-    if (next == 1) break;
-    std::swap(pa, qa);
-    std::swap(pr, qr);
-    groups = next;
-    span_log2 += 7;
-  }
-  if (groups == 1)
-    rt::hip_check(hipMemcpyAsync(result, pa, sizeof(G1Xyzz), hipMemcpyDeviceToDevice, c.stream));
   if (ev) {
     mark(6);
     if (ev->entries_out)
       rt::hip_check(hipMemcpyAsync(ev->entries_out, totals, sizeof(uint32_t),
                                    hipMemcpyDeviceToDevice, c.stream));
   }
+}
+
+// sum_k A_k + 256 * sum_k k R_k over the level-1 groups, on the host.
+G1Xyzz finish(const Layer& layer, const G1Xyzz* slots) {
+  const size_t groups = result_slots(layer) / 2;
+  host::Xyzz64 total = host::from_device(slots[0]);
+  for (size_t k = 1; k < groups; ++k) total = host::add(total, host::from_device(slots[k]));
+  host::Xyzz64 run{}, acc{};
+  for (size_t k = groups; k-- > 1;) {
+    run = host::add(run, host::from_device(slots[groups + k]));
+    acc = host::add(acc, run);
+  }
+  for (int s = 0; s < 8; ++s) acc = host::dbl(acc);  // 256 buckets per group
+  return host::to_device(host::add(total, acc));
 }
 
 void affine_to_packet(const G1Affine& a, uint8_t* out) {
@@ -636,7 +644,7 @@ void affine_to_packet(const G1Affine& a, uint8_t* out) {
   std::memcpy(out + 49, y.v, 48);
 }
 
-void to_packet(const G1Xyzz& p, uint8_t* out) { affine_to_packet(to_affine(p), out); }
+void to_packet(const G1Xyzz& p, uint8_t* out) { host::to_packet(host::from_device(p), out); }
 
 void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets) {
   const size_t n = layer.n;

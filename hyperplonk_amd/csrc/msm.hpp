@@ -43,9 +43,12 @@ struct StageEvents {
 };
 
 // Queue the device part of sum_i scalar[i] * base_i (scalars: Montgomery Fr
-// table of layer.n entries).  The XYZZ result lands in *result.
+// table of layer.n entries).  It leaves result_slots(layer) XYZZ values in
+// `result` (device); finish() combines them on the host after they are copied.
+size_t result_slots(const Layer& layer);
 void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* result,
              StageEvents* events = nullptr);
+G1Xyzz finish(const Layer& layer, const G1Xyzz* host_slots);
 
 // Host: XYZZ -> 97-byte packet (unique affine representative).
 void to_packet(const G1Xyzz& p, uint8_t* out97);
