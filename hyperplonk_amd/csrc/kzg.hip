@@ -62,24 +62,15 @@ kzg::Params* bind(HpMi355xContext* ctx, rt::Context& c, const HpMi355xKzgParams*
   return params->impl.get();
 }
 
-// Device result slots of several queued MSMs, finished on the host.
-struct MsmBatch {
-  std::vector<const msm::Layer*> layers;
-  std::vector<size_t> offsets;
-  size_t slots = 0;
-  void add(const msm::Layer* l) {
-    layers.push_back(l);
-    offsets.push_back(slots);
-    slots += msm::result_slots(*l);
-  }
-};
-
-void finish_points(rt::Context& c, const MsmBatch& batch, const G1Xyzz* dev, uint8_t* out97) {
-  std::vector<G1Xyzz> host(batch.slots);
-  rt::download_bytes(c, dev, batch.slots * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
-  for (size_t i = 0; i < batch.layers.size(); ++i)
-    msm::to_packet(msm::finish(*batch.layers[i], host.data() + batch.offsets[i]),
-                   out97 + rt::kG1Bytes * i);
+// Run a batch of MSMs and write one 97-byte packet per MSM.
+void run_batch(rt::Context& c, const msm::Batch& batch, uint8_t* out97) {
+  const size_t slots = batch.result_slots();
+  rt::BufferRef res = c.allocate(std::max<size_t>(slots, 1) * sizeof(G1Xyzz));
+  batch.enqueue(c, static_cast<G1Xyzz*>(res->ptr));
+  std::vector<G1Xyzz> host(slots);
+  rt::download_bytes(c, res->ptr, slots * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
+  for (size_t j = 0; j < batch.size(); ++j)
+    msm::to_packet(batch.finish(j, host.data()), out97 + rt::kG1Bytes * j);
 }
 
 }  // namespace
@@ -132,6 +123,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_generate(HpMi355xC
     if (N > 0 && (t == nullptr || !rt::canonical_fr_array(t, N))) return rt::invalid_parameters();
     rt::BufferRef tr = c.allocate(std::max<size_t>(N, 1) * rt::kFrBytes);
     if (N) rt::upload_fr_values(c, static_cast<Fr*>(tr->ptr), t, N);
+    rt::BufferRef table = msm::make_fixed_table(c);
     for (size_t i = 0; i <= N; ++i) {
       const size_t len = N - i, n = size_t(1) << len;
       rt::BufferRef scal = c.allocate(n * rt::kFrBytes);
@@ -143,7 +135,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_generate(HpMi355xC
       }
       rt::BufferRef pts = c.allocate(n * sizeof(G1Affine));
       msm::generate_points(c, static_cast<const Fr*>(scal->ptr), n,
-                           static_cast<G1Affine*>(pts->ptr));
+                           static_cast<G1Affine*>(pts->ptr),
+                           static_cast<const G1Affine*>(table->ptr));
       msm::Layer layer;
       msm::prepare_layer_from_device(c, layer, static_cast<const G1Affine*>(pts->ptr), n,
                                      p->window_override);
@@ -191,12 +184,9 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_commit(HpMi355xContext* c
     if (!f) return rt::invalid_parameters();
     const size_t layer = p->num_vars - num_vars;
     if (!p->ready[layer]) return rt::invalid_state();
-    MsmBatch batch;
-    batch.add(&p->layers[layer]);
-    rt::BufferRef res = c.allocate(batch.slots * sizeof(G1Xyzz));
-    msm::enqueue(c, p->layers[layer], static_cast<const Fr*>(f->ptr),
-                 static_cast<G1Xyzz*>(res->ptr));
-    finish_points(c, batch, static_cast<const G1Xyzz*>(res->ptr), out_commitment);
+    msm::Batch batch;
+    batch.add(p->layers[layer], static_cast<const Fr*>(f->ptr));
+    run_batch(c, batch, out_commitment);
     return rt::ok();
   });
 }
@@ -219,14 +209,10 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_multi_commit(
     }
     for (size_t i = 0; i < count; ++i)
       if (!p->ready[p->num_vars - num_vars[i]]) return rt::invalid_state();
-    MsmBatch batch;
-    for (size_t i = 0; i < count; ++i) batch.add(&p->layers[p->num_vars - num_vars[i]]);
-    rt::BufferRef res = c.allocate(batch.slots * sizeof(G1Xyzz));
-    G1Xyzz* r = static_cast<G1Xyzz*>(res->ptr);
+    msm::Batch batch;
     for (size_t i = 0; i < count; ++i)
-      msm::enqueue(c, *batch.layers[i], static_cast<const Fr*>(tables[i]->ptr),
-                   r + batch.offsets[i]);
-    finish_points(c, batch, r, out_commitments);
+      batch.add(p->layers[p->num_vars - num_vars[i]], static_cast<const Fr*>(tables[i]->ptr));
+    run_batch(c, batch, out_commitments);
     return rt::ok();
   });
 }
@@ -253,25 +239,26 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
     const size_t n = size_t(1) << num_vars;
     rt::BufferRef dpoint = c.allocate(num_vars * rt::kFrBytes);
     rt::upload_fr_values(c, static_cast<Fr*>(dpoint->ptr), point, num_vars);
-    rt::BufferRef work = c.allocate((n / 2 + n / 4 + n / 2 + 1) * rt::kFrBytes);
+    // folded tables ping-pong in fa/fb; quotient j (2^(nv-1-j) entries) is kept
+    // at q + (n - 2^(nv-j)) so that all nv MSMs run as one batch afterwards
+    rt::BufferRef work = c.allocate((n / 2 + n / 4 + 1 + n) * rt::kFrBytes);
     Fr* fa = static_cast<Fr*>(work->ptr);
     Fr* fb = fa + n / 2;
     Fr* q = fb + n / 4 + 1;
-    MsmBatch batch;
-    for (size_t j = 0; j < num_vars; ++j) batch.add(&p->layers[N - num_vars + 1 + j]);
-    rt::BufferRef res = c.allocate(batch.slots * sizeof(G1Xyzz));
-    G1Xyzz* r = static_cast<G1Xyzz*>(res->ptr);
+    msm::Batch batch;
     const Fr* src = static_cast<const Fr*>(f->ptr);
+    size_t qoff = 0;
     for (size_t j = 0; j < num_vars; ++j) {
       const size_t half = n >> (j + 1);
       Fr* dst = (j & 1) ? fb : fa;
       hipLaunchKernelGGL(kzg::k_fold_quotient, dim3(kzg::grid_for(half)), dim3(256), 0, c.stream,
-                         src, dst, q, static_cast<const Fr*>(dpoint->ptr) + j, half);
+                         src, dst, q + qoff, static_cast<const Fr*>(dpoint->ptr) + j, half);
       rt::check_launch();
-      msm::enqueue(c, *batch.layers[j], q, r + batch.offsets[j]);
+      batch.add(p->layers[N - num_vars + 1 + j], q + qoff);
+      qoff += half;
       src = dst;
     }
-    finish_points(c, batch, r, out_quotients);
+    run_batch(c, batch, out_quotients);
     rt::download_fr(c, src, 1, out_evaluation);
     return rt::ok();
   });
@@ -293,11 +280,12 @@ extern "C" HpMi355xStatus hp_mi355x_v1_msm_profile(HpMi355xContext* ctx,
     if (!p->ready[layer]) return rt::invalid_state();
     msm::StageEvents ev;
     for (auto& e : ev.marks) rt::hip_check(hipEventCreate(&e));
-    const size_t slots = msm::result_slots(p->layers[layer]);
+    msm::Batch batch;
+    batch.add(p->layers[layer], static_cast<const Fr*>(f->ptr));
+    const size_t slots = batch.result_slots();
     rt::BufferRef res = c.allocate(slots * sizeof(G1Xyzz) + sizeof(uint32_t));
     ev.entries_out = reinterpret_cast<uint32_t*>(static_cast<G1Xyzz*>(res->ptr) + slots);
-    msm::enqueue(c, p->layers[layer], static_cast<const Fr*>(f->ptr),
-                 static_cast<G1Xyzz*>(res->ptr), &ev);
+    batch.enqueue(c, static_cast<G1Xyzz*>(res->ptr), &ev);
     c.sync();
     for (int i = 0; i < 6; ++i)
       rt::hip_check(hipEventElapsedTime(&stage_ms[i], ev.marks[i], ev.marks[i + 1]));

@@ -10,8 +10,8 @@
 //                scatters base references (w n + i | sign) into
 //                bucket-contiguous order.
 //   2. tasks     each bucket is cut into tasks of <= task_cap entries.
-//   3. accumulate one wavefront per task: lanes stride over the entries with
-//                XYZZ += affine (8M + 2S), then a shuffle tree reduction.
+//   3. accumulate one lane per task (<= 64 entries of one bucket): XYZZ +=
+//                affine (8M + 2S); 64 independent accumulators per wavefront.
 //   4. buckets   per-bucket sum of its task partials.
 //   5. reduce    sum_b (b + 1) S_b: blocks of 256 buckets run an LDS suffix
 //                scan (U_t = sum_{t' >= t} S_t') and a tree sum of U, giving
@@ -41,7 +41,7 @@ namespace {
 constexpr uint32_t kSkip = 0xffffffffu;
 
 struct Task {
-  uint32_t start, len, bucket, pad;
+  uint32_t start, len, bucket, job;
 };
 
 __device__ __forceinline__ uint32_t scalar_bits(const uint32_t* s, uint32_t pos, uint32_t c) {
@@ -68,71 +68,101 @@ __device__ __forceinline__ uint32_t window_digit(const uint32_t* s, uint32_t w, 
   return raw ? raw - 1 : kSkip;
 }
 
-// Buckets are counted and scattered in ranges of kRangeBuckets so that the
-// per-block counters live in LDS (LDS atomics instead of global atomics).
-// Block (r, e) handles bucket range r and scalar chunk e; digits are
-// recomputed from the scalars in both passes, so no key/value arrays exist.
+// Jobs of one batch share one global bucket space: job j owns global buckets
+// [bucket_base, bucket_base + buckets) (bucket_base a multiple of 256, the
+// padding buckets stay empty) and (bucket, chunk) counters
+// [cc_base, cc_base + buckets * chunks).  Counting and scattering run in
+// items (job, bucket range of <= kRangeBuckets, scalar chunk) so that the
+// per-block counters live in LDS; digits are recomputed from the scalars in
+// both passes, so no key/value arrays are ever written.
+struct Job {
+  const Fr* scalars;
+  const G1Affine* bases;  // W window-shifted copies of the layer, window-major
+  uint32_t n, c, windows, buckets;
+  uint32_t bucket_base, chunks, cc_base, pad;
+};
+struct SortItem {
+  uint32_t job, b0, chunk, pad;
+};
+
 constexpr uint32_t kRangeBuckets = 8192;
 constexpr unsigned kSortThreads = 1024;
 
-// counts[b * chunks + e] = number of digits of chunk e in bucket b.
+// counts[cc_base + b * chunks + e] = digits of chunk e of the job in bucket b.
 __global__ void __launch_bounds__(kSortThreads)
-    k_count_ranges(const Fr* scalars, size_t n, uint32_t c, uint32_t windows, uint32_t buckets,
-                   uint32_t chunks, uint32_t* counts) {
+    k_count_items(const SortItem* items, const Job* jobs, uint32_t* counts) {
   __shared__ uint32_t cnt[kRangeBuckets];
-  const uint32_t b0 = blockIdx.x * kRangeBuckets;
-  const uint32_t span = min(buckets - b0, kRangeBuckets);
+  const SortItem it = items[blockIdx.x];
+  const Job j = jobs[it.job];
+  const uint32_t b0 = it.b0, span = min(j.buckets - b0, kRangeBuckets);
   for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
-  const size_t i0 = n * blockIdx.y / chunks, i1 = n * (blockIdx.y + 1) / chunks;
+  const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const Fr s = from_mont(load_fr(scalars, i));
+    const Fr s = from_mont(load_fr(j.scalars, i));
     uint32_t carry = 0;
-    for (uint32_t w = 0; w < windows; ++w) {
+    for (uint32_t w = 0; w < j.windows; ++w) {
       bool neg;
-      const uint32_t key = window_digit(s.v, w, c, carry, neg);
+      const uint32_t key = window_digit(s.v, w, j.c, carry, neg);
       if (key - b0 < span) atomicAdd(&cnt[key - b0], 1u);  // kSkip never lands in range
     }
   }
   __syncthreads();
   for (uint32_t b = threadIdx.x; b < span; b += blockDim.x)
-    counts[(size_t)(b0 + b) * chunks + blockIdx.y] = cnt[b];
+    counts[j.cc_base + (size_t)(b0 + b) * j.chunks + it.chunk] = cnt[b];
 }
 
-// sorted[offsets[b * chunks + e] + rank] = (w * n + i) | sign << 31
+// sorted[offsets[...] + rank] = (w * n + i) | sign << 31  (index into job bases)
 __global__ void __launch_bounds__(kSortThreads)
-    k_scatter_ranges(const Fr* scalars, size_t n, uint32_t c, uint32_t windows, uint32_t buckets,
-                     uint32_t chunks, const uint32_t* offsets, uint32_t* sorted) {
+    k_scatter_items(const SortItem* items, const Job* jobs, const uint32_t* offsets,
+                    uint32_t* sorted) {
   __shared__ uint32_t cur[kRangeBuckets];
-  const uint32_t b0 = blockIdx.x * kRangeBuckets;
-  const uint32_t span = min(buckets - b0, kRangeBuckets);
+  const SortItem it = items[blockIdx.x];
+  const Job j = jobs[it.job];
+  const uint32_t b0 = it.b0, span = min(j.buckets - b0, kRangeBuckets);
   for (uint32_t b = threadIdx.x; b < span; b += blockDim.x)
-    cur[b] = offsets[(size_t)(b0 + b) * chunks + blockIdx.y];
+    cur[b] = offsets[j.cc_base + (size_t)(b0 + b) * j.chunks + it.chunk];
   __syncthreads();
-  const size_t i0 = n * blockIdx.y / chunks, i1 = n * (blockIdx.y + 1) / chunks;
+  const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const Fr s = from_mont(load_fr(scalars, i));
+    const Fr s = from_mont(load_fr(j.scalars, i));
     uint32_t carry = 0;
-    for (uint32_t w = 0; w < windows; ++w) {
+    for (uint32_t w = 0; w < j.windows; ++w) {
       bool neg;
-      const uint32_t key = window_digit(s.v, w, c, carry, neg);
+      const uint32_t key = window_digit(s.v, w, j.c, carry, neg);
       if (key - b0 < span) {
         const uint32_t pos = atomicAdd(&cur[key - b0], 1u);
-        sorted[pos] = (uint32_t)(w * n + i) | (neg ? 0x80000000u : 0u);
+        sorted[pos] = (uint32_t)(w * j.n + i) | (neg ? 0x80000000u : 0u);
       }
     }
   }
 }
 
-// Per-bucket start and count from the (bucket, chunk) exclusive offsets.
-__global__ void k_bucket_ranges(const uint32_t* offsets, uint32_t buckets, uint32_t chunks,
-                                const uint32_t* total, uint32_t* starts, uint32_t* counts) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
-       b += gridDim.x * blockDim.x) {
-    const uint32_t s = offsets[(size_t)b * chunks];
-    const uint32_t e = b + 1 < buckets ? offsets[(size_t)(b + 1) * chunks] : *total;
-    starts[b] = s;
-    counts[b] = e - s;
+// Global bucket g: start, count, owning job (padding buckets: count 0).
+__global__ void k_bucket_ranges(const Job* jobs, uint32_t njobs, uint32_t gbuckets,
+                                const uint32_t* offsets, const uint32_t* total, uint32_t* starts,
+                                uint32_t* counts, uint32_t* owner) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < gbuckets;
+       g += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = njobs;  // last job with bucket_base <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (jobs[mid].bucket_base <= g) lo = mid; else hi = mid;
+    }
+    const Job j = jobs[lo];
+    const uint32_t b = g - j.bucket_base;
+    owner[g] = lo;
+    if (b >= j.buckets) {
+      starts[g] = 0;
+      counts[g] = 0;
+      continue;
+    }
+    const uint32_t s = offsets[j.cc_base + (size_t)b * j.chunks];
+    uint32_t e;
+    if (b + 1 < j.buckets) e = offsets[j.cc_base + (size_t)(b + 1) * j.chunks];
+    else e = lo + 1 < njobs ? offsets[jobs[lo + 1].cc_base] : *total;
+    starts[g] = s;
+    counts[g] = e - s;
   }
 }
 
@@ -209,61 +239,106 @@ __global__ void k_task_counts(const uint32_t* counts, uint32_t buckets, uint32_t
     ntask[b] = (counts[b] + cap - 1) / cap;
 }
 
-__global__ void k_build_tasks(const uint32_t* counts, const uint32_t* offsets,
-                              const uint32_t* task_off, uint32_t buckets, uint32_t cap,
-                              Task* tasks) {
+__global__ void k_build_tasks(const uint32_t* counts, const uint32_t* starts,
+                              const uint32_t* owner, const uint32_t* task_off, uint32_t buckets,
+                              uint32_t cap, Task* tasks) {
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
        b += gridDim.x * blockDim.x) {
-    const uint32_t cnt = counts[b], start = offsets[b];
+    const uint32_t cnt = counts[b], start = starts[b], job = owner[b];
     uint32_t t = task_off[b];
     for (uint32_t done = 0; done < cnt; done += cap, ++t)
-      tasks[t] = {start + done, min(cap, cnt - done), b, 0u};
+      tasks[t] = {start + done, min(cap, cnt - done), b, job};
   }
 }
 
-__device__ __forceinline__ G1Xyzz shfl_down_xyzz(const G1Xyzz& v, unsigned off) {
-  G1Xyzz r;
-#pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    r.X.v[i] = __shfl_down(v.X.v[i], off, 64);
-    r.Y.v[i] = __shfl_down(v.Y.v[i], off, 64);
-    r.ZZ.v[i] = __shfl_down(v.ZZ.v[i], off, 64);
-    r.ZZZ.v[i] = __shfl_down(v.ZZZ.v[i], off, 64);
-  }
-  return r;
-}
-
-// One wavefront per task.
+// One lane per task: each task is <= task_cap consecutive entries of one
+// bucket, so a wavefront holds 64 independent XYZZ accumulators and no lane
+// waits on a cross-lane reduction.
 __global__ void __launch_bounds__(256) k_accumulate(const Task* tasks, const uint32_t* ntasks,
-                                                    const uint32_t* sorted,
-                                                    const G1Affine* bases, G1Xyzz* partials) {
-  const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t lane = threadIdx.x & 63;
-  if (task >= *ntasks) return;  // uniform per wavefront
+                                                    const uint32_t* sorted, const Job* jobs,
+                                                    G1Xyzz* partials) {
+  const uint32_t task = blockIdx.x * blockDim.x + threadIdx.x;
+  if (task >= *ntasks) return;
   const Task t = tasks[task];
+  const G1Affine* bases = jobs[t.job].bases;
   G1Xyzz acc = G1Xyzz::identity();
-  for (uint32_t e = lane; e < t.len; e += 64) {
+  for (uint32_t e = 0; e < t.len; ++e) {
     const uint32_t v = sorted[t.start + e];
     G1Affine p = load_affine(bases, v & 0x7fffffffu);
     if (v >> 31) p.y = neg(p.y);
     acc = add_mixed(acc, p);
   }
-  for (unsigned off = 32; off > 0; off >>= 1) {
-    const G1Xyzz o = shfl_down_xyzz(acc, off);
-    if (lane < off) acc = add(acc, o);
-  }
-  if (lane == 0) store_xyzz(partials, task, acc);
+  store_xyzz(partials, task, acc);
 }
 
-__global__ void k_bucket_sums(const uint32_t* task_off, const uint32_t* ntask,
-                              const G1Xyzz* partials, uint32_t buckets, G1Xyzz* sums) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
-       b += gridDim.x * blockDim.x) {
+__device__ __forceinline__ G1Xyzz shfl_xor_xyzz(const G1Xyzz& v, unsigned mask) {
+  G1Xyzz r;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    r.X.v[i] = __shfl_xor(v.X.v[i], mask, 64);
+    r.Y.v[i] = __shfl_xor(v.Y.v[i], mask, 64);
+    r.ZZ.v[i] = __shfl_xor(v.ZZ.v[i], mask, 64);
+    r.ZZZ.v[i] = __shfl_xor(v.ZZZ.v[i], mask, 64);
+  }
+  return r;
+}
+
+// Bucket sum of its task partials by a group of 2^lanes_log2 lanes: each lane
+// adds every group-th partial, then a butterfly over the group.  Buckets with
+// more than kSeqMax partials per lane are appended to `heavy` instead and
+// summed by k_heavy_sums.
+constexpr uint32_t kSeqMax = 8;
+__global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
+                                                     const uint32_t* ntask,
+                                                     const G1Xyzz* partials, uint32_t buckets,
+                                                     uint32_t lanes_log2, G1Xyzz* sums,
+                                                     uint32_t* heavy, uint32_t* heavy_count) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t group = 1u << lanes_log2;
+  const uint32_t b = gid >> lanes_log2, lane = gid & (group - 1);
+  bool live = b < buckets;  // whole groups share liveness
+  G1Xyzz s = G1Xyzz::identity();
+  if (live) {
+    const uint32_t first = task_off[b], cnt = ntask[b];
+    if (cnt > kSeqMax * group) {
+      if (lane == 0) heavy[atomicAdd(heavy_count, 1u)] = b;
+      live = false;
+    } else {
+      for (uint32_t k = lane; k < cnt; k += group) s = add(s, load_xyzz(partials, first + k));
+    }
+  }
+  for (uint32_t m = 1; m < group; m <<= 1) {
+    const G1Xyzz o = shfl_xor_xyzz(s, m);
+    s = add(s, o);
+  }
+  if (live && lane == 0) store_xyzz(sums, b, s);
+}
+
+// Heavy buckets (many partials): one workgroup per bucket, strided sums then
+// an LDS tree.  Grid-strides over the heavy list.
+constexpr unsigned kHeavyThreads = 256;
+__global__ void __launch_bounds__(kHeavyThreads)
+    k_heavy_sums(const uint32_t* task_off, const uint32_t* ntask, const G1Xyzz* partials,
+                 const uint32_t* heavy, const uint32_t* heavy_count, G1Xyzz* sums) {
+  __shared__ G1Xyzz lds[kHeavyThreads];
+  const uint32_t total = *heavy_count;
+  for (uint32_t h = blockIdx.x; h < total; h += gridDim.x) {
+    const uint32_t b = heavy[h];
     const uint32_t first = task_off[b], cnt = ntask[b];
     G1Xyzz s = G1Xyzz::identity();
-    if (cnt > 0) s = load_xyzz(partials, first);
-    for (uint32_t k = 1; k < cnt; ++k) s = add(s, load_xyzz(partials, first + k));
-    store_xyzz(sums, b, s);
+    for (uint32_t k = threadIdx.x; k < cnt; k += kHeavyThreads)
+      s = add(s, load_xyzz(partials, first + k));
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (unsigned d = kHeavyThreads / 2; d > 0; d >>= 1) {
+      if (threadIdx.x < d) {
+        s = add(s, lds[threadIdx.x + d]);
+        lds[threadIdx.x] = s;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) store_xyzz(sums, b, s);
+    __syncthreads();
   }
 }
 
@@ -455,19 +530,21 @@ Plan choose_plan(size_t n, uint32_t override_c) {
   Plan p;
   uint32_t c = override_c;
   if (c == 0) {
-    // ~512 entries per bucket: B = 2^(c-1) ~ n W / 512.
+    // B = 2^(c-1) buckets for n W digits: ~512 digits per bucket for large
+    // MSMs (fewer windows, one cheap bucket reduction), ~64 for small ones
+    // (more buckets keep enough lanes busy).
+    const double per_bucket = n >= (size_t(1) << 18) ? 512.0 : 64.0;
     c = 16;
     for (int it = 0; it < 3; ++it) {
       const uint32_t w = (256 + c - 1) / c;
-      const double lg = std::log2(std::max<double>(1.0, (double)n * w));
-      int cc = (int)std::lround(lg) - 8;
+      const double lg = std::log2(std::max<double>(1.0, (double)n * w / per_bucket));
+      const int cc = (int)std::lround(lg) + 1;
       c = (uint32_t)std::min(20, std::max(4, cc));
     }
   }
   p.c = c;
   p.windows = (256 + c - 1) / c;
   p.buckets = 1u << (c - 1);
-  p.task_cap = 64 * 16;
   return p;
 }
 
@@ -539,76 +616,127 @@ bool upload_layer(rt::Context& c, Layer& layer, const uint8_t* packets, size_t n
   return true;
 }
 
-size_t result_slots(const Layer& layer) {
-  return 2 * ((layer.plan.buckets + kRed1 - 1) / kRed1);
+void Batch::add(const Layer& layer, const Fr* scalars) {
+  Entry e;
+  e.layer = &layer;
+  e.scalars = scalars;
+  e.bucket_base = gbuckets_;
+  e.chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, layer.n / 4096));
+  e.cc_base = cc_total_;
+  const uint32_t B = layer.plan.buckets;
+  gbuckets_ += (B + kRed1 - 1) / kRed1 * kRed1;
+  cc_total_ += (size_t)B * e.chunks;
+  m_total_ += layer.n * layer.plan.windows;
+  // positions and counter indices are 32-bit on the device
+  if (cc_total_ >= (size_t(1) << 32) || m_total_ >= (size_t(1) << 31)) rt::fail_params();
+  entries_.push_back(e);
 }
 
-void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* result,
-             StageEvents* ev) {
+size_t Batch::result_slots() const { return 2 * (size_t)(gbuckets_ / kRed1); }
+
+void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   auto mark = [&](int i) {
     if (ev) rt::hip_check(hipEventRecord(ev->marks[i], c.stream));
   };
-  const Plan& p = layer.plan;
-  const size_t n = layer.n;
-  const size_t m = n * p.windows;
-  const uint32_t B = p.buckets;
-  const uint32_t chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, n / 4096));
-  const uint32_t ranges = (B + kRangeBuckets - 1) / kRangeBuckets;
+  const uint32_t njobs = (uint32_t)entries_.size();
+  if (njobs == 0) return;
+  // job table and sort work items, uploaded together
+  std::vector<Job> jobs(njobs);
+  std::vector<SortItem> items;
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const Entry& e = entries_[j];
+    const Plan& p = e.layer->plan;
+    jobs[j] = {e.scalars, static_cast<const G1Affine*>(e.layer->bases->ptr), (uint32_t)e.layer->n,
+               p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base, 0u};
+    for (uint32_t b0 = 0; b0 < p.buckets; b0 += kRangeBuckets)
+      for (uint32_t ch = 0; ch < e.chunks; ++ch) items.push_back({j, b0, ch, 0u});
+  }
+  const size_t job_bytes = njobs * sizeof(Job), item_bytes = items.size() * sizeof(SortItem);
+  rt::BufferRef table = c.allocate(job_bytes + item_bytes);
+  {
+    uint8_t* stage = c.staging(job_bytes + item_bytes);
+    std::memcpy(stage, jobs.data(), job_bytes);
+    std::memcpy(stage + job_bytes, items.data(), item_bytes);
+    rt::hip_check(hipMemcpyAsync(table->ptr, stage, job_bytes + item_bytes, hipMemcpyHostToDevice,
+                                 c.stream));
+    c.staging_used();
+  }
+  const Job* djobs = static_cast<const Job*>(table->ptr);
+  const SortItem* ditems =
+      reinterpret_cast<const SortItem*>(static_cast<const uint8_t*>(table->ptr) + job_bytes);
+
+  const uint32_t G = gbuckets_;
   std::vector<rt::BufferRef> keep;
-  rt::BufferRef sorted_buf = c.allocate(std::max<size_t>(m, 1) * sizeof(uint32_t));
+  rt::BufferRef sorted_buf = c.allocate(std::max<size_t>(m_total_, 1) * sizeof(uint32_t));
   uint32_t* sorted = static_cast<uint32_t*>(sorted_buf->ptr);
-  // (bucket, chunk) counts and offsets | per-bucket start, count, ntask, task_off | totals
-  const size_t bc = (size_t)B * chunks;
-  rt::BufferRef meta = c.allocate((2 * bc + 4 * (size_t)B + 4) * sizeof(uint32_t));
+  rt::BufferRef meta = c.allocate((2 * cc_total_ + 5 * (size_t)G + 4) * sizeof(uint32_t));
   uint32_t* cc_counts = static_cast<uint32_t*>(meta->ptr);
-  uint32_t* cc_offsets = cc_counts + bc;
-  uint32_t* starts = cc_offsets + bc;
-  uint32_t* counts = starts + B;
-  uint32_t* ntask = counts + B;
-  uint32_t* task_off = ntask + B;
-  uint32_t* totals = task_off + B;  // [0] entries, [1] tasks
+  uint32_t* cc_offsets = cc_counts + cc_total_;
+  uint32_t* starts = cc_offsets + cc_total_;
+  uint32_t* counts = starts + G;
+  uint32_t* owner = counts + G;
+  uint32_t* ntask = owner + G;
+  uint32_t* task_off = ntask + G;
+  uint32_t* totals = task_off + G;  // [0] entries, [1] tasks
 
   mark(0);
   mark(1);
-  hipLaunchKernelGGL(k_count_ranges, dim3(ranges, chunks), dim3(kSortThreads), 0, c.stream,
-                     scalars, n, p.c, p.windows, B, chunks, cc_counts);
+  hipLaunchKernelGGL(k_count_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0, c.stream,
+                     ditems, djobs, cc_counts);
   rt::check_launch();
-  exclusive_scan(c, cc_counts, cc_offsets, bc, totals, keep);
-  hipLaunchKernelGGL(k_scatter_ranges, dim3(ranges, chunks), dim3(kSortThreads), 0, c.stream,
-                     scalars, n, p.c, p.windows, B, chunks, cc_offsets, sorted);
+  exclusive_scan(c, cc_counts, cc_offsets, cc_total_, totals, keep);
+  hipLaunchKernelGGL(k_scatter_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0,
+                     c.stream, ditems, djobs, cc_offsets, sorted);
   rt::check_launch();
-  hipLaunchKernelGGL(k_bucket_ranges, dim3(grid_for(B)), dim3(256), 0, c.stream, cc_offsets, B,
-                     chunks, totals, starts, counts);
+  hipLaunchKernelGGL(k_bucket_ranges, dim3(grid_for(G)), dim3(256), 0, c.stream, djobs, njobs, G,
+                     cc_offsets, totals, starts, counts, owner);
   rt::check_launch();
   mark(2);
-  hipLaunchKernelGGL(k_task_counts, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, B,
-                     p.task_cap, ntask);
+  // lane tasks: ~64 entries per lane at large sizes, down to 4 for small
+  // batches so that enough lanes run
+  const uint32_t cap = (uint32_t)std::max<size_t>(4, std::min<size_t>(64, m_total_ / 262144));
+  hipLaunchKernelGGL(k_task_counts, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, G, cap,
+                     ntask);
   rt::check_launch();
-  exclusive_scan(c, ntask, task_off, B, totals + 1, keep);
-  const size_t max_tasks = m / p.task_cap + B + 1;
+  exclusive_scan(c, ntask, task_off, G, totals + 1, keep);
+  const size_t max_tasks = m_total_ / cap + G + 1;
   rt::BufferRef tasks_buf = c.allocate(max_tasks * sizeof(Task));
   Task* tasks = static_cast<Task*>(tasks_buf->ptr);
-  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(B)), dim3(256), 0, c.stream, counts, starts,
-                     task_off, B, p.task_cap, tasks);
+  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, starts,
+                     owner, task_off, G, cap, tasks);
   rt::check_launch();
   rt::BufferRef partials_buf = c.allocate(max_tasks * sizeof(G1Xyzz));
   G1Xyzz* partials = static_cast<G1Xyzz*>(partials_buf->ptr);
   mark(3);
-  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((max_tasks + 3) / 4)), dim3(256), 0, c.stream,
-                     tasks, totals + 1, sorted, static_cast<const G1Affine*>(layer.bases->ptr),
-                     partials);
+  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((max_tasks + 255) / 256)), dim3(256), 0,
+                     c.stream, tasks, totals + 1, sorted, djobs, partials);
   rt::check_launch();
-  rt::BufferRef sums_buf = c.allocate((size_t)B * sizeof(G1Xyzz));
+  rt::BufferRef sums_buf = c.allocate((size_t)G * sizeof(G1Xyzz));
   G1Xyzz* sums = static_cast<G1Xyzz*>(sums_buf->ptr);
   mark(4);
-  hipLaunchKernelGGL(k_bucket_sums, dim3(grid_for(B, 64)), dim3(64), 0, c.stream, task_off, ntask,
-                     partials, B, sums);
+  // lanes per bucket: enough groups for ~2 wavefronts per SIMD, and no more
+  // lanes than about half the expected partials per bucket
+  const double per_bucket = (double)(m_total_ / cap + 1) / std::max<uint32_t>(G, 1);
+  uint32_t lanes_log2 = 0;
+  while (lanes_log2 < 6 && ((size_t)G << (lanes_log2 + 1)) <= 131072 &&
+         (double)(2u << lanes_log2) < per_bucket)
+    ++lanes_log2;
+  const size_t sum_threads = (size_t)G << lanes_log2;
+  rt::BufferRef heavy_buf = c.allocate(((size_t)G + 1) * sizeof(uint32_t));
+  uint32_t* heavy_count = static_cast<uint32_t*>(heavy_buf->ptr);
+  uint32_t* heavy = heavy_count + 1;
+  zero_u32(c, heavy_count, 1);
+  hipLaunchKernelGGL(k_bucket_sums, dim3((unsigned)((sum_threads + 255) / 256)), dim3(256), 0,
+                     c.stream, task_off, ntask, partials, G, lanes_log2, sums, heavy,
+                     heavy_count);
   rt::check_launch();
-  // level 1 of the weighted bucket reduction on the device: (A_k, R_k) per
-  // block of 256 buckets, written to result[0..G) and result[G..2G)
-  const size_t groups = (B + kRed1 - 1) / kRed1;
+  hipLaunchKernelGGL(k_heavy_sums, dim3(256), dim3(kHeavyThreads), 0, c.stream, task_off, ntask,
+                     partials, heavy, heavy_count, sums);
+  rt::check_launch();
+  // level 1 of the weighted bucket reduction: (A_k, R_k) per 256 buckets
+  const size_t groups = G / kRed1;
   mark(5);
-  hipLaunchKernelGGL(k_reduce_buckets, dim3((unsigned)groups), dim3(kRed1), 0, c.stream, sums, B,
+  hipLaunchKernelGGL(k_reduce_buckets, dim3((unsigned)groups), dim3(kRed1), 0, c.stream, sums, G,
                      result, result + groups);
   rt::check_launch();
   if (ev) {
@@ -619,14 +747,18 @@ void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* resu
   }
 }
 
-// sum_k A_k + 256 * sum_k k R_k over the level-1 groups, on the host.
-G1Xyzz finish(const Layer& layer, const G1Xyzz* slots) {
-  const size_t groups = result_slots(layer) / 2;
-  host::Xyzz64 total = host::from_device(slots[0]);
-  for (size_t k = 1; k < groups; ++k) total = host::add(total, host::from_device(slots[k]));
+// Job j: sum_k A_k + 256 * sum_k k R_k over its level-1 groups, on the host.
+G1Xyzz Batch::finish(size_t job, const G1Xyzz* slots) const {
+  const size_t all = gbuckets_ / kRed1;
+  const size_t g0 = entries_[job].bucket_base / kRed1;
+  const size_t groups = (entries_[job].layer->plan.buckets + kRed1 - 1) / kRed1;
+  const G1Xyzz* A = slots + g0;
+  const G1Xyzz* R = slots + all + g0;
+  host::Xyzz64 total = host::from_device(A[0]);
+  for (size_t k = 1; k < groups; ++k) total = host::add(total, host::from_device(A[k]));
   host::Xyzz64 run{}, acc{};
   for (size_t k = groups; k-- > 1;) {
-    run = host::add(run, host::from_device(slots[groups + k]));
+    run = host::add(run, host::from_device(R[k]));
     acc = host::add(acc, run);
   }
   for (int s = 0; s < 8; ++s) acc = host::dbl(acc);  // 256 buckets per group
@@ -667,7 +799,7 @@ void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets) {
   }
 }
 
-void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out) {
+rt::BufferRef make_fixed_table(rt::Context& c) {
   // standard generator (canonical) -> Montgomery on the host
   static const uint32_t gx[12] = {0xdb22c6bbu, 0xfb3af00au, 0xf97a1aefu, 0x6c55e83fu,
                                   0x171bac58u, 0xa14e3a3fu, 0x9774b905u, 0xc3688c4fu,
@@ -684,12 +816,17 @@ void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out)
   hipLaunchKernelGGL(k_fixed_table, dim3(32), dim3(256), 0, c.stream, g,
                      static_cast<G1Affine*>(table->ptr));
   rt::check_launch();
+  return table;
+}
+
+void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out,
+                     const G1Affine* table) {
   const size_t threads = std::max<size_t>(1, std::min<size_t>(n, 65536));
   rt::BufferRef scratch = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Xyzz));
   rt::BufferRef prefix = c.allocate(std::max<size_t>(n, 1) * sizeof(Fq));
   hipLaunchKernelGGL(k_fixed_base, dim3(grid_for(threads, 64)), dim3(64), 0, c.stream, scalars, n,
-                     static_cast<const G1Affine*>(table->ptr), out,
-                     static_cast<G1Xyzz*>(scratch->ptr), static_cast<Fq*>(prefix->ptr));
+                     table, out, static_cast<G1Xyzz*>(scratch->ptr),
+                     static_cast<Fq*>(prefix->ptr));
   rt::check_launch();
 }
 

@@ -17,7 +17,6 @@ struct Plan {
   uint32_t c = 0;        // window bits
   uint32_t windows = 0;  // W
   uint32_t buckets = 0;  // B
-  uint32_t task_cap = 0; // entries per accumulation task (one wavefront)
 };
 
 Plan choose_plan(size_t n, uint32_t override_c = 0);
@@ -42,13 +41,30 @@ struct StageEvents {
   uint32_t* entries_out = nullptr;  // device address of the non-zero digit count
 };
 
-// Queue the device part of sum_i scalar[i] * base_i (scalars: Montgomery Fr
-// table of layer.n entries).  It leaves result_slots(layer) XYZZ values in
-// `result` (device); finish() combines them on the host after they are copied.
-size_t result_slots(const Layer& layer);
-void enqueue(rt::Context& c, const Layer& layer, const Fr* scalars, G1Xyzz* result,
-             StageEvents* events = nullptr);
-G1Xyzz finish(const Layer& layer, const G1Xyzz* host_slots);
+// A batch of MSMs (layer, Montgomery scalar table of layer.n entries) run
+// together: every stage is one launch over all of them in one global bucket
+// space, so many small MSMs cost about as much as one.  enqueue() leaves
+// result_slots() XYZZ values in `result` (device); after they are copied to
+// the host, finish(j) returns MSM j.  The layers must outlive the launch.
+class Batch {
+ public:
+  void add(const Layer& layer, const Fr* scalars);
+  size_t size() const { return entries_.size(); }
+  size_t result_slots() const;
+  void enqueue(rt::Context& c, G1Xyzz* result, StageEvents* events = nullptr) const;
+  G1Xyzz finish(size_t job, const G1Xyzz* host_slots) const;
+
+ private:
+  struct Entry {
+    const Layer* layer;
+    const Fr* scalars;
+    uint32_t bucket_base, chunks;
+    size_t cc_base;
+  };
+  std::vector<Entry> entries_;
+  uint32_t gbuckets_ = 0;
+  size_t cc_total_ = 0, m_total_ = 0;
+};
 
 // Host: XYZZ -> 97-byte packet (unique affine representative).
 void to_packet(const G1Xyzz& p, uint8_t* out97);
@@ -57,8 +73,11 @@ void affine_to_packet(const G1Affine& p, uint8_t* out97);
 // Copy layer points (window 0) back as packets.
 void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets);
 
+// Byte table d * 2^(8 w) * g (w < 32, d < 256) of the standard generator.
+rt::BufferRef make_fixed_table(rt::Context& c);
 // Device fixed-base generation of g * s_i for a Montgomery scalar table.
-void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out);
+void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out,
+                     const G1Affine* table);
 
 }  // namespace msm
 }  // namespace hp

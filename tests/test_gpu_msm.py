@@ -165,3 +165,23 @@ def test_commit_before_layer_upload_is_invalid_state(backend):
     st = backend.lib.hp_mi355x_v1_multilinear_kzg_commit(backend.context, raw.value, m.address, 2, out)
     assert st.kind == 2
     assert backend.lib.hp_mi355x_v1_multilinear_kzg_params_destroy(backend.context, raw.value).kind == 0
+
+
+def test_heavy_buckets(backend, oracle, rng_factory, srs20):
+    """Tables of repeated small values put most digits into a few buckets
+    (selector columns full of 0/1 do this).  sum_x eq(t, x) = 1, so committing
+    the constant table k gives k * g against any layer."""
+    N, t, pp = srs20
+    for nv in (16, 20):
+        ones = backend.mle_from_evaluations(nv, [1] * (1 << nv))
+        assert backend.commit(pp, ones) == pf.G
+        minus = backend.mle_from_evaluations(nv, [R - 1] * (1 << nv))
+        assert backend.commit(pp, minus) == pf.aff_neg(pf.G)
+    nv = 16
+    rng = rng_factory(900)
+    tab = rng.fr_bytes(1 << nv)
+    tab[::2] = 0
+    tab[::2, 0] = 1  # every other entry is 1
+    m = backend.mle_from_evaluations(nv, tab)
+    layer = backend.download_layer(pp, N - nv)
+    assert backend.commit_packet(pp, m) == oracle.msm_pippenger_packets(layer, tab.tobytes(), 1 << nv, 8)
