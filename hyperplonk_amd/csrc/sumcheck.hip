@@ -65,9 +65,8 @@ __device__ void block_reduce_store(Fr (&acc)[T], int count, Fr* partial, size_t 
 }
 
 // new[f][b] = old[f][2b] + c (old[f][2b+1] - old[f][2b]) for every factor f.
-__global__ void k_fold_factors(const Fr* const* src, Fr* const* dst, size_t factors,
-                               const Fr* challenge, size_t half) {
-  const Fr c = load_fr(challenge, 0);
+__global__ void k_fold_factors(const Fr* const* src, Fr* const* dst, size_t factors, Fr c,
+                               size_t half) {
   const size_t total = factors * half;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -127,6 +126,80 @@ __global__ void __launch_bounds__(kThreads) k_message(const Fr* const* tbl, Prod
     }
   }
   block_reduce_store<D + 1>(acc, D + 1, partial, D + 1);
+}
+
+// One round in one launch, for max degree D <= 6.  With fold set, thread b
+// first folds its two new pairs of every distinct factor from the previous
+// tables with challenge c (new[2b], new[2b+1] from old[4b..4b+3]) and writes
+// them; it then evaluates every product on the pair (re-read from cache).
+// Each block leaves its D+1 partial sums; the last block to finish (ticket
+// counter) adds all partials, writes the canonical message and resets the
+// counter for the next round.
+template <int D>
+__global__ void __launch_bounds__(kThreads)
+    k_round(const Fr* const* src, Fr* const* dst, uint32_t factors, Fr c, int fold,
+            Products prods, size_t pairs, Fr* partial, unsigned* ticket, Fr* out) {
+  Fr acc[D + 1];
+#pragma unroll
+  for (int t = 0; t <= D; ++t) acc[t] = Fr::zero();
+  const Fr* const* tbl = fold ? dst : src;
+  for (size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x; b < pairs;
+       b += (size_t)gridDim.x * blockDim.x) {
+    if (fold) {
+      for (uint32_t f = 0; f < factors; ++f) {
+        const Fr* o = src[f];
+        const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
+        const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
+        store_fr(dst[f], 2 * b, a0 + c * (a1 - a0));
+        store_fr(dst[f], 2 * b + 1, a2 + c * (a3 - a2));
+      }
+    }
+    for (uint32_t p = 0; p < prods.count; ++p) {
+      const uint32_t begin = prods.offsets[p];
+      const uint32_t arity = prods.offsets[p + 1] - begin;
+      Fr val[D], step[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        if (k < (int)arity) {
+          const Fr* t = tbl[prods.slots[begin + k]];
+          const Fr lo = load_fr(t, 2 * b), hi = load_fr(t, 2 * b + 1);
+          val[k] = lo;
+          step[k] = hi - lo;
+        }
+      }
+      const Fr coeff = load_fr(prods.coeffs, p);
+      val[0] = coeff * val[0];
+      step[0] = coeff * step[0];
+#pragma unroll
+      for (int t = 0; t <= D; ++t) {
+        Fr term = val[0];
+#pragma unroll
+        for (int k = 1; k < D; ++k)
+          if (k < (int)arity) term = term * val[k];
+        acc[t] = acc[t] + term;
+        if (t < D) {
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+            if (k < (int)arity) val[k] = val[k] + step[k];
+        }
+      }
+    }
+  }
+  block_reduce_store<D + 1>(acc, D + 1, partial, D + 1);
+  // last block: sum all block partials
+  __shared__ unsigned last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x <= (unsigned)D) {
+    Fr s = Fr::zero();
+    for (unsigned blk = 0; blk < gridDim.x; ++blk) s = s + load_fr(partial, blk * (D + 1) + threadIdx.x);
+    store_fr(out, threadIdx.x, from_mont(s));
+  }
+  if (threadIdx.x == 0) *ticket = 0;
 }
 
 // Any degree: message values for t in [t0, t0 + 8) only (host loops over t0).
@@ -202,7 +275,7 @@ struct State {
   Products prods{};
   const Fr** table_ptrs[3] = {nullptr, nullptr, nullptr};  // inputs, slot A, slot B
   size_t distinct = 0, product_count = 0;
-  rt::BufferRef partial, result, challenge;
+  rt::BufferRef partial, result, ticket;
   Fr* slot_base[2] = {nullptr, nullptr};
 };
 
@@ -253,6 +326,32 @@ void launch_message(rt::Context& c, sc::State& s, const Fr* const* tables, size_
   rt::check_launch();
   hipLaunchKernelGGL(sc::k_reduce_partials, dim3(1), dim3(64), 0, c.stream, partial,
                      (size_t)blocks, width, static_cast<Fr*>(s.result->ptr));
+  rt::check_launch();
+}
+
+void launch_round(rt::Context& c, sc::State& s, const Fr* const* src, Fr* const* dst,
+                  const Fr& ch, bool fold, size_t pairs) {
+  const unsigned blocks = message_blocks(pairs);
+  Fr* partial = static_cast<Fr*>(s.partial->ptr);
+  unsigned* ticket = static_cast<unsigned*>(s.ticket->ptr);
+  Fr* out = static_cast<Fr*>(s.result->ptr);
+  const uint32_t F = static_cast<uint32_t>(s.distinct);
+  switch (s.max_degree) {
+#define HP_ROUND_CASE(D)                                                                      \
+  case D:                                                                                     \
+    hipLaunchKernelGGL(sc::k_round<D>, dim3(blocks), dim3(sc::kThreads), 0, c.stream, src, \
+                       dst, F, ch, fold ? 1 : 0, s.prods, pairs, partial, ticket, out);       \
+    break;
+    HP_ROUND_CASE(1)
+    HP_ROUND_CASE(2)
+    HP_ROUND_CASE(3)
+    HP_ROUND_CASE(4)
+    HP_ROUND_CASE(5)
+    HP_ROUND_CASE(6)
+#undef HP_ROUND_CASE
+    default:
+      rt::fail_state(-3);
+  }
   rt::check_launch();
 }
 
@@ -365,7 +464,11 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     const size_t pairs0 = size_t(1) << (num_vars - 1);
     s->partial = c.allocate(message_blocks(pairs0) * width * rt::kFrBytes);
     s->result = c.allocate(width * rt::kFrBytes);
-    s->challenge = c.allocate(rt::kFrBytes);
+    s->ticket = c.allocate(sizeof(unsigned));
+    hipLaunchKernelGGL(sc::k_zero_fr, dim3(1), dim3(64), 0, c.stream,
+                       static_cast<Fr*>(s->result->ptr), width);
+    rt::check_launch();
+    rt::hip_check(hipMemsetAsync(s->ticket->ptr, 0, sizeof(unsigned), c.stream));
     c.sync();
     auto* handle = new HpMi355xSumCheckState{ctx, s};
     c.resources[handle] = s;
@@ -400,30 +503,31 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
                            static_cast<Fr*>(s.result->ptr), width);
         rt::check_launch();
       } else {
-        const Fr* const* tables = s.table_ptrs[0];
+        Fr ch = Fr::zero();
         if (k > 0) {
-          Fr ch;
           std::memcpy(ch.v, challenge_or_null, rt::kFrBytes);
           ch = to_mont(ch);
-          uint8_t* stage = c.staging(rt::kFrBytes);
-          std::memcpy(stage, ch.v, rt::kFrBytes);
-          rt::hip_check(hipMemcpyAsync(s.challenge->ptr, stage, rt::kFrBytes,
-                                       hipMemcpyHostToDevice, c.stream));
-          c.staging_used();
-          // round k writes slot (k-1)%2 (A holds 2^(n-1), B 2^(n-2) entries) and
-          // reads the inputs (k = 1) or slot (k-2)%2
-          const Fr* const* src = k == 1 ? s.table_ptrs[0] : s.table_ptrs[1 + ((k - 2) & 1)];
-          Fr* const* dst = const_cast<Fr* const*>(s.table_ptrs[1 + ((k - 1) & 1)]);
-          const size_t half = 2 * pairs;  // new table length
-          const size_t total = s.distinct * half;
-          unsigned blocks = static_cast<unsigned>(
-              std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 65535)));
-          hipLaunchKernelGGL(sc::k_fold_factors, dim3(blocks), dim3(256), 0, c.stream, src, dst,
-                             s.distinct, static_cast<const Fr*>(s.challenge->ptr), half);
-          rt::check_launch();
-          tables = s.table_ptrs[1 + ((k - 1) & 1)];
         }
-        launch_message(c, s, tables, pairs);
+        // round k writes slot (k-1)%2 (A holds 2^(n-1), B 2^(n-2) entries) and
+        // reads the inputs (k = 1) or slot (k-2)%2
+        const Fr* const* src = k <= 1 ? s.table_ptrs[0] : s.table_ptrs[1 + ((k - 2) & 1)];
+        Fr* const* dst = k == 0 ? nullptr : const_cast<Fr* const*>(s.table_ptrs[1 + ((k - 1) & 1)]);
+        if (s.max_degree <= 6) {
+          launch_round(c, s, src, dst, ch, k > 0, pairs);
+        } else {
+          const Fr* const* tables = src;
+          if (k > 0) {
+            const size_t half = 2 * pairs;  // new table length
+            const size_t total = s.distinct * half;
+            unsigned blocks = static_cast<unsigned>(
+                std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 65535)));
+            hipLaunchKernelGGL(sc::k_fold_factors, dim3(blocks), dim3(256), 0, c.stream, src, dst,
+                               s.distinct, ch, half);
+            rt::check_launch();
+            tables = dst;
+          }
+          launch_message(c, s, tables, pairs);
+        }
       }
       std::vector<uint8_t> out(width * rt::kFrBytes);
       rt::download_bytes(c, s.result->ptr, out.size(), out.data());
