@@ -13,9 +13,10 @@
 // retires them, and a VirtualPolynomial with identity-deduplicated factors
 // (backend/src/virtual_polynomial.rs:45-235).
 //
-// Field elements are canonical little-endian 32-byte values (Fr), G1 points
-// 97-byte packets; both are plain byte arrays here so no field library is
-// needed on the host.
+// Field elements cross this interface as canonical little-endian 32-byte
+// values (Fr) and G1 points as 97-byte packets, exactly as the C ABI takes
+// them; the VirtualPolynomial container is shared with every other backend
+// the protocol layer drives (virtual_polynomial.hpp).
 #pragma once
 #include <array>
 #include <cstdint>
@@ -29,12 +30,13 @@
 #include <vector>
 
 #include "../../include/hp_mi355x.h"
+#include "virtual_polynomial.hpp"
 
 namespace hyperplonk {
 namespace mi355x {
 
-using Fr = std::array<uint8_t, 32>;
-using G1Packet = std::array<uint8_t, 97>;
+using Fr = FrBytes;
+using hyperplonk::G1Packet;
 
 inline Fr fr_from_u64(uint64_t v) {
   Fr r{};
@@ -42,14 +44,8 @@ inline Fr fr_from_u64(uint64_t v) {
   return r;
 }
 
-// backend/src/errors.rs:16-21
-struct BackendError : std::runtime_error {
-  enum class Kind { InvalidParameters, InvalidState };
-  Kind kind;
-  int32_t native_code;
-  BackendError(Kind k, const std::string& what, int32_t code = 0)
-      : std::runtime_error(what), kind(k), native_code(code) {}
-};
+using hyperplonk::BackendError;
+using hyperplonk::invalid;
 
 inline void check(HpMi355xStatus s, const char* what) {
   if (s.kind == HP_MI355X_STATUS_SUCCESS && s.native_code == 0) return;
@@ -59,10 +55,6 @@ inline void check(HpMi355xStatus s, const char* what) {
   throw BackendError(kind, std::string(what) + ": status " + std::to_string(s.kind) +
                                " native " + std::to_string(s.native_code),
                      s.native_code);
-}
-
-[[noreturn]] inline void invalid(const char* what) {
-  throw BackendError(BackendError::Kind::InvalidParameters, what);
 }
 
 // Owns the native context; every handle keeps it alive.
@@ -106,50 +98,7 @@ class Mle {
   std::shared_ptr<State> state_;
 };
 
-struct VPAuxInfo {
-  size_t max_degree = 0;
-  size_t num_variables = 0;
-};
-
-// backend/src/virtual_polynomial.rs:45-235
-class VirtualPolynomial {
- public:
-  explicit VirtualPolynomial(size_t num_variables = 0) { aux_info.num_variables = num_variables; }
-  static VirtualPolynomial new_from_mle(const Mle& mle, const Fr& coefficient) {
-    VirtualPolynomial vp(mle.num_vars());
-    vp.aux_info.max_degree = 1;
-    vp.flattened_.push_back(mle);
-    vp.lookup_[mle.id()] = 0;
-    vp.products.push_back({coefficient, {0}});
-    return vp;
-  }
-  void add_mle_list(const std::vector<Mle>& mles, const Fr& coefficient) {
-    if (mles.empty()) invalid("input mle_list is empty");
-    for (const auto& m : mles)
-      if (m.num_vars() != aux_info.num_variables)
-        invalid("product has a multiplicand with wrong number of variables");
-    aux_info.max_degree = std::max(aux_info.max_degree, mles.size());
-    std::vector<size_t> idx;
-    for (const auto& m : mles) idx.push_back(index_of(m));
-    products.push_back({coefficient, idx});
-  }
-  const std::vector<Mle>& flattened_ml_extensions() const { return flattened_; }
-
-  VPAuxInfo aux_info;
-  std::vector<std::pair<Fr, std::vector<size_t>>> products;
-
- private:
-  size_t index_of(const Mle& m) {
-    auto it = lookup_.find(m.id());
-    if (it != lookup_.end()) return it->second;
-    const size_t i = flattened_.size();
-    flattened_.push_back(m);
-    lookup_[m.id()] = i;
-    return i;
-  }
-  std::vector<Mle> flattened_;
-  std::unordered_map<uintptr_t, size_t> lookup_;
-};
+using VirtualPolynomial = BasicVirtualPolynomial<Mle>;
 
 class PreparedProverParam {
  public:
@@ -188,6 +137,11 @@ class ProverState {
 
 class Mi355xBackend {
  public:
+  using Mle = mi355x::Mle;
+  using VirtualPolynomial = mi355x::VirtualPolynomial;
+  using PreparedProverParam = mi355x::PreparedProverParam;
+  using ProverState = mi355x::ProverState;
+
   explicit Mi355xBackend(int32_t device_id = 0) : ctx_(std::make_shared<Context>(device_id)) {}
 
   // ---- MleBackend --------------------------------------------------------

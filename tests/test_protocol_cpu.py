@@ -1,0 +1,69 @@
+"""Host protocol layer on the CPU (tests/cpp/protocol_test.cpp over the oracle
+backend): transcript, serialization and pairing known answers, and HyperPlonk
+prove + verify with tampering checks.
+
+Pins: Keccak-f[1600] through SHA3-256 against hashlib at lengths around the
+rate; the Merlin construction through merlin's published simple-transcript
+challenge; from_le_bytes_mod_order through Python integers; the compressed G1
+encoding through the standard compressed BLS12-381 generator; the pairing
+through bilinearity, non-degeneracy and order r (checked inside the program)."""
+import hashlib
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "_build", "protocol_test")
+R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+MERLIN_SIMPLE = "d5a21972d0d5fe320c0d263fac7fffb8145aa640af6e9bca177c03c7efcf0615"
+G1_GEN_COMPRESSED = (
+    "97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac586c55e83ff97a1aeffb3af00adb22c6bb"
+)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp"), os.path.join(ROOT, "_build", "protocol_test")],
+                   check=True)
+
+
+@pytest.fixture(scope="module")
+def kat_lines():
+    build()
+    res = subprocess.run([BIN, "kat"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "KAT PASS" in res.stdout
+    return [line.split() for line in res.stdout.splitlines()]
+
+
+def test_keccak_against_hashlib(kat_lines):
+    rows = [l for l in kat_lines if l[0] == "sha3"]
+    assert len(rows) == 8
+    for _, n, digest in rows:
+        msg = bytes((i * 7 + 3) & 0xFF for i in range(int(n)))
+        assert hashlib.sha3_256(msg).hexdigest() == digest
+
+
+def test_merlin_known_answer(kat_lines):
+    assert [l[1] for l in kat_lines if l[0] == "merlin"] == [MERLIN_SIMPLE]
+
+
+def test_mod_order_reduction(kat_lines):
+    got = [int(l[1], 16) for l in kat_lines if l[0] == "modorder"]
+    assert got == [2**256 % R, (2**512 - 1) % R]
+
+
+def test_compressed_g1(kat_lines):
+    d = {l[0]: l[1] for l in kat_lines}
+    assert d["g1gen"] == G1_GEN_COMPRESSED
+    # -g flips only the sort flag (0x20); infinity is 0xC0 then zeros
+    assert d["g1neg"][2:] == G1_GEN_COMPRESSED[2:] and int(d["g1neg"][:2], 16) == 0x97 | 0x20
+    assert d["g1inf"] == "c0" + "00" * 47
+
+
+@pytest.mark.parametrize("nv,gate,seed", [(4, "vanilla", 1), (5, "jellyfish", 2), (6, "vanilla", 3)])
+def test_prove_verify_on_oracle_backend(nv, gate, seed):
+    build()
+    res = subprocess.run([BIN, "prove", str(nv), gate, str(seed)], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "PROVE PASS" in res.stdout
