@@ -365,3 +365,20 @@ class Mi355xBackend:
             "sumcheck round",
         )
         return fr_from_bytes(out.raw)
+
+    def prove_rounds_batch(self, states: Sequence[ProverState], challenges: Sequence[Optional[int]]) -> List[List[int]]:
+        """One round of each distinct state in shared launches (extension)."""
+        n = len(states)
+        if len(challenges) != n:
+            raise InvalidParameters("one challenge slot per state")
+        outs = [ctypes.create_string_buffer(FR_BYTES * (s.max_degree + 1)) for s in states]
+        chs = [ctypes.create_string_buffer(_fr_scalars([c]), FR_BYTES) if c is not None else None for c in challenges]
+        st_arr = (ctypes.c_void_p * max(n, 1))(*[s.raw for s in states])
+        ch_arr = (ctypes.c_void_p * max(n, 1))(*[ctypes.addressof(c) if c is not None else None for c in chs])
+        out_arr = (ctypes.c_void_p * max(n, 1))(*[ctypes.addressof(o) for o in outs])
+        cnt_arr = (ctypes.c_size_t * max(n, 1))(*[s.max_degree + 1 for s in states])
+        check(
+            self.lib.hp_mi355x_v1_sumcheck_prover_state_round_batch(self.context, st_arr, ch_arr, out_arr, cnt_arr, n),
+            "sumcheck round batch",
+        )
+        return [fr_from_bytes(o.raw) for o in outs]

@@ -7,17 +7,20 @@
 //          for t = 0..d, evaluated directly at every t (the CPU backend
 //          evaluates t <= arity and extrapolates, cpu/sum_check.rs:125-177:
 //          the same polynomial values).
-// Tables are HBM-resident Montgomery arrays; a round folds every distinct
-// factor into its ping-pong slot (one launch for all factors), evaluates the
-// message with per-block partial sums, and reduces the partials in a third
-// kernel; (d+1) canonical values come back to the host.  Products are
+// Tables are HBM-resident Montgomery arrays.  A round is one launch
+// (k_round_jobs) that folds with the previous challenge, evaluates the message
+// with per-block partial sums and lets the last block write the canonical
+// message; rounds of many states share launches (round_batch).  Products are
 // (coefficient, factor slots) with repeats kept; the coefficient is applied to
-// the first factor's line once per pair, not once per t.
+// the first factor's line once per pair, not once per t.  Degrees above 8 take
+// a generic path (separate fold, message in chunks of 8 values, reduction).
 //
 // Validation and statuses follow the reference bridge
 // (backend/src/ascend910b3/native/bridge_sumcheck.cpp:210-331 create,
 // :519-550 round).
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <unordered_map>
 #include <vector>
 
@@ -84,122 +87,181 @@ struct Products {
   uint32_t count;
 };
 
-// Message values for t in [0, D] over pairs b < pairs, tables tbl[slot].
-// MAXA bounds the arity (= D); lines are kept in registers per product.
+// ---- one round for a list of states (jobs) in one launch -----------------
+//
+// A job is one state's round.  Its blocks are a contiguous range of the grid
+// (block_begin, blocks); the last of them to finish (per-job ticket) adds the
+// job's block partials and writes the canonical message to `out`.
+// Two work decompositions:
+//   pairs    thread per pair b: fold every distinct factor on the pair (when
+//            folding), then evaluate every product on it; coalesced streaming,
+//            used while the tables are large;
+//   items    thread per (product p, pair b), product-major so a wavefront
+//            shares p: the thread folds only the factors of p on the fly (the
+//            first occurrence of each factor writes the folded pair) and
+//            evaluates p; many short chains instead of one long one, used once
+//            the tables are small and latency dominates.
+struct RoundJob {
+  const Fr* const* src;    // tables read (inputs or the previous fold slot)
+  Fr* const* dst;          // fold destination tables (folding rounds)
+  const Fr* coeffs;        // product coefficients, Montgomery
+  const uint32_t* offsets; // product_count + 1
+  const uint32_t* slots;   // flattened factor slots
+  const uint8_t* owner;    // per flattened entry: 1 = writes its factor's fold
+  Fr* partial;             // blocks x (degree + 1)
+  unsigned* ticket;
+  Fr* out;                 // degree + 1 canonical values
+  Fr challenge;            // Montgomery
+  uint32_t pairs, products, factors, degree, fold, items_mode, block_begin, blocks;
+};
+
+constexpr int kJobsPerLaunch = 16;
+struct JobPack {
+  RoundJob job[kJobsPerLaunch];
+  uint32_t count;
+};
+
+// term[t] = coeff * prod_k (lo_k + t d_k) for t = 0..deg, streaming over the
+// factors: the first factor's line is scaled by the coefficient (2 products),
+// every further factor costs deg + 1 products and its line values are reached
+// by repeated addition of d_k.
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_message(const Fr* const* tbl, Products prods,
-                                                      size_t pairs, Fr* partial) {
-  Fr acc[D + 1];
+struct ProductTerms {
+  Fr term[D + 1];
+  __device__ __forceinline__ void first(const Fr& coeff, const Fr& lo, const Fr& hi, uint32_t deg) {
+    Fr line = coeff * lo;
+    const Fr d = coeff * (hi - lo);
 #pragma unroll
-  for (int t = 0; t <= D; ++t) acc[t] = Fr::zero();
-  for (size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x; b < pairs;
-       b += (size_t)gridDim.x * blockDim.x) {
-    for (uint32_t p = 0; p < prods.count; ++p) {
-      const uint32_t begin = prods.offsets[p];
-      const uint32_t arity = prods.offsets[p + 1] - begin;
-      Fr val[D], step[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        if (k < (int)arity) {
-          const Fr* t = tbl[prods.slots[begin + k]];
-          const Fr lo = load_fr(t, 2 * b), hi = load_fr(t, 2 * b + 1);
-          val[k] = lo;
-          step[k] = hi - lo;
-        }
-      }
-      const Fr coeff = load_fr(prods.coeffs, p);
-      val[0] = coeff * val[0];
-      step[0] = coeff * step[0];
-#pragma unroll
-      for (int t = 0; t <= D; ++t) {
-        Fr term = val[0];
-#pragma unroll
-        for (int k = 1; k < D; ++k)
-          if (k < (int)arity) term = term * val[k];
-        acc[t] = acc[t] + term;
-        if (t < D) {
-#pragma unroll
-          for (int k = 0; k < D; ++k)
-            if (k < (int)arity) val[k] = val[k] + step[k];
-        }
+    for (int t = 0; t <= D; ++t) {
+      if (t <= (int)deg) {
+        term[t] = line;
+        line = line + d;
       }
     }
   }
-  block_reduce_store<D + 1>(acc, D + 1, partial, D + 1);
+  __device__ __forceinline__ void times(const Fr& lo, const Fr& hi, uint32_t deg) {
+    Fr line = lo;
+    const Fr d = hi - lo;
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+      if (t <= (int)deg) {
+        term[t] = term[t] * line;
+        line = line + d;
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ uint32_t job_of_block(const JobPack& pack) {
+  uint32_t j = 0;
+  while (j + 1 < pack.count && blockIdx.x >= pack.job[j + 1].block_begin) ++j;
+  return j;
 }
 
-// One round in one launch, for max degree D <= 6.  With fold set, thread b
-// first folds its two new pairs of every distinct factor from the previous
-// tables with challenge c (new[2b], new[2b+1] from old[4b..4b+3]) and writes
-// them; it then evaluates every product on the pair (re-read from cache).
-// Each block leaves its D+1 partial sums; the last block to finish (ticket
-// counter) adds all partials, writes the canonical message and resets the
-// counter for the next round.
+// Block partials of a job, then the last block of the job adds them and
+// writes the canonical message.
 template <int D>
-__global__ void __launch_bounds__(kThreads)
-    k_round(const Fr* const* src, Fr* const* dst, uint32_t factors, Fr c, int fold,
-            Products prods, size_t pairs, Fr* partial, unsigned* ticket, Fr* out) {
-  Fr acc[D + 1];
-#pragma unroll
-  for (int t = 0; t <= D; ++t) acc[t] = Fr::zero();
-  const Fr* const* tbl = fold ? dst : src;
-  for (size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x; b < pairs;
-       b += (size_t)gridDim.x * blockDim.x) {
-    if (fold) {
-      for (uint32_t f = 0; f < factors; ++f) {
-        const Fr* o = src[f];
-        const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
-        const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
-        store_fr(dst[f], 2 * b, a0 + c * (a1 - a0));
-        store_fr(dst[f], 2 * b + 1, a2 + c * (a3 - a2));
-      }
-    }
-    for (uint32_t p = 0; p < prods.count; ++p) {
-      const uint32_t begin = prods.offsets[p];
-      const uint32_t arity = prods.offsets[p + 1] - begin;
-      Fr val[D], step[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        if (k < (int)arity) {
-          const Fr* t = tbl[prods.slots[begin + k]];
-          const Fr lo = load_fr(t, 2 * b), hi = load_fr(t, 2 * b + 1);
-          val[k] = lo;
-          step[k] = hi - lo;
-        }
-      }
-      const Fr coeff = load_fr(prods.coeffs, p);
-      val[0] = coeff * val[0];
-      step[0] = coeff * step[0];
-#pragma unroll
-      for (int t = 0; t <= D; ++t) {
-        Fr term = val[0];
-#pragma unroll
-        for (int k = 1; k < D; ++k)
-          if (k < (int)arity) term = term * val[k];
-        acc[t] = acc[t] + term;
-        if (t < D) {
-#pragma unroll
-          for (int k = 0; k < D; ++k)
-            if (k < (int)arity) val[k] = val[k] + step[k];
-        }
-      }
-    }
-  }
-  block_reduce_store<D + 1>(acc, D + 1, partial, D + 1);
-  // last block: sum all block partials
+__device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1]) {
+  const uint32_t deg = job.degree;
+  block_reduce_store<D + 1>(acc, deg + 1, job.partial + (size_t)lblock * (deg + 1), 0);
   __shared__ unsigned last;
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  if (threadIdx.x == 0) last = atomicAdd(job.ticket, 1u) == job.blocks - 1;
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x <= (unsigned)D) {
+  if (threadIdx.x <= deg) {
     Fr s = Fr::zero();
-    for (unsigned blk = 0; blk < gridDim.x; ++blk) s = s + load_fr(partial, blk * (D + 1) + threadIdx.x);
-    store_fr(out, threadIdx.x, from_mont(s));
+    for (uint32_t blk = 0; blk < job.blocks; ++blk)
+      s = s + load_fr(job.partial, (size_t)blk * (deg + 1) + threadIdx.x);
+    store_fr(job.out, threadIdx.x, from_mont(s));
   }
-  if (threadIdx.x == 0) *ticket = 0;
+  if (threadIdx.x == 0) *job.ticket = 0;
+}
+
+// Thread per pair (grid-strided): fold every distinct factor on the pair, then
+// every product.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
+  const RoundJob& job = pack.job[job_of_block(pack)];
+  const uint32_t lblock = blockIdx.x - job.block_begin;
+  const uint32_t deg = job.degree;
+  const size_t stride = (size_t)job.blocks * kThreads;
+  Fr acc[D + 1];
+#pragma unroll
+  for (int t = 0; t <= D; ++t) acc[t] = Fr::zero();
+  const Fr* const* tbl = job.fold ? job.dst : job.src;
+  for (size_t b = (size_t)lblock * kThreads + threadIdx.x; b < job.pairs; b += stride) {
+    if (job.fold) {
+      for (uint32_t f = 0; f < job.factors; ++f) {
+        const Fr* o = job.src[f];
+        const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
+        const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
+        store_fr(job.dst[f], 2 * b, a0 + job.challenge * (a1 - a0));
+        store_fr(job.dst[f], 2 * b + 1, a2 + job.challenge * (a3 - a2));
+      }
+    }
+    for (uint32_t p = 0; p < job.products; ++p) {
+      const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
+      ProductTerms<D> pt;
+      const Fr* t0 = tbl[job.slots[begin]];
+      pt.first(load_fr(job.coeffs, p), load_fr(t0, 2 * b), load_fr(t0, 2 * b + 1), deg);
+      for (uint32_t e = begin + 1; e < end; ++e) {
+        const Fr* t = tbl[job.slots[e]];
+        pt.times(load_fr(t, 2 * b), load_fr(t, 2 * b + 1), deg);
+      }
+#pragma unroll
+      for (int t = 0; t <= D; ++t)
+        if (t <= (int)deg) acc[t] = acc[t] + pt.term[t];
+    }
+  }
+  finish_job<D>(job, lblock, acc);
+}
+
+// Thread per (product p, pair b), product-major, one item per thread: the
+// factors of p are folded on the fly (the first occurrence of a factor writes
+// its folded pair) and p is evaluated; short independent chains for the small
+// late rounds where latency dominates.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
+  const RoundJob& job = pack.job[job_of_block(pack)];
+  const uint32_t lblock = blockIdx.x - job.block_begin;
+  const uint32_t deg = job.degree;
+  const size_t items = (size_t)job.pairs * job.products;
+  const size_t it = (size_t)lblock * kThreads + threadIdx.x;
+  ProductTerms<D> pt;
+#pragma unroll
+  for (int t = 0; t <= D; ++t) pt.term[t] = Fr::zero();
+  if (it < items) {
+    const uint32_t p = static_cast<uint32_t>(it / job.pairs);
+    const size_t b = it - (size_t)p * job.pairs;
+    const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
+    for (uint32_t e = begin; e < end; ++e) {
+      const uint32_t f = job.slots[e];
+      Fr lo, hi;
+      if (job.fold) {
+        const Fr* o = job.src[f];
+        const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
+        const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
+        lo = a0 + job.challenge * (a1 - a0);
+        hi = a2 + job.challenge * (a3 - a2);
+        if (job.owner[e]) {
+          store_fr(job.dst[f], 2 * b, lo);
+          store_fr(job.dst[f], 2 * b + 1, hi);
+        }
+      } else {
+        const Fr* t = job.src[f];
+        lo = load_fr(t, 2 * b);
+        hi = load_fr(t, 2 * b + 1);
+      }
+      if (e == begin)
+        pt.first(load_fr(job.coeffs, p), lo, hi, deg);
+      else
+        pt.times(lo, hi, deg);
+    }
+  }
+  finish_job<D>(job, lblock, pt.term);
 }
 
 // Any degree: message values for t in [t0, t0 + 8) only (host loops over t0).
@@ -271,8 +333,9 @@ struct State {
   int32_t failure_code = 0;
   std::vector<rt::BufferRef> inputs;  // distinct factors (shared, never written)
   rt::BufferRef slots;                // ping-pong fold slots: 2 per distinct factor
-  rt::BufferRef meta;                 // coeffs | offsets | slots | table pointers
+  rt::BufferRef meta;                 // coeffs | table pointers | offsets | slots | owner
   Products prods{};
+  const uint8_t* owner = nullptr;
   const Fr** table_ptrs[3] = {nullptr, nullptr, nullptr};  // inputs, slot A, slot B
   size_t distinct = 0, product_count = 0;
   rt::BufferRef partial, result, ticket;
@@ -291,68 +354,184 @@ using namespace hp;
 
 namespace {
 
+constexpr unsigned kMaxJobDegree = 8;
+
 unsigned message_blocks(size_t pairs) {
   size_t b = (pairs + sc::kThreads - 1) / sc::kThreads;
   return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, sc::kMaxBlocks)));
 }
 
-// Launch the message for the current tables (pairs entries) of state s.
+// tables at or below this many pairs per round use the per-(product, pair) decomposition
+size_t items_below() {
+  static const size_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_ITEMS_BELOW");
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(1) << 15;
+  }();
+  return v;
+}
+
+// grid blocks of round k of a state (same rule at create time and at run time)
+unsigned job_blocks(const sc::State& s, size_t k, bool* items) {
+  const size_t pairs = size_t(1) << (s.num_vars - k - 1);
+  *items = s.product_count == 0 || pairs < items_below();
+  if (!*items) return message_blocks(pairs);
+  // one item per thread
+  return static_cast<unsigned>(std::max<size_t>(
+      1, (pairs * s.product_count + sc::kThreads - 1) / sc::kThreads));
+}
+
+// Generic path (degree > kMaxJobDegree): message for t in chunks of 8 values.
 void launch_message(rt::Context& c, sc::State& s, const Fr* const* tables, size_t pairs) {
   const unsigned blocks = message_blocks(pairs);
   const size_t width = s.max_degree + 1;
   Fr* partial = static_cast<Fr*>(s.partial->ptr);
-  switch (s.max_degree) {
-#define HP_MSG_CASE(D)                                                                     \
-  case D:                                                                                  \
-    hipLaunchKernelGGL(sc::k_message<D>, dim3(blocks), dim3(sc::kThreads), 0, c.stream, \
-                       tables, s.prods, pairs, partial);                                   \
-    break;
-    HP_MSG_CASE(1)
-    HP_MSG_CASE(2)
-    HP_MSG_CASE(3)
-    HP_MSG_CASE(4)
-    HP_MSG_CASE(5)
-    HP_MSG_CASE(6)
-#undef HP_MSG_CASE
-    default:
-      for (size_t t0 = 0; t0 < width; t0 += 8) {
-        const uint32_t tc = static_cast<uint32_t>(std::min<size_t>(8, width - t0));
-        hipLaunchKernelGGL(sc::k_message_chunk, dim3(blocks), dim3(sc::kThreads), 0, c.stream,
-                           tables, s.prods, pairs, static_cast<uint32_t>(t0), tc, partial,
-                           width);
-        rt::check_launch();
-      }
+  for (size_t t0 = 0; t0 < width; t0 += 8) {
+    const uint32_t tc = static_cast<uint32_t>(std::min<size_t>(8, width - t0));
+    hipLaunchKernelGGL(sc::k_message_chunk, dim3(blocks), dim3(sc::kThreads), 0, c.stream,
+                       tables, s.prods, pairs, static_cast<uint32_t>(t0), tc, partial, width);
+    rt::check_launch();
   }
-  rt::check_launch();
   hipLaunchKernelGGL(sc::k_reduce_partials, dim3(1), dim3(64), 0, c.stream, partial,
                      (size_t)blocks, width, static_cast<Fr*>(s.result->ptr));
   rt::check_launch();
 }
 
-void launch_round(rt::Context& c, sc::State& s, const Fr* const* src, Fr* const* dst,
-                  const Fr& ch, bool fold, size_t pairs) {
-  const unsigned blocks = message_blocks(pairs);
-  Fr* partial = static_cast<Fr*>(s.partial->ptr);
-  unsigned* ticket = static_cast<unsigned*>(s.ticket->ptr);
-  Fr* out = static_cast<Fr*>(s.result->ptr);
-  const uint32_t F = static_cast<uint32_t>(s.distinct);
-  switch (s.max_degree) {
-#define HP_ROUND_CASE(D)                                                                      \
-  case D:                                                                                     \
-    hipLaunchKernelGGL(sc::k_round<D>, dim3(blocks), dim3(sc::kThreads), 0, c.stream, src, \
-                       dst, F, ch, fold ? 1 : 0, s.prods, pairs, partial, ticket, out);       \
-    break;
-    HP_ROUND_CASE(1)
-    HP_ROUND_CASE(2)
-    HP_ROUND_CASE(3)
-    HP_ROUND_CASE(4)
-    HP_ROUND_CASE(5)
-    HP_ROUND_CASE(6)
-#undef HP_ROUND_CASE
-    default:
-      rt::fail_state(-3);
-  }
+// Tables read and written by round k: round k folds into slot (k-1)%2 (slot A
+// holds 2^(n-1) entries per factor, slot B 2^(n-2)) from the inputs (k = 1) or
+// slot (k-2)%2.
+void round_tables(const sc::State& s, size_t k, const Fr* const** src, Fr* const** dst) {
+  *src = k <= 1 ? s.table_ptrs[0] : s.table_ptrs[1 + ((k - 2) & 1)];
+  *dst = k == 0 ? nullptr : const_cast<Fr* const*>(s.table_ptrs[1 + ((k - 1) & 1)]);
+}
+
+sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
+  const size_t k = s.round;
+  sc::RoundJob j{};
+  round_tables(s, k, &j.src, &j.dst);
+  j.coeffs = s.prods.coeffs;
+  j.offsets = s.prods.offsets;
+  j.slots = s.prods.slots;
+  j.owner = s.owner;
+  j.partial = static_cast<Fr*>(s.partial->ptr);
+  j.ticket = static_cast<unsigned*>(s.ticket->ptr);
+  j.out = out;
+  j.challenge = ch;
+  j.pairs = static_cast<uint32_t>(size_t(1) << (s.num_vars - k - 1));
+  j.products = static_cast<uint32_t>(s.product_count);
+  j.factors = static_cast<uint32_t>(s.distinct);
+  j.degree = static_cast<uint32_t>(s.max_degree);
+  j.fold = (k > 0 && s.product_count > 0) ? 1 : 0;
+  bool items = false;
+  j.blocks = job_blocks(s, k, &items);
+  j.items_mode = items ? 1 : 0;
+  return j;
+}
+
+template <int D>
+void launch_pack(rt::Context& c, sc::JobPack& pack, unsigned grid, bool items) {
+  if (items)
+    hipLaunchKernelGGL(sc::k_round_items<D>, dim3(grid), dim3(sc::kThreads), 0, c.stream, pack);
+  else
+    hipLaunchKernelGGL(sc::k_round_pairs<D>, dim3(grid), dim3(sc::kThreads), 0, c.stream, pack);
   rt::check_launch();
+}
+
+// Launch every job (degree <= kMaxJobDegree): one kernel per (degree,
+// decomposition) and per kJobsPerLaunch jobs.
+void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
+  auto key = [&](size_t i) {
+    return std::make_pair(std::max<uint32_t>(jobs[i].degree, 1), jobs[i].items_mode);
+  };
+  std::vector<size_t> order(jobs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key(a) < key(b); });
+  size_t i = 0;
+  while (i < order.size()) {
+    const auto k = key(order[i]);
+    sc::JobPack pack{};
+    unsigned grid = 0;
+    while (i < order.size() && key(order[i]) == k && pack.count < (uint32_t)sc::kJobsPerLaunch) {
+      sc::RoundJob j = jobs[order[i]];
+      j.block_begin = grid;
+      grid += j.blocks;
+      pack.job[pack.count++] = j;
+      ++i;
+    }
+    const bool items = k.second != 0;
+    switch (k.first) {
+      case 1: launch_pack<1>(c, pack, grid, items); break;
+      case 2: launch_pack<2>(c, pack, grid, items); break;
+      case 3: launch_pack<3>(c, pack, grid, items); break;
+      case 4: launch_pack<4>(c, pack, grid, items); break;
+      case 5: launch_pack<5>(c, pack, grid, items); break;
+      case 6: launch_pack<6>(c, pack, grid, items); break;
+      case 7: launch_pack<7>(c, pack, grid, items); break;
+      case 8: launch_pack<8>(c, pack, grid, items); break;
+      default: rt::fail_state(-3);
+    }
+  }
+}
+
+// Checks of one round call before any side effect (bridge_sumcheck.cpp:519-550).
+HpMi355xStatus check_round(HpMi355xContext* ctx, rt::Context& c, HpMi355xSumCheckState* state,
+                           const uint8_t* challenge, const uint8_t* out, size_t count) {
+  if (state == nullptr || state->ctx != ctx || !c.resources.count(state))
+    return rt::invalid_parameters();
+  const sc::State& s = *state->impl;
+  if (s.failed) return rt::invalid_state(s.failure_code);
+  if (out == nullptr) return rt::invalid_parameters();
+  if (count != s.max_degree + 1) return rt::invalid_parameters();
+  if (s.round >= s.num_vars) return rt::invalid_state();
+  if (s.round == 0) {
+    if (challenge != nullptr) return rt::invalid_state();
+  } else {
+    if (challenge == nullptr) return rt::invalid_state();
+    if (!rt::canonical_fr(challenge)) return rt::invalid_parameters();
+  }
+  return rt::ok();
+}
+
+Fr challenge_of(const sc::State& s, const uint8_t* bytes) {
+  Fr ch = Fr::zero();
+  if (s.round > 0) {
+    std::memcpy(ch.v, bytes, rt::kFrBytes);
+    ch = to_mont(ch);
+  }
+  return ch;
+}
+
+// Generic round of one state of degree > kMaxJobDegree into s.result.
+void generic_round(rt::Context& c, sc::State& s, const Fr& ch) {
+  const size_t k = s.round;
+  const size_t pairs = size_t(1) << (s.num_vars - k - 1);
+  if (s.product_count == 0) {
+    hipLaunchKernelGGL(sc::k_zero_fr, dim3(1), dim3(64), 0, c.stream,
+                       static_cast<Fr*>(s.result->ptr), s.max_degree + 1);
+    rt::check_launch();
+    return;
+  }
+  const Fr* const* src;
+  Fr* const* dst;
+  round_tables(s, k, &src, &dst);
+  const Fr* const* tables = src;
+  if (k > 0) {
+    const size_t half = 2 * pairs;
+    const size_t total = s.distinct * half;
+    unsigned blocks =
+        static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 65535)));
+    hipLaunchKernelGGL(sc::k_fold_factors, dim3(blocks), dim3(256), 0, c.stream, src, dst,
+                       s.distinct, ch, half);
+    rt::check_launch();
+    tables = dst;
+  }
+  launch_message(c, s, tables, pairs);
+}
+
+void mark_failed(sc::State& s, const rt::Failure& f) {
+  if (f.poison) {
+    s.failed = true;
+    s.failure_code = f.status.native_code ? f.status.native_code : -1;
+  }
 }
 
 }  // namespace
@@ -419,12 +598,13 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
       s->slot_base[0] = static_cast<Fr*>(s->slots->ptr);
       s->slot_base[1] = s->slot_base[0] + F * a;
     }
-    // metadata block: coeffs (Fr) | table ptrs x3 (F each) | offsets | slots
+    // metadata block: coeffs (Fr) | table ptrs x3 (F each) | offsets | slots | owner
     const size_t coeff_bytes = std::max<size_t>(product_count, 1) * rt::kFrBytes;
     const size_t ptr_bytes = 3 * std::max<size_t>(F, 1) * sizeof(void*);
     const size_t off_bytes = (product_count + 1) * sizeof(uint32_t);
     const size_t idx_bytes = std::max<size_t>(index_count, 1) * sizeof(uint32_t);
-    const size_t meta_bytes = coeff_bytes + ptr_bytes + off_bytes + idx_bytes;
+    const size_t own_bytes = std::max<size_t>(index_count, 1);
+    const size_t meta_bytes = coeff_bytes + ptr_bytes + off_bytes + idx_bytes + own_bytes;
     s->meta = c.allocate(meta_bytes);
     std::vector<uint8_t> host(meta_bytes, 0);
     for (size_t p = 0; p < product_count; ++p) {
@@ -448,6 +628,13 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     for (size_t p = 0; p <= product_count; ++p) offs[p] = static_cast<uint32_t>(product_offsets[p]);
     uint32_t* idx = offs + (product_count + 1);
     for (size_t i = 0; i < index_count; ++i) idx[i] = slot_of[factor_indices[i]];
+    // the first occurrence of each factor among all product entries writes its fold
+    uint8_t* own = host.data() + coeff_bytes + ptr_bytes + off_bytes + idx_bytes;
+    std::vector<uint8_t> taken(std::max<size_t>(F, 1), 0);
+    for (size_t i = 0; i < index_count; ++i) {
+      own[i] = taken[idx[i]] ? 0 : 1;
+      taken[idx[i]] = 1;
+    }
     uint8_t* stage = c.staging(meta_bytes);
     std::memcpy(stage, host.data(), meta_bytes);
     rt::hip_check(hipMemcpyAsync(dev, stage, meta_bytes, hipMemcpyHostToDevice, c.stream));
@@ -460,9 +647,14 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     s->table_ptrs[0] = dptrs;
     s->table_ptrs[1] = dptrs + F;
     s->table_ptrs[2] = dptrs + 2 * F;
+    s->owner = dev + coeff_bytes + ptr_bytes + off_bytes + idx_bytes;
     const size_t width = max_degree + 1;
-    const size_t pairs0 = size_t(1) << (num_vars - 1);
-    s->partial = c.allocate(message_blocks(pairs0) * width * rt::kFrBytes);
+    unsigned max_blocks = message_blocks(size_t(1) << (num_vars - 1));
+    for (size_t k = 0; k < num_vars; ++k) {
+      bool items = false;
+      max_blocks = std::max(max_blocks, job_blocks(*s, k, &items));
+    }
+    s->partial = c.allocate(size_t(max_blocks) * width * rt::kFrBytes);
     s->result = c.allocate(width * rt::kFrBytes);
     s->ticket = c.allocate(sizeof(unsigned));
     hipLaunchKernelGGL(sc::k_zero_fr, dim3(1), dim3(64), 0, c.stream,
@@ -481,53 +673,18 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
     HpMi355xContext* ctx, HpMi355xSumCheckState* state, const uint8_t* challenge_or_null,
     uint8_t* out_evaluations, size_t evaluation_count) {
   return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
-    if (state == nullptr || state->ctx != ctx || !c.resources.count(state))
-      return rt::invalid_parameters();
+    const HpMi355xStatus st =
+        check_round(ctx, c, state, challenge_or_null, out_evaluations, evaluation_count);
+    if (st.kind != HP_MI355X_STATUS_SUCCESS) return st;
     sc::State& s = *state->impl;
-    if (s.failed) return rt::invalid_state(s.failure_code);
-    if (out_evaluations == nullptr) return rt::invalid_parameters();
-    if (evaluation_count != s.max_degree + 1) return rt::invalid_parameters();
-    const size_t k = s.round;
-    if (k >= s.num_vars) return rt::invalid_state();
-    if (k == 0) {
-      if (challenge_or_null != nullptr) return rt::invalid_state();
-    } else {
-      if (challenge_or_null == nullptr) return rt::invalid_state();
-      if (!rt::canonical_fr(challenge_or_null)) return rt::invalid_parameters();
-    }
     try {
       const size_t width = s.max_degree + 1;
-      const size_t pairs = size_t(1) << (s.num_vars - k - 1);
-      if (s.product_count == 0) {
-        hipLaunchKernelGGL(sc::k_zero_fr, dim3(1), dim3(64), 0, c.stream,
-                           static_cast<Fr*>(s.result->ptr), width);
-        rt::check_launch();
+      const Fr ch = challenge_of(s, challenge_or_null);
+      if (s.max_degree <= kMaxJobDegree) {
+        std::vector<sc::RoundJob> jobs{make_job(s, ch, static_cast<Fr*>(s.result->ptr))};
+        launch_jobs(c, jobs);
       } else {
-        Fr ch = Fr::zero();
-        if (k > 0) {
-          std::memcpy(ch.v, challenge_or_null, rt::kFrBytes);
-          ch = to_mont(ch);
-        }
-        // round k writes slot (k-1)%2 (A holds 2^(n-1), B 2^(n-2) entries) and
-        // reads the inputs (k = 1) or slot (k-2)%2
-        const Fr* const* src = k <= 1 ? s.table_ptrs[0] : s.table_ptrs[1 + ((k - 2) & 1)];
-        Fr* const* dst = k == 0 ? nullptr : const_cast<Fr* const*>(s.table_ptrs[1 + ((k - 1) & 1)]);
-        if (s.max_degree <= 6) {
-          launch_round(c, s, src, dst, ch, k > 0, pairs);
-        } else {
-          const Fr* const* tables = src;
-          if (k > 0) {
-            const size_t half = 2 * pairs;  // new table length
-            const size_t total = s.distinct * half;
-            unsigned blocks = static_cast<unsigned>(
-                std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 65535)));
-            hipLaunchKernelGGL(sc::k_fold_factors, dim3(blocks), dim3(256), 0, c.stream, src, dst,
-                               s.distinct, ch, half);
-            rt::check_launch();
-            tables = dst;
-          }
-          launch_message(c, s, tables, pairs);
-        }
+        generic_round(c, s, ch);
       }
       std::vector<uint8_t> out(width * rt::kFrBytes);
       rt::download_bytes(c, s.result->ptr, out.size(), out.data());
@@ -535,10 +692,65 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
       s.round += 1;
       return rt::ok();
     } catch (const rt::Failure& f) {
-      if (f.poison) {
-        s.failed = true;
-        s.failure_code = f.status.native_code ? f.status.native_code : -1;
+      mark_failed(s, f);
+      throw;
+    }
+  });
+}
+
+// One round of each of `count` distinct states: the round kernels of all
+// states (degree <= 8) share launches and one copy of the messages back.
+extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
+    HpMi355xContext* ctx, HpMi355xSumCheckState* const* states,
+    const uint8_t* const* challenges_or_null, uint8_t* const* out_evaluations,
+    const size_t* evaluation_counts, size_t count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (count == 0) return rt::ok();
+    if (states == nullptr || out_evaluations == nullptr || evaluation_counts == nullptr)
+      return rt::invalid_parameters();
+    std::unordered_map<const void*, size_t> seen;
+    for (size_t i = 0; i < count; ++i) {
+      if (!seen.emplace(states[i], i).second) return rt::invalid_parameters();
+      const uint8_t* ch = challenges_or_null ? challenges_or_null[i] : nullptr;
+      const HpMi355xStatus st =
+          check_round(ctx, c, states[i], ch, out_evaluations[i], evaluation_counts[i]);
+      if (st.kind != HP_MI355X_STATUS_SUCCESS) return st;
+    }
+    std::vector<sc::State*> ss(count);
+    std::vector<size_t> offset(count);
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+      ss[i] = states[i]->impl.get();
+      offset[i] = total;
+      total += ss[i]->max_degree + 1;
+    }
+    try {
+      rt::BufferRef outs = c.allocate(total * rt::kFrBytes);
+      Fr* out_base = static_cast<Fr*>(outs->ptr);
+      std::vector<sc::RoundJob> jobs;
+      for (size_t i = 0; i < count; ++i) {
+        sc::State& s = *ss[i];
+        const Fr ch = challenge_of(s, challenges_or_null ? challenges_or_null[i] : nullptr);
+        if (s.max_degree <= kMaxJobDegree) {
+          jobs.push_back(make_job(s, ch, out_base + offset[i]));
+        } else {
+          generic_round(c, s, ch);
+          rt::hip_check(hipMemcpyAsync(out_base + offset[i], s.result->ptr,
+                                       (s.max_degree + 1) * rt::kFrBytes,
+                                       hipMemcpyDeviceToDevice, c.stream));
+        }
       }
+      if (!jobs.empty()) launch_jobs(c, jobs);
+      std::vector<uint8_t> host(total * rt::kFrBytes);
+      rt::download_bytes(c, out_base, host.size(), host.data());
+      for (size_t i = 0; i < count; ++i) {
+        std::memcpy(out_evaluations[i], host.data() + offset[i] * rt::kFrBytes,
+                    (ss[i]->max_degree + 1) * rt::kFrBytes);
+        ss[i]->round += 1;
+      }
+      return rt::ok();
+    } catch (const rt::Failure& f) {
+      for (sc::State* s : ss) mark_failed(*s, f);
       throw;
     }
   });

@@ -171,6 +171,15 @@ HpMi355xStatus hp_mi355x_v1_msm_profile(HpMi355xContext *ctx, const HpMi355xKzgP
                                         HpMi355xDeviceAddress polynomial, size_t num_vars,
                                         float *stage_ms, size_t stage_count,
                                         uint32_t *plan_info);
+/* One round of each of `count` distinct sumcheck states in shared launches:
+ * per state i the arguments and checks of sumcheck_prover_state_round
+ * (challenges_or_null may be null when every state is at round zero).  All
+ * states are checked before any work; on a device failure every state of the
+ * batch is poisoned.  Used to batch the rounds of independent proofs. */
+HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
+    HpMi355xContext *ctx, HpMi355xSumCheckState *const *states,
+    const uint8_t *const *challenges_or_null, uint8_t *const *out_evaluations,
+    const size_t *evaluation_counts, size_t count);
 #ifdef __cplusplus
 }
 #endif

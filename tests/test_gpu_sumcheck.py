@@ -232,3 +232,47 @@ def test_invalid_challenge_rejected_by_host_mirror(backend):
     backend.prove_round_and_update_state(st, None)
     with pytest.raises(InvalidParameters):
         backend.prove_round_and_update_state(st, R)
+
+
+def test_round_batch_matches_oracle(backend, oracle, rng_factory):
+    """Rounds of several states of different sizes and degrees (including the
+    generic path above degree 8) advanced together through round_batch,
+    every message byte-exact against the oracle."""
+    rng = rng_factory(900)
+    specs = [(3, [(1, [0, 1])]), (9, [(5, [0, 1, 2]), (R - 1, [2])]), (6, [(3, [0, 0, 1, 1, 2])]),
+             (11, [(7, [0, 1, 2, 0, 1, 2, 0])]), (5, [(2, [0, 1, 2] * 3)]), (1, [(9, [0, 1])]),
+             (17, [(1, [0, 1]), (R - 3, [2, 0])])]
+    states, osts = [], []
+    for nv, prods in specs:
+        tabs = [rng.fr_bytes(1 << nv) for _ in range(3)]
+        mles = [backend.mle_from_evaluations(nv, t) for t in tabs]
+        vp = VirtualPolynomial(nv)
+        for c, idx in prods:
+            vp.add_mle_list([mles[i] for i in idx], c)
+        states.append(backend.prover_init(vp))
+        osts.append(oracle.SumCheckState(nv, vp.aux_info.max_degree,
+                                         [fr_from_bytes(t.tobytes()) for t in tabs], prods))
+    srng = pf.SplitMix64(31)
+    chal = [None] * len(specs)
+    rounds = [0] * len(specs)
+    while True:
+        live = [i for i, (nv, _) in enumerate(specs) if rounds[i] < nv]
+        if not live:
+            break
+        got = backend.prove_rounds_batch([states[i] for i in live], [chal[i] for i in live])
+        for i, msg in zip(live, got):
+            assert msg == osts[i].round(chal[i]), (i, rounds[i])
+            rounds[i] += 1
+            chal[i] = srng.nonboolean()
+
+
+def test_round_batch_checks_before_work(backend):
+    f = backend.mle_from_evaluations(2, [2, 3, 5, 7])
+    a = backend.prover_init(VirtualPolynomial.new_from_mle(f, 2))
+    b = backend.prover_init(VirtualPolynomial.new_from_mle(f, 3))
+    with pytest.raises(InvalidParameters):  # the same state twice
+        backend.prove_rounds_batch([a, a], [None, None])
+    with pytest.raises(InvalidState):  # b has a challenge at round zero
+        backend.prove_rounds_batch([a, b], [None, 5])
+    # nothing advanced: both still answer round zero
+    assert backend.prove_rounds_batch([a, b], [None, None]) == [[14, 20], [21, 30]]
