@@ -93,6 +93,17 @@ def _fr_scalars(values: Sequence[int]) -> bytes:
     return fr_to_bytes(values)
 
 
+class _KzgRequest(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("polynomial", ctypes.c_void_p),
+        ("num_vars", ctypes.c_size_t),
+        ("point", ctypes.c_void_p),
+        ("out_points", ctypes.c_void_p),
+        ("out_evaluation", ctypes.c_void_p),
+    ]
+
+
 class Mi355xBackend:
     """MleBackend + MultilinearKzgBackend + SumCheckProver on one MI355X device."""
 
@@ -163,6 +174,23 @@ class Mi355xBackend:
         pt = _fr_scalars(point) if point else None
         check(self.lib.hp_mi355x_v1_mle_evaluate(self.context, mle.address, mle.num_vars, pt, out), "evaluate")
         return fr_from_bytes(out.raw)[0]
+
+    def evaluate_mles(self, mles: Sequence[Mle], points: Sequence[Sequence[int]]) -> List[int]:
+        """Many evaluations in one pass (extension: mle_evaluate_batch)."""
+        n = len(mles)
+        if len(points) != n:
+            raise InvalidParameters("one point per polynomial")
+        for m, pt in zip(mles, points):
+            self._check_mle(m)
+            if len(pt) != m.num_vars:
+                raise InvalidParameters("wrong number of variables")
+        bufs = [ctypes.create_string_buffer(_fr_scalars(pt), FR_BYTES * len(pt)) if pt else None for pt in points]
+        addrs = (ctypes.c_void_p * max(n, 1))(*[m.address for m in mles])
+        nvs = (ctypes.c_size_t * max(n, 1))(*[m.num_vars for m in mles])
+        pts = (ctypes.c_void_p * max(n, 1))(*[ctypes.addressof(b) if b is not None else None for b in bufs])
+        out = ctypes.create_string_buffer(FR_BYTES * max(n, 1))
+        check(self.lib.hp_mi355x_v1_mle_evaluate_batch(self.context, addrs, nvs, pts, out, n), "evaluate batch")
+        return fr_from_bytes(out.raw[: FR_BYTES * n])
 
     def build_eq_x_r(self, r: Sequence[int]) -> Mle:
         if not r:
@@ -365,6 +393,45 @@ class Mi355xBackend:
             "sumcheck round",
         )
         return fr_from_bytes(out.raw)
+
+    def kzg_batch(self, pp: PreparedProverParam, ops: Sequence[tuple]) -> List[tuple]:
+        """Commits ("commit", mle) and opens ("open", mle, point) with all their
+        MSMs in one batch (extension: multilinear_kzg_batch).  Returns packets
+        per op: (commitment,) or (quotient packets, evaluation)."""
+        reqs = (_KzgRequest * max(len(ops), 1))()
+        keep = []
+        for i, op in enumerate(ops):
+            kind, mle = op[0], op[1]
+            self._check_mle(mle)
+            nv = mle.num_vars
+            r = reqs[i]
+            r.polynomial = mle.address
+            r.num_vars = nv
+            if kind == "commit":
+                r.kind = 0
+                out = ctypes.create_string_buffer(97)
+                keep.append((out, None))
+            else:
+                r.kind = 1
+                point = op[2]
+                if len(point) != nv:
+                    raise InvalidParameters("polynomial num_vars does not match point length")
+                pbuf = ctypes.create_string_buffer(_fr_scalars(point), FR_BYTES * nv) if nv else None
+                out = ctypes.create_string_buffer(97 * max(nv, 1))
+                ev = ctypes.create_string_buffer(FR_BYTES)
+                keep.append((out, ev, pbuf))
+                r.point = ctypes.addressof(pbuf) if pbuf is not None else None
+                r.out_evaluation = ctypes.addressof(ev)
+            r.out_points = ctypes.addressof(out)
+        check(self.lib.hp_mi355x_v1_multilinear_kzg_batch(self.context, pp.raw, reqs, len(ops)), "kzg batch")
+        res = []
+        for op, k in zip(ops, keep):
+            if op[0] == "commit":
+                res.append((k[0].raw,))
+            else:
+                nv = op[1].num_vars
+                res.append(([k[0].raw[97 * j : 97 * (j + 1)] for j in range(nv)], fr_from_bytes(k[1].raw)[0]))
+        return res
 
     def prove_rounds_batch(self, states: Sequence[ProverState], challenges: Sequence[Optional[int]]) -> List[List[int]]:
         """One round of each distinct state in shared launches (extension)."""

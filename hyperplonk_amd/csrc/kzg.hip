@@ -10,6 +10,7 @@
 // Prepared layers hold window-shifted copies of every point (see msm.hip).
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "kernels_common.hpp"
@@ -217,6 +218,72 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_multi_commit(
   });
 }
 
+namespace {
+
+// Folds of one open, queued on the stream: quotient j (2^(nv-1-j) entries)
+// lands at q + (n - 2^(nv-j)) and its MSM against layer N - nv + 1 + j joins
+// `batch`; *value points at the device cell holding f(point) afterwards.
+// The buffers stay alive through `keep`.
+void enqueue_open(rt::Context& c, const kzg::Params& p, const Fr* table, size_t num_vars,
+                  const uint8_t* point, msm::Batch& batch, std::vector<rt::BufferRef>& keep,
+                  const Fr** value) {
+  if (num_vars == 0) {
+    *value = table;
+    return;
+  }
+  const size_t N = p.num_vars;
+  const size_t n = size_t(1) << num_vars;
+  rt::BufferRef dpoint = c.allocate(num_vars * rt::kFrBytes);
+  rt::upload_fr_values(c, static_cast<Fr*>(dpoint->ptr), point, num_vars);
+  // folded tables ping-pong in fa/fb; the quotients are kept so that all
+  // MSMs run as one batch afterwards
+  rt::BufferRef work = c.allocate((n / 2 + n / 4 + 1 + n) * rt::kFrBytes);
+  Fr* fa = static_cast<Fr*>(work->ptr);
+  Fr* fb = fa + n / 2;
+  Fr* q = fb + n / 4 + 1;
+  const Fr* src = table;
+  size_t qoff = 0;
+  for (size_t j = 0; j < num_vars; ++j) {
+    const size_t half = n >> (j + 1);
+    Fr* dst = (j & 1) ? fb : fa;
+    hipLaunchKernelGGL(kzg::k_fold_quotient, dim3(kzg::grid_for(half)), dim3(256), 0, c.stream,
+                       src, dst, q + qoff, static_cast<const Fr*>(dpoint->ptr) + j, half);
+    rt::check_launch();
+    batch.add(p.layers[N - num_vars + 1 + j], q + qoff);
+    qoff += half;
+    src = dst;
+  }
+  *value = src;
+  keep.push_back(std::move(dpoint));
+  keep.push_back(std::move(work));
+}
+
+// f(point) cells of several opens -> canonical bytes, one copy back
+__global__ void k_gather_values(const Fr* const* cells, size_t count, Fr* out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    store_fr(out, i, load_fr(cells[i], 0));
+}
+
+void download_values(rt::Context& c, const std::vector<const Fr*>& cells, uint8_t* const* outs) {
+  if (cells.empty()) return;
+  const size_t k = cells.size();
+  rt::BufferRef ptrs = c.allocate(k * sizeof(void*));
+  uint8_t* stage = c.staging(k * sizeof(void*));
+  std::memcpy(stage, cells.data(), k * sizeof(void*));
+  rt::hip_check(hipMemcpyAsync(ptrs->ptr, stage, k * sizeof(void*), hipMemcpyHostToDevice, c.stream));
+  c.staging_used();
+  rt::BufferRef vals = c.allocate(k * rt::kFrBytes);
+  hipLaunchKernelGGL(k_gather_values, dim3(1), dim3(256), 0, c.stream,
+                     static_cast<const Fr* const*>(ptrs->ptr), k, static_cast<Fr*>(vals->ptr));
+  rt::check_launch();
+  std::vector<uint8_t> host(k * rt::kFrBytes);
+  rt::download_fr(c, static_cast<const Fr*>(vals->ptr), k, host.data());
+  for (size_t i = 0; i < k; ++i) std::memcpy(outs[i], host.data() + i * rt::kFrBytes, rt::kFrBytes);
+}
+
+}  // namespace
+
 extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
     HpMi355xContext* ctx, const HpMi355xKzgParams* params, HpMi355xDeviceAddress polynomial,
     size_t num_vars, const uint8_t* point, uint8_t* out_quotients, uint8_t* out_evaluation) {
@@ -232,34 +299,79 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
     const size_t N = p->num_vars;
     for (size_t j = 0; j < num_vars; ++j)
       if (!p->ready[N - num_vars + 1 + j]) return rt::invalid_state();
-    if (num_vars == 0) {
-      rt::download_fr(c, static_cast<const Fr*>(f->ptr), 1, out_evaluation);
-      return rt::ok();
-    }
-    const size_t n = size_t(1) << num_vars;
-    rt::BufferRef dpoint = c.allocate(num_vars * rt::kFrBytes);
-    rt::upload_fr_values(c, static_cast<Fr*>(dpoint->ptr), point, num_vars);
-    // folded tables ping-pong in fa/fb; quotient j (2^(nv-1-j) entries) is kept
-    // at q + (n - 2^(nv-j)) so that all nv MSMs run as one batch afterwards
-    rt::BufferRef work = c.allocate((n / 2 + n / 4 + 1 + n) * rt::kFrBytes);
-    Fr* fa = static_cast<Fr*>(work->ptr);
-    Fr* fb = fa + n / 2;
-    Fr* q = fb + n / 4 + 1;
     msm::Batch batch;
-    const Fr* src = static_cast<const Fr*>(f->ptr);
-    size_t qoff = 0;
-    for (size_t j = 0; j < num_vars; ++j) {
-      const size_t half = n >> (j + 1);
-      Fr* dst = (j & 1) ? fb : fa;
-      hipLaunchKernelGGL(kzg::k_fold_quotient, dim3(kzg::grid_for(half)), dim3(256), 0, c.stream,
-                         src, dst, q + qoff, static_cast<const Fr*>(dpoint->ptr) + j, half);
-      rt::check_launch();
-      batch.add(p->layers[N - num_vars + 1 + j], q + qoff);
-      qoff += half;
-      src = dst;
+    std::vector<rt::BufferRef> keep;
+    const Fr* value = nullptr;
+    enqueue_open(c, *p, static_cast<const Fr*>(f->ptr), num_vars, point, batch, keep, &value);
+    if (batch.size()) run_batch(c, batch, out_quotients);
+    rt::download_fr(c, value, 1, out_evaluation);
+    return rt::ok();
+  });
+}
+
+// Commits and opens of many polynomials with all their MSMs in one batch.
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext* ctx,
+                                                             const HpMi355xKzgParams* params,
+                                                             const HpMi355xKzgRequest* requests,
+                                                             size_t count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p) return rt::invalid_parameters();
+    if (count == 0) return rt::ok();
+    if (requests == nullptr) return rt::invalid_parameters();
+    const size_t N = p->num_vars;
+    std::vector<rt::BufferRef> tables(count);
+    for (size_t i = 0; i < count; ++i) {
+      const HpMi355xKzgRequest& r = requests[i];
+      if (r.kind != HP_MI355X_KZG_COMMIT && r.kind != HP_MI355X_KZG_OPEN)
+        return rt::invalid_parameters();
+      if (r.num_vars > N) return rt::invalid_parameters();
+      tables[i] = c.lookup(r.polynomial, r.num_vars);
+      if (!tables[i]) return rt::invalid_parameters();
+      if (r.kind == HP_MI355X_KZG_COMMIT) {
+        if (r.out_points == nullptr) return rt::invalid_parameters();
+      } else {
+        if (r.out_evaluation == nullptr) return rt::invalid_parameters();
+        if (r.num_vars > 0 && (r.point == nullptr || r.out_points == nullptr))
+          return rt::invalid_parameters();
+        if (r.num_vars && !rt::canonical_fr_array(r.point, r.num_vars)) return rt::invalid_parameters();
+      }
     }
-    run_batch(c, batch, out_quotients);
-    rt::download_fr(c, src, 1, out_evaluation);
+    for (size_t i = 0; i < count; ++i) {
+      const HpMi355xKzgRequest& r = requests[i];
+      if (r.kind == HP_MI355X_KZG_COMMIT) {
+        if (!p->ready[N - r.num_vars]) return rt::invalid_state();
+      } else {
+        for (size_t j = 0; j < r.num_vars; ++j)
+          if (!p->ready[N - r.num_vars + 1 + j]) return rt::invalid_state();
+      }
+    }
+    msm::Batch batch;
+    std::vector<rt::BufferRef> keep;
+    std::vector<const Fr*> cells;
+    std::vector<uint8_t*> value_outs;
+    std::vector<uint8_t*> packet_outs;  // one per MSM, in batch order
+    for (size_t i = 0; i < count; ++i) {
+      const HpMi355xKzgRequest& r = requests[i];
+      const Fr* table = static_cast<const Fr*>(tables[i]->ptr);
+      if (r.kind == HP_MI355X_KZG_COMMIT) {
+        batch.add(p->layers[N - r.num_vars], table);
+        packet_outs.push_back(r.out_points);
+      } else {
+        const Fr* value = nullptr;
+        enqueue_open(c, *p, table, r.num_vars, r.point, batch, keep, &value);
+        for (size_t j = 0; j < r.num_vars; ++j) packet_outs.push_back(r.out_points + rt::kG1Bytes * j);
+        cells.push_back(value);
+        value_outs.push_back(r.out_evaluation);
+      }
+    }
+    if (batch.size()) {
+      std::vector<uint8_t> packets(batch.size() * rt::kG1Bytes);
+      run_batch(c, batch, packets.data());
+      for (size_t j = 0; j < packet_outs.size(); ++j)
+        std::memcpy(packet_outs[j], packets.data() + rt::kG1Bytes * j, rt::kG1Bytes);
+    }
+    download_values(c, cells, value_outs.data());
     return rt::ok();
   });
 }

@@ -7,6 +7,7 @@
 // (backend/src/ascend910b3/native/bridge_mle.cpp:173-733).  Table order is
 // low-bit-first: index bit i <-> variable i <-> point[i].
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "kernels_common.hpp"
@@ -124,6 +125,65 @@ __global__ void k_eq_outer(EqChunks chunks, Fr* out, size_t n) {
       v = v * load_fr(chunks.table[c], (x >> chunks.shift[c]) & ((size_t(1) << chunks.bits[c]) - 1));
     store_fr(out, x, v);
   }
+}
+
+// ---- batched evaluation: f(r) = sum_x f[x] eq(r, x) ----------------------
+struct DotJob {
+  const Fr* table;
+  const Fr* eq;  // null: the zero-variable eq table [1]
+  Fr* partial;   // one Fr per block of the job
+  unsigned* ticket;
+  Fr* out;       // canonical result
+  uint32_t len, block_begin, blocks, pad;
+};
+
+constexpr unsigned kDotPerThread = 16;
+
+__device__ __forceinline__ Fr wave_sum_fr(Fr v) {
+#pragma unroll
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    Fr o;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o.v[i] = __shfl_xor(v.v[i], offset, 64);
+    v = v + o;
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_eval_dots(const DotJob* jobs, uint32_t count) {
+  __shared__ Fr lds[kThreads / 64];
+  __shared__ unsigned last;
+  uint32_t lo = 0, hi = count;  // last job with block_begin <= blockIdx.x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].block_begin <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  const DotJob job = jobs[lo];
+  const uint32_t lblock = blockIdx.x - job.block_begin;
+  Fr acc = Fr::zero();
+  for (size_t x = (size_t)lblock * kThreads + threadIdx.x; x < job.len;
+       x += (size_t)job.blocks * kThreads)
+    acc = acc + (job.eq ? load_fr(job.table, x) * load_fr(job.eq, x) : load_fr(job.table, x));
+  acc = wave_sum_fr(acc);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Fr s = lds[0];
+    for (unsigned w = 1; w < kThreads / 64; ++w) s = s + lds[w];
+    store_fr(job.partial, lblock, s);
+    __threadfence();
+    last = atomicAdd(job.ticket, 1u) == job.blocks - 1;
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();
+  Fr s = Fr::zero();
+  for (uint32_t b = 0; b < job.blocks; ++b) s = s + load_fr(job.partial, b);
+  store_fr(job.out, 0, from_mont(s));
+}
+
+__global__ void k_zero_words(unsigned* p, size_t n) {
+  GRID_STRIDE(i, n) p[i] = 0;
 }
 
 __global__ void k_zero_u32(unsigned* p) {
@@ -388,6 +448,84 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate(HpMi355xContext* ctx,
     mle::evaluate(c, static_cast<const Fr*>(table->ptr), num_vars,
                   static_cast<const Fr*>(dpoint->ptr), static_cast<Fr*>(result->ptr));
     rt::download_fr(c, static_cast<const Fr*>(result->ptr), 1, out_evaluation);
+    return rt::ok();
+  });
+}
+
+// Every (table, point) evaluation in one pass: one eq(point, .) table per
+// distinct point, then one launch of dot products (f(r) = sum_x f[x] eq(r, x),
+// the same field value as the fold chain) and one copy back.
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(
+    HpMi355xContext* ctx, const HpMi355xDeviceAddress* polynomials, const size_t* num_vars,
+    const uint8_t* const* points, uint8_t* out_evaluations, size_t count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (count == 0) return rt::ok();
+    if (polynomials == nullptr || num_vars == nullptr || points == nullptr ||
+        out_evaluations == nullptr)
+      return rt::invalid_parameters();
+    std::vector<rt::BufferRef> tables(count);
+    for (size_t i = 0; i < count; ++i) {
+      if (!dims_ok(num_vars[i])) return rt::invalid_parameters();
+      if (num_vars[i] && (points[i] == nullptr || !rt::canonical_fr_array(points[i], num_vars[i])))
+        return rt::invalid_parameters();
+      tables[i] = c.lookup(polynomials[i], num_vars[i]);
+      if (!tables[i]) return rt::invalid_parameters();
+    }
+    // distinct points (by value), their eq tables
+    std::vector<size_t> point_of(count);
+    std::vector<size_t> firsts;
+    for (size_t i = 0; i < count; ++i) {
+      size_t d = 0;
+      for (; d < firsts.size(); ++d) {
+        const size_t j = firsts[d];
+        if (num_vars[j] == num_vars[i] &&
+            std::memcmp(points[j], points[i], num_vars[i] * rt::kFrBytes) == 0)
+          break;
+      }
+      if (d == firsts.size()) firsts.push_back(i);
+      point_of[i] = d;
+    }
+    std::vector<rt::BufferRef> eqs(firsts.size());
+    for (size_t d = 0; d < firsts.size(); ++d) {
+      const size_t j = firsts[d];
+      if (num_vars[j] == 0) continue;
+      rt::BufferRef dp = upload_small(c, points[j], num_vars[j]);
+      eqs[d] = c.allocate((size_t(1) << num_vars[j]) * rt::kFrBytes);
+      mle::build_eq(c, static_cast<const Fr*>(dp->ptr), num_vars[j], static_cast<Fr*>(eqs[d]->ptr));
+    }
+    std::vector<DotJob> jobs(count);
+    uint32_t grid_blocks = 0;
+    for (size_t i = 0; i < count; ++i) {
+      const size_t len = size_t(1) << num_vars[i];
+      const size_t per_block = size_t(kThreads) * kDotPerThread;
+      jobs[i].len = static_cast<uint32_t>(len);
+      jobs[i].blocks = static_cast<uint32_t>(std::max<size_t>(1, (len + per_block - 1) / per_block));
+      jobs[i].block_begin = grid_blocks;
+      grid_blocks += jobs[i].blocks;
+    }
+    rt::BufferRef partial = c.allocate(size_t(grid_blocks) * rt::kFrBytes);
+    rt::BufferRef tickets = c.allocate(count * sizeof(unsigned));
+    rt::BufferRef results = c.allocate(count * rt::kFrBytes);
+    for (size_t i = 0; i < count; ++i) {
+      jobs[i].table = static_cast<const Fr*>(tables[i]->ptr);
+      jobs[i].eq = eqs[point_of[i]] ? static_cast<const Fr*>(eqs[point_of[i]]->ptr) : nullptr;
+      jobs[i].partial = static_cast<Fr*>(partial->ptr) + jobs[i].block_begin;
+      jobs[i].ticket = static_cast<unsigned*>(tickets->ptr) + i;
+      jobs[i].out = static_cast<Fr*>(results->ptr) + i;
+    }
+    rt::BufferRef djobs = c.allocate(count * sizeof(DotJob));
+    uint8_t* stage = c.staging(count * sizeof(DotJob));
+    std::memcpy(stage, jobs.data(), count * sizeof(DotJob));
+    rt::hip_check(hipMemcpyAsync(djobs->ptr, stage, count * sizeof(DotJob), hipMemcpyHostToDevice,
+                                 c.stream));
+    c.staging_used();
+    hipLaunchKernelGGL(k_zero_words, dim3(grid(count)), dim3(kThreads), 0, c.stream,
+                       static_cast<unsigned*>(tickets->ptr), count);
+    rt::check_launch();
+    hipLaunchKernelGGL(k_eval_dots, dim3(grid_blocks), dim3(kThreads), 0, c.stream,
+                       static_cast<const DotJob*>(djobs->ptr), static_cast<uint32_t>(count));
+    rt::check_launch();
+    rt::download_bytes(c, results->ptr, count * rt::kFrBytes, out_evaluations);
     return rt::ok();
   });
 }

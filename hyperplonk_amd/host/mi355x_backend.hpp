@@ -192,6 +192,28 @@ class Mi355xBackend {
     return out;
   }
 
+  // extension: many evaluations in one pass (mle_evaluate_batch)
+  std::vector<Fr> evaluate_mles(const std::vector<Mle>& mles,
+                                const std::vector<std::vector<Fr>>& points) const {
+    if (mles.size() != points.size()) invalid("one point per polynomial");
+    std::vector<Fr> out(mles.size());
+    if (mles.empty()) return out;
+    std::vector<HpMi355xDeviceAddress> addrs;
+    std::vector<size_t> nvs;
+    std::vector<const uint8_t*> pts;
+    for (size_t i = 0; i < mles.size(); ++i) {
+      own(mles[i]);
+      if (points[i].size() != mles[i].num_vars()) invalid("wrong number of variables");
+      addrs.push_back(mles[i].address());
+      nvs.push_back(mles[i].num_vars());
+      pts.push_back(points[i].empty() ? nullptr : points[i][0].data());
+    }
+    check(hp_mi355x_v1_mle_evaluate_batch(ctx_->raw(), addrs.data(), nvs.data(), pts.data(),
+                                          out[0].data(), out.size()),
+          "evaluate_mles");
+    return out;
+  }
+
   Mle build_eq_x_r(const std::vector<Fr>& r) const {
     if (r.empty()) invalid("r length is 0");
     HpMi355xDeviceAddress out = nullptr;
@@ -303,6 +325,62 @@ class Mi355xBackend {
                                             nv ? proofs[0].data() : nullptr, value.data()),
           "open");
     return {proofs, value};
+  }
+
+  // extension: commits and opens of many polynomials in one batched MSM pass
+  struct KzgOp {
+    bool open = false;
+    Mle polynomial;
+    std::vector<Fr> point;  // opens only
+  };
+  struct KzgResult {
+    std::vector<G1Packet> points;  // commit: 1; open: num_vars quotients
+    Fr value{};                    // opens only
+  };
+  std::vector<KzgResult> kzg_batch(const PreparedProverParam& pp, const std::vector<KzgOp>& ops) const {
+    std::vector<KzgResult> res(ops.size());
+    std::vector<HpMi355xKzgRequest> reqs(ops.size());
+    for (size_t i = 0; i < ops.size(); ++i) {
+      const KzgOp& op = ops[i];
+      own(op.polynomial);
+      const size_t nv = op.polynomial.num_vars();
+      if (op.open && op.point.size() != nv) invalid("polynomial num_vars does not match point length");
+      res[i].points.resize(op.open ? nv : 1);
+      reqs[i].kind = op.open ? HP_MI355X_KZG_OPEN : HP_MI355X_KZG_COMMIT;
+      reqs[i].polynomial = op.polynomial.address();
+      reqs[i].num_vars = nv;
+      reqs[i].point = op.open && nv ? op.point[0].data() : nullptr;
+      reqs[i].out_points = res[i].points.empty() ? nullptr : res[i].points[0].data();
+      reqs[i].out_evaluation = op.open ? res[i].value.data() : nullptr;
+    }
+    if (!reqs.empty())
+      check(hp_mi355x_v1_multilinear_kzg_batch(ctx_->raw(), pp.raw(), reqs.data(), reqs.size()),
+            "kzg_batch");
+    return res;
+  }
+
+  // extension: one round of several states in shared launches
+  std::vector<std::vector<Fr>> prove_rounds_batch(std::vector<ProverState*>& states,
+                                                  const std::vector<std::optional<Fr>>& challenges) const {
+    const size_t n = states.size();
+    if (challenges.size() != n) invalid("one challenge slot per state");
+    std::vector<std::vector<Fr>> out(n);
+    std::vector<HpMi355xSumCheckState*> raw(n);
+    std::vector<const uint8_t*> ch(n);
+    std::vector<uint8_t*> outs(n);
+    std::vector<size_t> counts(n);
+    for (size_t i = 0; i < n; ++i) {
+      out[i].resize(states[i]->max_degree() + 1);
+      raw[i] = states[i]->state_->raw;
+      ch[i] = challenges[i] ? challenges[i]->data() : nullptr;
+      outs[i] = out[i][0].data();
+      counts[i] = out[i].size();
+    }
+    if (n)
+      check(hp_mi355x_v1_sumcheck_prover_state_round_batch(ctx_->raw(), raw.data(), ch.data(),
+                                                           outs.data(), counts.data(), n),
+            "prove_rounds_batch");
+    return out;
   }
 
   // ---- SumCheckProver ----------------------------------------------------

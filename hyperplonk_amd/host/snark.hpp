@@ -296,10 +296,9 @@ HyperPlonkProof prove(const B& backend, const HyperPlonkProvingKey<B>& pk,
   std::vector<Fr> r_pi = transcript.get_and_append_challenge_vectors("r_pi", ell);
   r_pi.resize(nv, Fr::zero());
   insert(wpolys[0], r_pi);
-  std::vector<Fr> evals;
-  evals.reserve(polys.size());
-  for (size_t i = 0; i < polys.size(); ++i)
-    evals.push_back(from_wire(backend.evaluate_mle(polys[i], to_wire(points[i]))));
+  std::vector<std::vector<FrBytes>> wire_points;
+  for (const auto& pt : points) wire_points.push_back(to_wire(pt));
+  const std::vector<Fr> evals = from_wire(backend.evaluate_mles(polys, wire_points));
 
   // 5. batch opening
   proof.batch_openings = multi_open(backend, pp, polys, points, evals, transcript);

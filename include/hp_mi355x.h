@@ -180,6 +180,36 @@ HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
     HpMi355xContext *ctx, HpMi355xSumCheckState *const *states,
     const uint8_t *const *challenges_or_null, uint8_t *const *out_evaluations,
     const size_t *evaluation_counts, size_t count);
+/* Evaluate `count` resident tables, table i (num_vars[i] variables) at
+ * points[i] (num_vars[i] canonical Fr, low variable first), into
+ * out_evaluations (32 * count bytes): evaluate_mle for many tables in one
+ * pass; all arguments are checked before any work. */
+HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(HpMi355xContext *ctx,
+                                               const HpMi355xDeviceAddress *polynomials,
+                                               const size_t *num_vars,
+                                               const uint8_t *const *points,
+                                               uint8_t *out_evaluations, size_t count);
+/* One request of multilinear_kzg_batch: a commit (out_points: one packet)
+ * or an open at `point` (out_points: num_vars packets, out_evaluation: 32
+ * bytes), with the arguments and checks of multilinear_kzg_commit / _open. */
+#define HP_MI355X_KZG_COMMIT 0
+#define HP_MI355X_KZG_OPEN 1
+typedef struct HpMi355xKzgRequest {
+  int32_t kind;
+  HpMi355xDeviceAddress polynomial;
+  size_t num_vars;
+  const uint8_t *point;
+  uint8_t *out_points;
+  uint8_t *out_evaluation;
+} HpMi355xKzgRequest;
+/* Commits and opens of many polynomials (of any sizes) whose MSMs all run as
+ * one batched launch sequence; results identical to the one-at-a-time calls.
+ * Every request is checked before any work.  Used to batch the commitments
+ * and openings of independent proofs. */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext *ctx,
+                                                  const HpMi355xKzgParams *params,
+                                                  const HpMi355xKzgRequest *requests,
+                                                  size_t count);
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,12 @@ class OracleBackend {
        "evaluate");
     return out;
   }
+  std::vector<FrBytes> evaluate_mles(const std::vector<Mle>& ms,
+                                     const std::vector<std::vector<FrBytes>>& points) const {
+    std::vector<FrBytes> out;
+    for (size_t i = 0; i < ms.size(); ++i) out.push_back(evaluate_mle(ms[i], points[i]));
+    return out;
+  }
   Mle build_eq_x_r(const std::vector<FrBytes>& r) const {
     std::vector<uint8_t> out(32 << r.size());
     ok(oracle_eq_x_r(r[0].data(), r.size(), out.data()), "eq");

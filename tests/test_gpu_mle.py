@@ -193,3 +193,22 @@ def test_evaluate_large_table_matches_eq_weighted_sum(backend, rng_factory):
     for i in range(1 << nv):
         acc += int.from_bytes(vals[i].tobytes(), "little") * int.from_bytes(eq[i].tobytes(), "little")
     assert got == acc % R
+
+
+def test_evaluate_batch_matches_oracle(backend, oracle, rng_factory):
+    """Batched evaluations: mixed sizes (including a constant and a table above
+    one dot block), repeated and shared points, byte-exact against the oracle."""
+    rng = rng_factory(77)
+    nvs = [0, 1, 3, 12, 12, 12, 15, 17, 5]
+    tabs = [rng.fr_bytes(1 << nv) for nv in nvs]
+    mles = [backend.mle_from_evaluations(nv, t) for nv, t in zip(nvs, tabs)]
+    p12 = fr_from_bytes(rng.fr_bytes(12).tobytes())
+    points = []
+    for nv in nvs:
+        points.append(p12 if nv == 12 else fr_from_bytes(rng.fr_bytes(nv).tobytes()) if nv else [])
+    points[5] = [0, 1] * 6  # a boolean point picks one entry
+    got = backend.evaluate_mles(mles, points)
+    for nv, t, pt, g in zip(nvs, tabs, points, got):
+        assert g == oracle.mle_evaluate(fr_from_bytes(t.tobytes()), pt)
+    assert got[5] == fr_from_bytes(tabs[5].tobytes())[int("10" * 6, 2)]
+    assert backend.evaluate_mles([], []) == []

@@ -185,3 +185,29 @@ def test_heavy_buckets(backend, oracle, rng_factory, srs20):
     m = backend.mle_from_evaluations(nv, tab)
     layer = backend.download_layer(pp, N - nv)
     assert backend.commit_packet(pp, m) == oracle.msm_pippenger_packets(layer, tab.tobytes(), 1 << nv, 8)
+
+
+def test_kzg_batch_matches_single_calls(backend, rng_factory, srs20):
+    """Commits and opens of mixed sizes in one batch give exactly the packets
+    and values of the one-at-a-time calls (which the tests above pin to the
+    oracle); an invalid request anywhere leaves every output untouched."""
+    N, t, pp = srs20
+    rng = rng_factory(901)
+    nvs = [12, 0, 5, 12, 9, 1, 14]
+    mles = [backend.mle_from_evaluations(nv, rng.fr_bytes(1 << nv)) for nv in nvs]
+    ops = []
+    for i, (nv, m) in enumerate(zip(nvs, mles)):
+        if i % 2:
+            ops.append(("open", m, fr_from_bytes(rng.fr_bytes(nv).tobytes()) if nv else []))
+        else:
+            ops.append(("commit", m))
+    ops.append(("commit", mles[0]))  # a repeat
+    got = backend.kzg_batch(pp, ops)
+    for op, g in zip(ops, got):
+        if op[0] == "commit":
+            assert g[0] == backend.commit_packet(pp, op[1])
+        else:
+            assert (g[0], g[1]) == backend.open_packets(pp, op[1], op[2])
+    with pytest.raises(InvalidParameters):
+        backend.kzg_batch(pp, [("commit", mles[0]), ("open", mles[2], [1, 2])])
+    assert backend.kzg_batch(pp, []) == []
