@@ -61,9 +61,15 @@ class SplitMix64 {
 
 struct MockCircuit {
   std::vector<Fr> public_inputs;
-  std::vector<std::vector<Fr>> witnesses;
+  std::vector<Column> witnesses;
   HyperPlonkIndex index;
 };
+
+inline Column encode_column(const std::vector<Fr>& v) {
+  Column out(v.size());
+  for (size_t i = 0; i < v.size(); ++i) out[i] = v[i].bytes();
+  return out;
+}
 
 inline Fr monomial_value(const Monomial& g, const std::vector<std::vector<Fr>>& sel,
                          const std::vector<std::vector<Fr>>& wit, size_t row, bool with_selector) {
@@ -83,32 +89,48 @@ inline MockCircuit mock_circuit(size_t nv, const CustomizedGates& gate, size_t s
   std::vector<uint64_t> sigma(slots);
   for (size_t i = 0; i < slots; ++i) sigma[i] = i;
   for (size_t i = slots; i > 1; --i) std::swap(sigma[i - 1], sigma[rng.below(i)]);
-  MockCircuit c;
-  c.witnesses.assign(nw, std::vector<Fr>(n));
+  std::vector<std::vector<Fr>> wit(nw, std::vector<Fr>(n));
   std::vector<uint8_t> seen(slots, 0);
   for (size_t s = 0; s < slots; ++s) {
     if (seen[s]) continue;
     const Fr v = rng.nonzero();
     for (size_t j = s; !seen[j]; j = sigma[j]) {
       seen[j] = 1;
-      c.witnesses[j / n][j % n] = v;
+      wit[j / n][j % n] = v;
     }
   }
-  c.index.selectors.assign(ns, std::vector<Fr>(n));
+  std::vector<std::vector<Fr>> sel(ns, std::vector<Fr>(n));
   const Monomial& sg = gate.gates.at(solved);
+  std::vector<Fr> residual(n), m(n);
   for (size_t row = 0; row < n; ++row) {
     for (size_t s = 0; s < ns; ++s)
-      if (s != *sg.selector) c.index.selectors[s][row] = rng.field();
-    Fr residual = Fr::zero();
+      if (s != *sg.selector) sel[s][row] = rng.field();
+    Fr r = Fr::zero();
     for (size_t gi = 0; gi < gate.gates.size(); ++gi)
-      if (gi != solved) residual += monomial_value(gate.gates[gi], c.index.selectors, c.witnesses, row, true);
-    const Fr m = monomial_value(sg, c.index.selectors, c.witnesses, row, false);
-    c.index.selectors[*sg.selector][row] = -(residual * m.inverse());
+      if (gi != solved) r += monomial_value(gate.gates[gi], sel, wit, row, true);
+    residual[row] = r;
+    m[row] = monomial_value(sg, sel, wit, row, false);
   }
+  // batch inversion of the solved monomials (all non-zero)
+  std::vector<Fr> prefix(n);
+  Fr acc = Fr::one();
+  for (size_t row = 0; row < n; ++row) {
+    prefix[row] = acc;
+    acc *= m[row];
+  }
+  Fr inv = acc.inverse();
+  for (size_t row = n; row-- > 0;) {
+    const Fr mi = inv * prefix[row];
+    inv *= m[row];
+    sel[*sg.selector][row] = -(residual[row] * mi);
+  }
+  MockCircuit c;
+  for (const auto& w : wit) c.witnesses.push_back(encode_column(w));
+  for (const auto& s : sel) c.index.selectors.push_back(encode_column(s));
   c.index.permutation.resize(slots);
-  for (size_t i = 0; i < slots; ++i) c.index.permutation[i] = Fr::from_u64(sigma[i]);
+  for (size_t i = 0; i < slots; ++i) c.index.permutation[i] = Fr::from_u64(sigma[i]).bytes();
   const size_t pub = std::min<size_t>(4, n);
-  c.public_inputs.assign(c.witnesses[0].begin(), c.witnesses[0].begin() + pub);
+  c.public_inputs.assign(wit[0].begin(), wit[0].begin() + pub);
   c.index.params.num_constraints = n;
   c.index.params.num_pub_input = pub;
   c.index.params.gate_func = gate;
@@ -125,9 +147,15 @@ inline MockCircuit jellyfish_circuit(size_t nv, uint64_t seed) {
 
 inline bool is_satisfied(const MockCircuit& c) {
   const auto& g = c.index.params.gate_func;
+  auto decode = [](const std::vector<Column>& cols) {
+    std::vector<std::vector<Fr>> out;
+    for (const auto& col : cols) out.push_back(from_wire(col));
+    return out;
+  };
+  const auto sel = decode(c.index.selectors), wit = decode(c.witnesses);
   for (size_t row = 0; row < c.index.params.num_constraints; ++row) {
     Fr acc = Fr::zero();
-    for (const auto& m : g.gates) acc += monomial_value(m, c.index.selectors, c.witnesses, row, true);
+    for (const auto& m : g.gates) acc += monomial_value(m, sel, wit, row, true);
     if (!acc.is_zero()) return false;
   }
   return true;

@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -70,9 +71,15 @@ class Context {
   Context(const Context&) = delete;
   Context& operator=(const Context&) = delete;
   HpMi355xContext* raw() const { return raw_; }
+  // The native context is single-threaded: every call into it, including the
+  // retirement of handles dropped on other threads, holds this lock.
+  std::unique_lock<std::recursive_mutex> guard() const {
+    return std::unique_lock<std::recursive_mutex>(mu_);
+  }
 
  private:
   HpMi355xContext* raw_ = nullptr;
+  mutable std::recursive_mutex mu_;
 };
 
 // Mle trait: shared handle to an immutable resident table.
@@ -92,7 +99,10 @@ class Mle {
     HpMi355xDeviceAddress address;
     size_t num_vars;
     ~State() {
-      if (address) (void)hp_mi355x_v1_buffer_retire(ctx->raw(), address);
+      if (address) {
+        const auto g = ctx->guard();
+        (void)hp_mi355x_v1_buffer_retire(ctx->raw(), address);
+      }
     }
   };
   std::shared_ptr<State> state_;
@@ -112,7 +122,10 @@ class PreparedProverParam {
     HpMi355xKzgParams* raw;
     size_t num_vars;
     ~State() {
-      if (raw) (void)hp_mi355x_v1_multilinear_kzg_params_destroy(ctx->raw(), raw);
+      if (raw) {
+        const auto g = ctx->guard();
+        (void)hp_mi355x_v1_multilinear_kzg_params_destroy(ctx->raw(), raw);
+      }
     }
   };
   std::shared_ptr<State> state_;
@@ -129,7 +142,10 @@ class ProverState {
     HpMi355xSumCheckState* raw;
     size_t max_degree;
     ~State() {
-      if (raw) (void)hp_mi355x_v1_sumcheck_prover_state_destroy(ctx->raw(), raw);
+      if (raw) {
+        const auto g = ctx->guard();
+        (void)hp_mi355x_v1_sumcheck_prover_state_destroy(ctx->raw(), raw);
+      }
     }
   };
   std::shared_ptr<State> state_;
@@ -146,6 +162,7 @@ class Mi355xBackend {
 
   // ---- MleBackend --------------------------------------------------------
   Mle mle_from_evaluations(size_t num_vars, const std::vector<Fr>& evaluations) const {
+    const auto lock = ctx_->guard();
     HpMi355xDeviceAddress out = nullptr;
     check(hp_mi355x_v1_mle_upload(ctx_->raw(), num_vars,
                                   evaluations.empty() ? nullptr : evaluations[0].data(),
@@ -165,6 +182,7 @@ class Mi355xBackend {
 
   Mle linear_combination(const std::vector<Mle>& polynomials,
                          const std::vector<Fr>& coefficients) const {
+    const auto lock = ctx_->guard();
     if (polynomials.empty()) invalid("linear combination requires nonempty polynomials");
     if (polynomials.size() != coefficients.size())
       invalid("linear combination requires matching counts and dimensions");
@@ -183,6 +201,7 @@ class Mi355xBackend {
   }
 
   Fr evaluate_mle(const Mle& mle, const std::vector<Fr>& point) const {
+    const auto lock = ctx_->guard();
     own(mle);
     if (point.size() != mle.num_vars()) invalid("wrong number of variables");
     Fr out{};
@@ -195,6 +214,7 @@ class Mi355xBackend {
   // extension: many evaluations in one pass (mle_evaluate_batch)
   std::vector<Fr> evaluate_mles(const std::vector<Mle>& mles,
                                 const std::vector<std::vector<Fr>>& points) const {
+    const auto lock = ctx_->guard();
     if (mles.size() != points.size()) invalid("one point per polynomial");
     std::vector<Fr> out(mles.size());
     if (mles.empty()) return out;
@@ -215,6 +235,7 @@ class Mi355xBackend {
   }
 
   Mle build_eq_x_r(const std::vector<Fr>& r) const {
+    const auto lock = ctx_->guard();
     if (r.empty()) invalid("r length is 0");
     HpMi355xDeviceAddress out = nullptr;
     check(hp_mi355x_v1_mle_build_eq_x_r(ctx_->raw(), r[0].data(), r.size(), &out), "build_eq_x_r");
@@ -222,6 +243,7 @@ class Mi355xBackend {
   }
 
   Mle compute_frac_poly(const std::vector<Mle>& fxs, const std::vector<Mle>& gxs) const {
+    const auto lock = ctx_->guard();
     if (fxs.empty() || fxs.size() != gxs.size()) invalid("frac requires equal nonempty lists");
     HpMi355xDeviceAddress out = nullptr;
     const auto f = addresses(fxs), g = addresses(gxs);
@@ -232,6 +254,7 @@ class Mi355xBackend {
   }
 
   Mle compute_product_poly(const Mle& frac) const {
+    const auto lock = ctx_->guard();
     own(frac);
     HpMi355xDeviceAddress out = nullptr;
     check(hp_mi355x_v1_mle_compute_product_poly(ctx_->raw(), frac.address(), frac.num_vars(), &out),
@@ -242,6 +265,7 @@ class Mi355xBackend {
   std::pair<std::vector<Mle>, std::vector<Mle>> compute_nums_and_denoms(
       const Fr& beta, const Fr& gamma, const std::vector<Mle>& fxs, const std::vector<Mle>& gxs,
       const std::vector<Mle>& perms) const {
+    const auto lock = ctx_->guard();
     const size_t k = fxs.size();
     if (k == 0 || gxs.size() != k || perms.size() != k) invalid("nums/denoms require equal lists");
     const size_t nv = fxs[0].num_vars();
@@ -260,6 +284,7 @@ class Mi355xBackend {
   }
 
   std::array<Mle, 2> compute_product_factors(const Mle& frac, const Mle& prod) const {
+    const auto lock = ctx_->guard();
     own(frac);
     own(prod);
     HpMi355xDeviceAddress a = nullptr, b = nullptr;
@@ -273,6 +298,7 @@ class Mi355xBackend {
   // Layer i holds 2^(num_vars - i) concatenated 97-byte packets.
   PreparedProverParam prepare_prover_param(size_t num_vars,
                                            const std::vector<std::vector<uint8_t>>& layers) const {
+    const auto lock = ctx_->guard();
     if (layers.size() != num_vars + 1) invalid("prover parameters need num_vars + 1 layers");
     PreparedProverParam pp = new_params(num_vars);
     for (size_t i = 0; i < layers.size(); ++i)
@@ -283,6 +309,7 @@ class Mi355xBackend {
   }
 
   PreparedProverParam generate_prover_param(size_t num_vars, const std::vector<Fr>& t) const {
+    const auto lock = ctx_->guard();
     if (t.size() != num_vars) invalid("trapdoor length must equal num_vars");
     PreparedProverParam pp = new_params(num_vars);
     check(hp_mi355x_v1_multilinear_kzg_params_generate(ctx_->raw(), pp.state_->raw,
@@ -292,6 +319,7 @@ class Mi355xBackend {
   }
 
   G1Packet commit(const PreparedProverParam& pp, const Mle& polynomial) const {
+    const auto lock = ctx_->guard();
     own(polynomial);
     G1Packet out{};
     check(hp_mi355x_v1_multilinear_kzg_commit(ctx_->raw(), pp.raw(), polynomial.address(),
@@ -302,6 +330,7 @@ class Mi355xBackend {
 
   std::vector<G1Packet> multi_commit(const PreparedProverParam& pp,
                                      const std::vector<Mle>& polynomials) const {
+    const auto lock = ctx_->guard();
     std::vector<G1Packet> out(polynomials.size());
     if (polynomials.empty()) return out;
     const auto addrs = addresses(polynomials);
@@ -315,6 +344,7 @@ class Mi355xBackend {
 
   std::pair<std::vector<G1Packet>, Fr> open(const PreparedProverParam& pp, const Mle& polynomial,
                                            const std::vector<Fr>& point) const {
+    const auto lock = ctx_->guard();
     own(polynomial);
     const size_t nv = polynomial.num_vars();
     if (point.size() != nv) invalid("polynomial num_vars does not match point length");
@@ -338,6 +368,7 @@ class Mi355xBackend {
     Fr value{};                    // opens only
   };
   std::vector<KzgResult> kzg_batch(const PreparedProverParam& pp, const std::vector<KzgOp>& ops) const {
+    const auto lock = ctx_->guard();
     std::vector<KzgResult> res(ops.size());
     std::vector<HpMi355xKzgRequest> reqs(ops.size());
     for (size_t i = 0; i < ops.size(); ++i) {
@@ -362,6 +393,7 @@ class Mi355xBackend {
   // extension: one round of several states in shared launches
   std::vector<std::vector<Fr>> prove_rounds_batch(std::vector<ProverState*>& states,
                                                   const std::vector<std::optional<Fr>>& challenges) const {
+    const auto lock = ctx_->guard();
     const size_t n = states.size();
     if (challenges.size() != n) invalid("one challenge slot per state");
     std::vector<std::vector<Fr>> out(n);
@@ -385,6 +417,7 @@ class Mi355xBackend {
 
   // ---- SumCheckProver ----------------------------------------------------
   ProverState prover_init(const VirtualPolynomial& poly) const {
+    const auto lock = ctx_->guard();
     const size_t nv = poly.aux_info.num_variables, d = poly.aux_info.max_degree;
     if (nv == 0 || d == 0) invalid("SumCheck requires positive variable count and degree");
     std::vector<uint8_t> coeffs;
@@ -414,6 +447,7 @@ class Mi355xBackend {
 
   std::vector<Fr> prove_round_and_update_state(ProverState& state,
                                                const std::optional<Fr>& challenge) const {
+    const auto lock = ctx_->guard();
     std::vector<Fr> out(state.max_degree() + 1);
     check(hp_mi355x_v1_sumcheck_prover_state_round(
               ctx_->raw(), state.state_->raw, challenge ? challenge->data() : nullptr,

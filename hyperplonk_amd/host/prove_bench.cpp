@@ -6,6 +6,12 @@
 //   sumcheck  <nv> <2|5> <reps> <seed>
 //             whole sumcheck proof with host Fiat-Shamir: from before
 //             prover_init through every round to the state drop
+//   serve     <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>
+//             a request trace (Poisson arrivals at `rate` per second; 0 = all at
+//             once; auto = 0.8 x the batched capacity measured on a closed-loop
+//             prefix) served with continuous batching and again serially
+//             (FIFO, one proof at a time); latency = completion - arrival;
+//             every proof of the two runs must be identical
 // Prints one JSON object per run on stdout.
 #include <algorithm>
 #include <chrono>
@@ -16,6 +22,7 @@
 
 #include "circuit.hpp"
 #include "mi355x_backend.hpp"
+#include "serve.hpp"
 
 using namespace hyperplonk;
 using namespace hyperplonk::host;
@@ -126,12 +133,106 @@ static int run_sumcheck(size_t nv, int degree, int reps, uint64_t seed) {
   return 0;
 }
 
+struct Summary {
+  double proofs_per_s = 0, p50 = 0, p99 = 0, mean = 0;
+  size_t errors = 0;
+};
+
+static Summary summarize(const std::vector<serve::Request>& trace, const serve::ServeReport& rep) {
+  Summary s;
+  std::vector<double> lat;
+  for (size_t i = 0; i < trace.size(); ++i) {
+    const auto& o = rep.outcomes[i];
+    if (!o.error.empty()) ++s.errors;
+    lat.push_back(1e3 * (o.done_s - trace[i].arrival_s));
+  }
+  s.proofs_per_s = trace.size() / rep.span_s;
+  s.p50 = percentile(lat, 0.5);
+  s.p99 = percentile(lat, 0.99);
+  for (double x : lat) s.mean += x / lat.size();
+  return s;
+}
+
+static double busy_ms(const serve::ServeReport& rep) {
+  double t = 0;
+  for (const auto& o : rep.outcomes) t += 1e3 * (o.done_s - o.start_s);
+  return t;
+}
+
+static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, size_t nv_max, double jelly,
+                     size_t slots, uint64_t seed) {
+  mi355x::Mi355xBackend be(0);
+  const size_t N = nv_max;
+  SplitMix64 trng(seed ^ 0x5352535f54415553ULL);
+  std::vector<Fr> t;
+  for (size_t i = 0; i < N; ++i) t.push_back(trng.field());
+  auto t0 = Clock::now();
+  const auto pp = be.generate_prover_param(N, to_wire(t));
+  const double srs_ms = ms_since(t0);
+  const auto vk = MultilinearVerifierParam::from_trapdoor(t);
+  serve::ServeConfig cfg;
+  cfg.slots = slots;
+  cfg.gen_threads = std::max<unsigned>(2, std::min<unsigned>(16, std::thread::hardware_concurrency()) - 2);
+  double rate = 0;
+  double capacity = 0;
+  if (rate_arg == "auto") {
+    // closed-loop prefix: everything at t = 0, batched
+    const size_t probe = std::min<size_t>(count, std::max<size_t>(4 * slots, 64));
+    const auto ptrace = serve::make_trace(probe, 0, nv_min, nv_max, jelly, seed ^ 0x70726f6265ULL);
+    const auto prep = serve::run_trace(be, pp, vk, ptrace, cfg);
+    capacity = probe / prep.span_s;
+    rate = 0.8 * capacity;
+  } else {
+    rate = std::atof(rate_arg.c_str());
+  }
+  const auto trace = serve::make_trace(count, rate, nv_min, nv_max, jelly, seed);
+  serve::ServeConfig batched = cfg;
+  const auto brep = serve::run_trace(be, pp, vk, trace, batched);
+  serve::ServeConfig serial = cfg;
+  serial.batched = false;
+  const auto srep = serve::run_trace(be, pp, vk, trace, serial);
+  size_t identical = 0;
+  for (size_t i = 0; i < trace.size(); ++i)
+    identical += brep.outcomes[i].error.empty() && brep.outcomes[i].proof == srep.outcomes[i].proof;
+  // verify the proofs of the first few requests on the host
+  size_t verified = 0, checked = 0;
+  for (size_t i = 0; i < trace.size() && checked < 3; ++i) {
+    if (trace[i].nv > 14) continue;
+    const auto c = serve::circuit_of(trace[i]);
+    auto [pk, vkey] = preprocess(be, c.index, pp, vk);
+    verified += verify(vkey, c.public_inputs, prove(be, pk, c.public_inputs, c.witnesses));
+    ++checked;
+  }
+  const Summary b = summarize(trace, brep), s = summarize(trace, srep);
+  size_t jelly_n = 0;
+  for (const auto& r : trace) jelly_n += r.jellyfish;
+  std::printf(
+      "{\"mode\": \"serve\", \"requests\": %zu, \"jellyfish\": %zu, \"nv\": [%zu, %zu], \"rate_per_s\": %.3f, "
+      "\"capacity_probe_per_s\": %.3f, \"slots\": %zu, \"srs_num_vars\": %zu, \"srs_ms\": %.1f, "
+      "\"batched\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
+      "\"errors\": %zu, \"max_concurrent\": %zu, \"ticks\": %zu, \"ops\": %zu, \"kzg_calls\": %zu, "
+      "\"round_calls\": %zu, \"eval_calls\": %zu, \"device_ms\": {\"gather\": %.1f, \"generic\": %.1f, "
+      "\"kzg\": %.1f, \"rounds\": %.1f, \"evals\": %.1f}, \"request_wait_ms\": %.1f, \"request_total_ms\": %.1f}, "
+      "\"serial\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
+      "\"errors\": %zu}, \"identical_proofs\": %zu, \"verified\": %zu, \"verify_checked\": %zu}\n",
+      trace.size(), jelly_n, nv_min, nv_max, rate, capacity, slots, N, srs_ms, b.proofs_per_s, b.p50, b.p99,
+      b.mean, b.errors, brep.max_concurrent, brep.stats.ticks, brep.stats.ops, brep.stats.kzg_calls,
+      brep.stats.round_calls, brep.stats.eval_calls, brep.stats.gather_ms, brep.stats.generic_ms,
+      brep.stats.kzg_ms, brep.stats.round_ms, brep.stats.eval_ms, brep.stats.waiting_ms, busy_ms(brep), s.proofs_per_s, s.p50, s.p99, s.mean, s.errors, identical,
+      verified, checked);
+  return identical == trace.size() && verified == checked ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "prove" && argc == 7)
       return run_prove(std::strtoul(argv[2], nullptr, 10), argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
                        std::strtoull(argv[6], nullptr, 0));
+    if (mode == "serve" && argc == 9)
+      return run_serve(std::strtoul(argv[2], nullptr, 10), argv[3], std::strtoul(argv[4], nullptr, 10),
+                       std::strtoul(argv[5], nullptr, 10), std::atof(argv[6]), std::strtoul(argv[7], nullptr, 10),
+                       std::strtoull(argv[8], nullptr, 0));
     if (mode == "sumcheck" && argc == 6)
       return run_sumcheck(std::strtoul(argv[2], nullptr, 10), std::atoi(argv[3]), std::atoi(argv[4]),
                           std::strtoull(argv[5], nullptr, 0));
@@ -141,7 +242,8 @@ int main(int argc, char** argv) {
   }
   std::fprintf(stderr,
                "usage: %s prove <nv> <vanilla|jellyfish> <proofs> <warmup> <seed>\n"
-               "       %s sumcheck <nv> <2|5> <reps> <seed>\n",
-               argv[0], argv[0]);
+               "       %s sumcheck <nv> <2|5> <reps> <seed>\n"
+               "       %s serve <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>\n",
+               argv[0], argv[0], argv[0]);
   return 2;
 }

@@ -1,0 +1,551 @@
+// Prover service on one MI355X: continuous batching of independent proofs.
+//
+// Every admitted request runs the ordinary protocol (snark.hpp: preprocess +
+// prove) on a worker thread of its own against BatchingBackend, which has the
+// backend interface of Mi355xBackend:
+//   * every call is handed to the Scheduler, whose device thread gathers what
+//     every active request is waiting for (until all of them wait, or a short
+//     window passes) and runs one tick: the table-producing calls (uploads,
+//     linear combinations, eq tables, frac, product, nums/denoms, factors,
+//     prover_init), which only queue kernels, one after another, then each
+//     kind of result-returning call as ONE batched call:
+//     multilinear_kzg_batch (all MSMs of all requests in one launch sequence),
+//     sumcheck round_batch (all rounds in shared launches), evaluate_batch.
+// Requests arrive on a trace; a request is admitted when a slot is free and
+// its estimated device footprint fits the HBM budget, so new requests join
+// the batches of those already on the GPU.  A proof depends only on its own
+// request, so it is byte-identical alone and in any mix.
+#pragma once
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "circuit.hpp"
+#include "mi355x_backend.hpp"
+
+namespace hyperplonk {
+namespace serve {
+
+using host::Fr;
+using mi355x::Mi355xBackend;
+
+class Scheduler {
+ public:
+  Scheduler(const Mi355xBackend& be, std::chrono::microseconds window)
+      : be_(be), window_(window), thread_([this] { loop(); }) {}
+  ~Scheduler() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_submit_.notify_all();
+    thread_.join();
+  }
+
+  void join() {
+    std::lock_guard<std::mutex> g(mu_);
+    ++active_;
+  }
+  void leave() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      --active_;
+    }
+    cv_submit_.notify_all();
+  }
+
+  std::vector<FrBytes> round(mi355x::ProverState& st, const std::optional<FrBytes>& ch) {
+    Pending p;
+    p.kind = Kind::Round;
+    p.state = &st;
+    p.challenge = ch;
+    submit(p);
+    return std::move(p.messages);
+  }
+  std::vector<Mi355xBackend::KzgResult> kzg(const mi355x::PreparedProverParam& pp,
+                                            std::vector<Mi355xBackend::KzgOp> ops) {
+    Pending p;
+    p.kind = Kind::Kzg;
+    p.pp = &pp;
+    p.kzg_ops = std::move(ops);
+    submit(p);
+    return std::move(p.kzg_results);
+  }
+  std::vector<FrBytes> evaluate(std::vector<mi355x::Mle> mles, std::vector<std::vector<FrBytes>> points) {
+    Pending p;
+    p.kind = Kind::Eval;
+    p.mles = std::move(mles);
+    p.points = std::move(points);
+    submit(p);
+    return std::move(p.messages);
+  }
+
+  // any other backend call, run on the device thread in the next tick
+  template <class F>
+  auto call(F&& f) -> decltype(f()) {
+    std::optional<decltype(f())> result;
+    Pending p;
+    p.kind = Kind::Generic;
+    p.fn = [&] { result.emplace(f()); };
+    submit(p);
+    return std::move(*result);
+  }
+
+  struct Stats {
+    size_t ticks = 0, ops = 0, kzg_calls = 0, round_calls = 0, eval_calls = 0;
+    double gather_ms = 0, generic_ms = 0, kzg_ms = 0, round_ms = 0, eval_ms = 0;  // device thread
+    double waiting_ms = 0;  // request threads, summed: from submit to result
+  };
+  Stats stats() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return stats_;
+  }
+
+ private:
+  enum class Kind { Round, Kzg, Eval, Generic };
+  struct Pending {
+    Kind kind;
+    std::function<void()> fn;
+    mi355x::ProverState* state = nullptr;
+    std::optional<FrBytes> challenge;
+    const mi355x::PreparedProverParam* pp = nullptr;
+    std::vector<Mi355xBackend::KzgOp> kzg_ops;
+    std::vector<mi355x::Mle> mles;
+    std::vector<std::vector<FrBytes>> points;
+    std::vector<FrBytes> messages;
+    std::vector<Mi355xBackend::KzgResult> kzg_results;
+    std::exception_ptr error;
+    bool done = false;
+  };
+
+  static double ms(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  }
+
+  void submit(Pending& p) {
+    const auto t = std::chrono::steady_clock::now();
+    std::unique_lock<std::mutex> lk(mu_);
+    queue_.push_back(&p);
+    cv_submit_.notify_all();
+    cv_done_.wait(lk, [&] { return p.done; });
+    stats_.waiting_ms += ms(t, std::chrono::steady_clock::now());
+    if (p.error) std::rethrow_exception(p.error);
+  }
+
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_submit_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+      if (queue_.empty()) return;  // stop requested and nothing pending
+      // gather: until every active request waits here, or the window passes
+      const auto g0 = std::chrono::steady_clock::now();
+      cv_submit_.wait_until(lk, g0 + window_, [&] { return queue_.size() >= active_; });
+      stats_.gather_ms += ms(g0, std::chrono::steady_clock::now());
+      std::vector<Pending*> batch(queue_.begin(), queue_.end());
+      queue_.clear();
+      ++stats_.ticks;
+      stats_.ops += batch.size();
+      lk.unlock();
+      execute(batch);
+      lk.lock();
+      for (Pending* p : batch) p->done = true;
+      cv_done_.notify_all();
+    }
+  }
+
+  template <class F>
+  static void guarded(const std::vector<Pending*>& group, F&& f) {
+    try {
+      f();
+    } catch (...) {
+      for (Pending* p : group) p->error = std::current_exception();
+    }
+  }
+
+  void execute(const std::vector<Pending*>& batch) {
+    std::vector<Pending*> rounds, evals;
+    std::map<const mi355x::PreparedProverParam*, std::vector<Pending*>> kzgs;
+    for (Pending* p : batch) {
+      if (p->kind == Kind::Round) rounds.push_back(p);
+      else if (p->kind == Kind::Eval) evals.push_back(p);
+      else if (p->kind == Kind::Kzg) kzgs[p->pp].push_back(p);
+    }
+    // calls that only queue device work first, then the batched groups
+    auto t = std::chrono::steady_clock::now();
+    auto lap = [&](double Stats::*field) {
+      const auto now = std::chrono::steady_clock::now();
+      std::lock_guard<std::mutex> g(mu_);
+      stats_.*field += ms(t, now);
+      t = now;
+    };
+    for (Pending* p : batch)
+      if (p->kind == Kind::Generic) guarded({p}, [&] { p->fn(); });
+    lap(&Stats::generic_ms);
+    for (auto& [pp, group] : kzgs) {
+      guarded(group, [&] {
+        std::vector<Mi355xBackend::KzgOp> ops;
+        for (Pending* p : group) ops.insert(ops.end(), p->kzg_ops.begin(), p->kzg_ops.end());
+        auto res = be_.kzg_batch(*pp, ops);
+        size_t k = 0;
+        for (Pending* p : group)
+          for (size_t i = 0; i < p->kzg_ops.size(); ++i) p->kzg_results.push_back(std::move(res[k++]));
+      });
+      lap(&Stats::kzg_ms);
+      std::lock_guard<std::mutex> g(mu_);
+      ++stats_.kzg_calls;
+    }
+    if (!rounds.empty()) {
+      guarded(rounds, [&] {
+        std::vector<mi355x::ProverState*> states;
+        std::vector<std::optional<FrBytes>> chs;
+        for (Pending* p : rounds) {
+          states.push_back(p->state);
+          chs.push_back(p->challenge);
+        }
+        auto res = be_.prove_rounds_batch(states, chs);
+        for (size_t i = 0; i < rounds.size(); ++i) rounds[i]->messages = std::move(res[i]);
+      });
+      lap(&Stats::round_ms);
+      std::lock_guard<std::mutex> g(mu_);
+      ++stats_.round_calls;
+    }
+    if (!evals.empty()) {
+      guarded(evals, [&] {
+        std::vector<mi355x::Mle> mles;
+        std::vector<std::vector<FrBytes>> pts;
+        for (Pending* p : evals) {
+          mles.insert(mles.end(), p->mles.begin(), p->mles.end());
+          pts.insert(pts.end(), p->points.begin(), p->points.end());
+        }
+        auto res = be_.evaluate_mles(mles, pts);
+        size_t k = 0;
+        for (Pending* p : evals)
+          for (size_t i = 0; i < p->mles.size(); ++i) p->messages.push_back(res[k++]);
+      });
+      lap(&Stats::eval_ms);
+      std::lock_guard<std::mutex> g(mu_);
+      ++stats_.eval_calls;
+    }
+  }
+
+  const Mi355xBackend& be_;
+  std::chrono::microseconds window_;
+  mutable std::mutex mu_;
+  std::condition_variable cv_submit_, cv_done_;
+  std::deque<Pending*> queue_;
+  size_t active_ = 0;
+  bool stop_ = false;
+  Stats stats_;
+  std::thread thread_;
+};
+
+// The backend interface of Mi355xBackend, with the waiting calls batched
+// across requests through a Scheduler.
+class BatchingBackend {
+ public:
+  using Mle = mi355x::Mle;
+  using VirtualPolynomial = mi355x::VirtualPolynomial;
+  using PreparedProverParam = mi355x::PreparedProverParam;
+  using ProverState = mi355x::ProverState;
+
+  BatchingBackend(const Mi355xBackend& be, Scheduler& s) : be_(be), s_(s) {}
+
+  Mle mle_from_evaluations(size_t nv, const std::vector<FrBytes>& e) const {
+    return s_.call([&] { return be_.mle_from_evaluations(nv, e); });
+  }
+  Mle linear_combination(const std::vector<Mle>& p, const std::vector<FrBytes>& c) const {
+    return s_.call([&] { return be_.linear_combination(p, c); });
+  }
+  Mle build_eq_x_r(const std::vector<FrBytes>& r) const {
+    return s_.call([&] { return be_.build_eq_x_r(r); });
+  }
+  Mle compute_frac_poly(const std::vector<Mle>& f, const std::vector<Mle>& g) const {
+    return s_.call([&] { return be_.compute_frac_poly(f, g); });
+  }
+  Mle compute_product_poly(const Mle& f) const {
+    return s_.call([&] { return be_.compute_product_poly(f); });
+  }
+  std::pair<std::vector<Mle>, std::vector<Mle>> compute_nums_and_denoms(
+      const FrBytes& beta, const FrBytes& gamma, const std::vector<Mle>& f, const std::vector<Mle>& g,
+      const std::vector<Mle>& p) const {
+    return s_.call([&] { return be_.compute_nums_and_denoms(beta, gamma, f, g, p); });
+  }
+  std::array<Mle, 2> compute_product_factors(const Mle& f, const Mle& p) const {
+    return s_.call([&] { return be_.compute_product_factors(f, p); });
+  }
+  ProverState prover_init(const VirtualPolynomial& vp) const {
+    return s_.call([&] { return be_.prover_init(vp); });
+  }
+
+  std::vector<FrBytes> prove_round_and_update_state(ProverState& st, const std::optional<FrBytes>& ch) const {
+    return s_.round(st, ch);
+  }
+  FrBytes evaluate_mle(const Mle& m, const std::vector<FrBytes>& point) const {
+    return s_.evaluate({m}, {point}).at(0);
+  }
+  std::vector<FrBytes> evaluate_mles(const std::vector<Mle>& ms,
+                                     const std::vector<std::vector<FrBytes>>& points) const {
+    return s_.evaluate(ms, points);
+  }
+  G1Packet commit(const PreparedProverParam& pp, const Mle& m) const {
+    return s_.kzg(pp, {{false, m, {}}}).at(0).points.at(0);
+  }
+  std::vector<G1Packet> multi_commit(const PreparedProverParam& pp, const std::vector<Mle>& ms) const {
+    if (ms.empty()) return {};
+    std::vector<Mi355xBackend::KzgOp> ops;
+    for (const auto& m : ms) ops.push_back({false, m, {}});
+    std::vector<G1Packet> out;
+    for (auto& r : s_.kzg(pp, std::move(ops))) out.push_back(r.points.at(0));
+    return out;
+  }
+  std::pair<std::vector<G1Packet>, FrBytes> open(const PreparedProverParam& pp, const Mle& m,
+                                                  const std::vector<FrBytes>& point) const {
+    auto r = s_.kzg(pp, {{true, m, point}});
+    return {std::move(r.at(0).points), r.at(0).value};
+  }
+
+ private:
+  const Mi355xBackend& be_;
+  Scheduler& s_;
+};
+
+// ------------------------------------------------------------------ traces
+struct Request {
+  size_t id = 0;
+  bool jellyfish = false;
+  size_t nv = 0;
+  uint64_t seed = 0;
+  double arrival_s = 0;  // offset from the start of the run
+};
+
+// Poisson arrivals at `rate` per second (rate <= 0: everything at t = 0),
+// gate jellyfish with probability jelly_frac, nv uniform in [nv_min, nv_max].
+inline std::vector<Request> make_trace(size_t count, double rate, size_t nv_min, size_t nv_max,
+                                       double jelly_frac, uint64_t seed) {
+  host::SplitMix64 rng(seed);
+  std::vector<Request> out;
+  double t = 0;
+  for (size_t i = 0; i < count; ++i) {
+    Request r;
+    r.id = i;
+    const double u = (rng.next() >> 11) * (1.0 / 9007199254740992.0);
+    r.jellyfish = u < jelly_frac;
+    r.nv = nv_min + rng.below(nv_max - nv_min + 1);
+    r.seed = rng.next();
+    if (rate > 0) {
+      const double e = ((rng.next() >> 11) + 1) * (1.0 / 9007199254740993.0);
+      t += -std::log(e) / rate;
+    }
+    r.arrival_s = t;
+    out.push_back(r);
+  }
+  return out;
+}
+
+inline host::MockCircuit circuit_of(const Request& r) {
+  return r.jellyfish ? host::jellyfish_circuit(r.nv, r.seed) : host::vanilla_circuit(r.nv, r.seed);
+}
+
+// Device bytes a request may hold at its peak: its tables (witnesses,
+// selectors, permutation oracles, nums/denoms, frac/prod/factors, eq tables),
+// the sumcheck fold slots and MSM working memory, ~64 tables of 2^nv Fr.
+inline size_t footprint_bytes(const Request& r) { return size_t(64) * 32 * (size_t(1) << r.nv); }
+
+// Circuits built ahead of admission by a pool of threads, at most `ahead`
+// requests past the one admitted last.
+class CircuitPipeline {
+ public:
+  CircuitPipeline(const std::vector<Request>& trace, size_t threads, size_t ahead)
+      : trace_(trace), ahead_(ahead) {
+    for (size_t i = 0; i < threads; ++i) pool_.emplace_back([this] { work(); });
+  }
+  ~CircuitPipeline() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : pool_) t.join();
+  }
+  // block until the first k requests of the trace are built
+  void prefill(size_t k) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&] {
+      for (size_t i = 0; i < k; ++i)
+        if (!ready_.count(i)) return false;
+      return true;
+    });
+  }
+  std::shared_ptr<host::MockCircuit> take(size_t id) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&] { return ready_.count(id) > 0; });
+    auto c = ready_[id];
+    ready_.erase(id);
+    taken_ = std::max(taken_, id + 1);
+    cv_.notify_all();
+    return c;
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      size_t id;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || (next_ < trace_.size() && next_ < taken_ + ahead_); });
+        if (stop_) return;
+        id = next_++;
+      }
+      auto c = std::make_shared<host::MockCircuit>(circuit_of(trace_[id]));
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        ready_[id] = std::move(c);
+      }
+      cv_.notify_all();
+    }
+  }
+  const std::vector<Request>& trace_;
+  size_t ahead_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::map<size_t, std::shared_ptr<host::MockCircuit>> ready_;
+  size_t next_ = 0, taken_ = 0;
+  bool stop_ = false;
+  std::vector<std::thread> pool_;
+};
+
+struct Outcome {
+  double start_s = 0, done_s = 0;  // offsets from the run start
+  std::vector<uint8_t> proof;      // serialized proof
+  std::string error;
+};
+
+struct ServeConfig {
+  size_t slots = 16;
+  size_t hbm_budget = size_t(64) << 30;
+  size_t gen_threads = 8;
+  size_t ahead = 64;
+  std::chrono::microseconds window{300};
+  bool batched = true;  // false: serial FIFO, one request at a time, plain backend
+};
+
+struct ServeReport {
+  std::vector<Outcome> outcomes;
+  double span_s = 0;  // first arrival to last completion
+  Scheduler::Stats stats;
+  size_t max_concurrent = 0;
+};
+
+// Run the trace.  Batched: admission by slots and HBM budget, every admitted
+// request on its own thread through the Scheduler.  Serial: FIFO, one at a
+// time on the plain backend.
+inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProverParam& pp,
+                             const host::MultilinearVerifierParam& vk, const std::vector<Request>& trace,
+                             const ServeConfig& cfg) {
+  using Clock = std::chrono::steady_clock;
+  ServeReport rep;
+  rep.outcomes.resize(trace.size());
+  CircuitPipeline pipe(trace, cfg.gen_threads, cfg.ahead);
+  pipe.prefill(std::min(cfg.ahead, trace.size()));  // inputs of the first requests exist before the clock starts
+  const auto t0 = Clock::now();
+  auto since = [&](Clock::time_point t) { return std::chrono::duration<double>(t - t0).count(); };
+  auto wait_arrival = [&](const Request& r) {
+    const auto at = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(r.arrival_s));
+    std::this_thread::sleep_until(at);
+  };
+  if (!cfg.batched) {
+    for (const Request& r : trace) {
+      auto c = pipe.take(r.id);
+      wait_arrival(r);
+      Outcome& o = rep.outcomes[r.id];
+      o.start_s = since(Clock::now());
+      try {
+        auto [pk, vkey] = host::preprocess(be, c->index, pp, vk);
+        o.proof = host::serialize_proof(host::prove(be, pk, c->public_inputs, c->witnesses));
+      } catch (const std::exception& e) {
+        o.error = e.what();
+      }
+      o.done_s = since(Clock::now());
+    }
+  } else {
+    Scheduler sched(be, cfg.window);
+    BatchingBackend bb(be, sched);
+    std::mutex mu;
+    std::condition_variable cv;
+    size_t running = 0, used = 0;
+    bool closed = false;
+    std::deque<std::pair<const Request*, std::shared_ptr<host::MockCircuit>>> admitted;
+    auto worker = [&] {
+      for (;;) {
+        std::pair<const Request*, std::shared_ptr<host::MockCircuit>> job;
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv.wait(lk, [&] { return closed || !admitted.empty(); });
+          if (admitted.empty()) return;
+          job = admitted.front();
+          admitted.pop_front();
+        }
+        const Request& r = *job.first;
+        Outcome& o = rep.outcomes[r.id];
+        try {
+          auto [pk, vkey] = host::preprocess(bb, job.second->index, pp, vk);
+          o.proof = host::serialize_proof(host::prove(bb, pk, job.second->public_inputs, job.second->witnesses));
+        } catch (const std::exception& e) {
+          o.error = e.what();
+        }
+        job.second.reset();
+        o.done_s = since(Clock::now());
+        sched.leave();
+        {
+          std::lock_guard<std::mutex> g(mu);
+          --running;
+          used -= std::min(footprint_bytes(r), cfg.hbm_budget);
+        }
+        cv.notify_all();
+      }
+    };
+    std::vector<std::thread> pool;
+    for (size_t i = 0; i < cfg.slots; ++i) pool.emplace_back(worker);
+    for (const Request& r : trace) {
+      auto c = pipe.take(r.id);
+      wait_arrival(r);
+      const size_t need = std::min(footprint_bytes(r), cfg.hbm_budget);
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return running < cfg.slots && used + need <= cfg.hbm_budget; });
+      ++running;
+      used += need;
+      rep.max_concurrent = std::max(rep.max_concurrent, running);
+      sched.join();
+      rep.outcomes[r.id].start_s = since(Clock::now());
+      admitted.emplace_back(&r, std::move(c));
+      lk.unlock();
+      cv.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> g(mu);
+      closed = true;
+    }
+    cv.notify_all();
+    for (auto& t : pool) t.join();
+    rep.stats = sched.stats();
+  }
+  double last = 0;
+  for (const auto& o : rep.outcomes) last = std::max(last, o.done_s);
+  rep.span_s = last - (trace.empty() ? 0 : trace.front().arrival_s);
+  return rep;
+}
+
+}  // namespace serve
+}  // namespace hyperplonk

@@ -88,10 +88,14 @@ struct HyperPlonkParams {
   }
 };
 
+// Columns are held in the encoding the backend takes (canonical LE32 Fr), so
+// a proof does no per-entry conversion on the host.
+using Column = std::vector<FrBytes>;
+
 struct HyperPlonkIndex {
   HyperPlonkParams params;
-  std::vector<Fr> permutation;              // #witnesses * 2^nv labels
-  std::vector<std::vector<Fr>> selectors;   // #selectors columns of 2^nv
+  Column permutation;              // #witnesses * 2^nv labels
+  std::vector<Column> selectors;   // #selectors columns of 2^nv
 };
 
 template <class B>
@@ -216,11 +220,11 @@ std::pair<HyperPlonkProvingKey<B>, HyperPlonkVerifyingKey> preprocess(
   pk.pcs_param = &pp;
   const size_t n = size_t(1) << nv;
   for (size_t i = 0; i < index.params.num_witness_columns(); ++i) {
-    std::vector<Fr> col(index.permutation.begin() + i * n, index.permutation.begin() + (i + 1) * n);
-    pk.permutation_oracles.push_back(backend.mle_from_evaluations(nv, to_wire(col)));
+    Column col(index.permutation.begin() + i * n, index.permutation.begin() + (i + 1) * n);
+    pk.permutation_oracles.push_back(backend.mle_from_evaluations(nv, col));
   }
   for (const auto& s : index.selectors)
-    pk.selector_oracles.push_back(backend.mle_from_evaluations(ceil_log2(s.size()), to_wire(s)));
+    pk.selector_oracles.push_back(backend.mle_from_evaluations(ceil_log2(s.size()), s));
   pk.permutation_commitments = backend.multi_commit(pp, pk.permutation_oracles);
   pk.selector_commitments = backend.multi_commit(pp, pk.selector_oracles);
   HyperPlonkVerifyingKey vk;
@@ -235,7 +239,7 @@ std::pair<HyperPlonkProvingKey<B>, HyperPlonkVerifyingKey> preprocess(
 template <class B>
 HyperPlonkProof prove(const B& backend, const HyperPlonkProvingKey<B>& pk,
                       const std::vector<Fr>& pub_input,
-                      const std::vector<std::vector<Fr>>& witnesses) {
+                      const std::vector<Column>& witnesses) {
   using Mle = typename B::Mle;
   IOPTranscript transcript("hyperplonk");
   const HyperPlonkParams& params = pk.params;
@@ -248,7 +252,8 @@ HyperPlonkProof prove(const B& backend, const HyperPlonkProvingKey<B>& pk,
   for (const auto& w : witnesses)
     if (w.size() != params.num_constraints) throw HyperPlonkError("witness length is not correct");
   for (size_t i = 0; i < pub_input.size(); ++i)
-    if (pub_input[i] != witnesses.at(0)[i]) throw HyperPlonkError("public input does not match witness[0]");
+    if (pub_input[i].bytes() != witnesses.at(0)[i])
+      throw HyperPlonkError("public input does not match witness[0]");
 
   const size_t nv = params.num_variables();
   const size_t ell = ceil_log2(params.num_pub_input);
@@ -256,7 +261,7 @@ HyperPlonkProof prove(const B& backend, const HyperPlonkProvingKey<B>& pk,
 
   // 1. witness commitments
   std::vector<Mle> wpolys;
-  for (const auto& w : witnesses) wpolys.push_back(backend.mle_from_evaluations(nv, to_wire(w)));
+  for (const auto& w : witnesses) wpolys.push_back(backend.mle_from_evaluations(nv, w));
   HyperPlonkProof proof;
   proof.witness_commits = backend.multi_commit(pp, wpolys);
   for (const auto& c : proof.witness_commits) transcript.append_message("w", ser_g1(c));

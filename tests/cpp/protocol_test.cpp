@@ -178,8 +178,8 @@ static int prove_case(size_t nv, const std::string& gate, uint64_t seed) {
   }
   {
     // a witness that breaks one gate row cannot produce an accepted proof
-    std::vector<std::vector<Fr>> w = c.witnesses;
-    w[1][3] += Fr::one();
+    std::vector<Column> w = c.witnesses;
+    w[1][3] = (from_wire(w[1][3]) + Fr::one()).bytes();
     bool ok_proof = true;
     try {
       ok_proof = verify(vk, c.public_inputs, prove(be, pk, c.public_inputs, w));
