@@ -11,10 +11,21 @@ Prints one JSON line (the driver contract).  `roofline` describes the dominant
 kernel (bucket accumulation) measured live with HIP events on the stream the
 library launches on; `cpu_baseline` times the CPU oracle's Pippenger on a
 bounded sample (an MSM of 2^18 points) on this host's cores.
+
+Beside the headline the line carries the other workloads of the metric, each
+run by the native prover benchmark (_build/prove_bench, one child process per
+workload): `fold` (HBM rate of the fix_variables fold, HIP events), `proofs`
+(plain one-at-a-time HyperPlonk prove, vanilla and jellyfish, nv 12, with the
+oracle prover as its CPU baseline), `sumcheck` (n = 20 whole proofs, degree 2
+and 5), `batch` (13 uneven commits + 64 sumchecks, serial vs batched, and the
+standalone MSM sizes 2^16..2^22) and `serving` (a Poisson request trace served
+with continuous batching vs serially: proofs/s and p50/p99 latency; the full
+2000-request, nv 10..20 trace with --serve-full).
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -30,6 +41,11 @@ VALU_MUL_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 LIMB_PRODUCTS_PER_FQM = 300
 # XYZZ mixed addition = 8M + 2S Fq products.
 FQM_PER_MIXED_ADD = 10
+# HBM bytes of one 2^20 k_accumulate launch, from the FETCH_SIZE and
+# WRITE_SIZE passes calibrated on a 96 MiB device copy
+# (profiles/k_accumulate_msm_2_20_pmc.md); rocprofv3 counters cannot be read
+# from inside this process.
+ACCUMULATE_TRAFFIC_2_20 = 4.542e9
 
 
 def parse():
@@ -41,7 +57,70 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log", type=int, default=18)
     ap.add_argument("--extra-sizes", default="16,18")
+    ap.add_argument("--skip-workloads", action="store_true", help="MSM headline only")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="serve the full 2000-request trace (nv 10..20) instead of the short one")
     return ap.parse_args()
+
+
+PROVE_BENCH = os.path.join(ROOT, "_build", "prove_bench")
+PROTOCOL_TEST = os.path.join(ROOT, "_build", "protocol_test")
+
+
+def run_json(cmd, timeout):
+    """Run a native benchmark child; its last stdout line is one JSON object."""
+    print(f"[bench] {' '.join(os.path.basename(c) if i == 0 else c for i, c in enumerate(cmd))}",
+          file=sys.stderr, flush=True)
+    try:
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timeout after {timeout}s"}
+    if res.returncode != 0:
+        return {"error": f"exit {res.returncode}: {res.stderr.strip()[-300:]}"}
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def workloads(args, be, rng):
+    out = {}
+    # fold (fix_variables by the lowest variable) of a 2^24 table: 2 reads + 1 write of 32 B per output
+    from hyperplonk_amd.fields import fr_from_bytes
+
+    lg = 24
+    table = be.mle_from_evaluations(lg, rng.fr_bytes(1 << lg))
+    ms = be.fold_profile(table, fr_from_bytes(rng.fr_bytes(1).tobytes())[0], reps=20)
+    moved = 96 * (1 << (lg - 1))
+    out["fold"] = {
+        "table": f"2^{lg}",
+        "ms": round(ms, 4),
+        "bytes": moved,
+        "hbm_GB_per_s": round(moved / (ms * 1e-3) / 1e9, 1),
+        "hbm_peak_GB_per_s": 8000,
+        "frac": round(moved / (ms * 1e-3) / 8e12, 4),
+    }
+    del table
+    if not os.path.exists(PROVE_BENCH):
+        out["error"] = "native prover benchmark not built (make -C hyperplonk_amd/host)"
+        return out
+    out["proofs"] = {
+        "vanilla_nv12": run_json([PROVE_BENCH, "prove", "12", "vanilla", "40", "3", "1"], 300),
+        "jellyfish_nv12": run_json([PROVE_BENCH, "prove", "12", "jellyfish", "20", "3", "2"], 300),
+    }
+    if os.path.exists(PROTOCOL_TEST):
+        threads = min(16, os.cpu_count() or 1)
+        cpu = run_json([PROTOCOL_TEST, "cpu-prove", "12", "vanilla", "3", str(threads)], 600)
+        cpu["kind"] = "port"
+        cpu["sample"] = "3 proofs of the nv-12 vanilla circuit on the CPU oracle backend (MSMs threaded)"
+        out["proofs"]["cpu_baseline"] = cpu
+    out["sumcheck"] = {
+        "degree2_n20": run_json([PROVE_BENCH, "sumcheck", "20", "2", "7", "7"], 300),
+        "degree5_n20": run_json([PROVE_BENCH, "sumcheck", "20", "5", "7", "7"], 300),
+    }
+    out["batch"] = run_json([PROVE_BENCH, "batch", "3", "4"], 600)
+    if args.serve_full:
+        out["serving"] = run_json([PROVE_BENCH, "serve", "2000", "auto", "10", "20", "0.3", "16", "5"], 3000)
+    else:
+        out["serving"] = run_json([PROVE_BENCH, "serve", "300", "auto", "10", "16", "0.3", "16", "5"], 600)
+    return out
 
 
 def reference_conventional_fqm(log_n: int) -> float:
@@ -135,6 +214,8 @@ def main():
             f"{threads} threads, {dt:.2f} s; GPU result byte-identical: {got == want}",
         }
 
+    extra_workloads = None if args.skip_workloads else workloads(args, be, rng)
+
     line = {
         "metric": "MSM pts/s at 2^20",
         "value": round(value, 1),
@@ -166,13 +247,15 @@ def main():
             "peak": round(VALU_MUL_PEAK_T, 2),
             "unit": "T u32-limb-products/s",
             "frac": round(achieved / VALU_MUL_PEAK_T, 4),
-            "traffic": None,
+            "traffic": ACCUMULATE_TRAFFIC_2_20 if log_n == 20 else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/k_accumulate_msm_2_20_pmc.md",
             "kernel_ms": round(acc_ms, 4),
             "stage_ms": {k: round(v, 4) for k, v in stages.items()},
             "whole_msm_conventional_T_per_s": round(whole_msm_rate, 3),
         },
         "cpu_baseline": cpu,
         "extra_sizes": extra,
+        "workloads": extra_workloads,
         "srs_prepare_s": round(srs_seconds, 3),
         "result_prefix": result[:8].hex(),
     }

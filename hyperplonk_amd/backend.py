@@ -192,6 +192,14 @@ class Mi355xBackend:
         check(self.lib.hp_mi355x_v1_mle_evaluate_batch(self.context, addrs, nvs, pts, out, n), "evaluate batch")
         return fr_from_bytes(out.raw[: FR_BYTES * n])
 
+    def fold_profile(self, mle: Mle, r: int, reps: int = 20) -> float:
+        """Mean ms of one fold of `mle` by its lowest variable (extension)."""
+        self._check_mle(mle)
+        ms = ctypes.c_float()
+        check(self.lib.hp_mi355x_v1_mle_fold_profile(self.context, mle.address, mle.num_vars,
+                                                     _fr_scalars([r]), reps, ctypes.byref(ms)), "fold profile")
+        return ms.value
+
     def build_eq_x_r(self, r: Sequence[int]) -> Mle:
         if not r:
             raise InvalidParameters("r length is 0")

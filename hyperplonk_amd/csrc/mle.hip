@@ -530,6 +530,42 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(
   });
 }
 
+// Time `reps` folds of one resident table by its lowest variable with HIP
+// events on the context stream (the fold of fix_variables and of every
+// sumcheck round: 2 reads and 1 write of 32 bytes per output entry).
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_fold_profile(HpMi355xContext* ctx,
+                                                       HpMi355xDeviceAddress polynomial,
+                                                       size_t num_vars, const uint8_t* r,
+                                                       size_t reps, float* ms_per_fold) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (r == nullptr || ms_per_fold == nullptr || reps == 0 || num_vars == 0 || !dims_ok(num_vars))
+      return rt::invalid_parameters();
+    if (!rt::canonical_fr_array(r, 1)) return rt::invalid_parameters();
+    rt::BufferRef table = c.lookup(polynomial, num_vars);
+    if (!table) return rt::invalid_parameters();
+    const size_t half = size_t(1) << (num_vars - 1);
+    rt::BufferRef dr = upload_small(c, r, 1);
+    rt::BufferRef out = c.allocate(half * rt::kFrBytes);
+    hipEvent_t e0, e1;
+    rt::hip_check(hipEventCreate(&e0));
+    rt::hip_check(hipEventCreate(&e1));
+    mle::fold(c, static_cast<const Fr*>(table->ptr), static_cast<Fr*>(out->ptr),
+              static_cast<const Fr*>(dr->ptr), half);  // warm
+    rt::hip_check(hipEventRecord(e0, c.stream));
+    for (size_t i = 0; i < reps; ++i)
+      mle::fold(c, static_cast<const Fr*>(table->ptr), static_cast<Fr*>(out->ptr),
+                static_cast<const Fr*>(dr->ptr), half);
+    rt::hip_check(hipEventRecord(e1, c.stream));
+    rt::hip_check(hipEventSynchronize(e1));
+    float ms = 0;
+    rt::hip_check(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_fold = ms / static_cast<float>(reps);
+    return rt::ok();
+  });
+}
+
 extern "C" HpMi355xStatus hp_mi355x_v1_mle_build_eq_x_r(HpMi355xContext* ctx, const uint8_t* r,
                                                         size_t num_vars,
                                                         HpMi355xDeviceAddress* out) {

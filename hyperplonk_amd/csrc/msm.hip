@@ -285,14 +285,15 @@ __device__ __forceinline__ G1Xyzz shfl_xor_xyzz(const G1Xyzz& v, unsigned mask) 
 
 // Bucket sum of its task partials by a group of 2^lanes_log2 lanes: each lane
 // adds every group-th partial, then a butterfly over the group.  Buckets with
-// more than kSeqMax partials per lane are appended to `heavy` instead and
-// summed by k_heavy_sums.
+// more than seq_max partials per lane (outliers, several times the mean) are
+// appended to `heavy` instead and summed by k_heavy_sums.
 constexpr uint32_t kSeqMax = 8;
 __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
                                                      const uint32_t* ntask,
                                                      const G1Xyzz* partials, uint32_t buckets,
-                                                     uint32_t lanes_log2, G1Xyzz* sums,
-                                                     uint32_t* heavy, uint32_t* heavy_count) {
+                                                     uint32_t lanes_log2, uint32_t seq_max,
+                                                     G1Xyzz* sums, uint32_t* heavy,
+                                                     uint32_t* heavy_count) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t group = 1u << lanes_log2;
   const uint32_t b = gid >> lanes_log2, lane = gid & (group - 1);
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
   G1Xyzz s = G1Xyzz::identity();
   if (live) {
     const uint32_t first = task_off[b], cnt = ntask[b];
-    if (cnt > kSeqMax * group) {
+    if (cnt > seq_max * group) {
       if (lane == 0) heavy[atomicAdd(heavy_count, 1u)] = b;
       live = false;
     } else {
@@ -722,12 +723,17 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
          (double)(2u << lanes_log2) < per_bucket)
     ++lanes_log2;
   const size_t sum_threads = (size_t)G << lanes_log2;
+  // sequential partials per lane before a bucket counts as heavy: at least
+  // kSeqMax and three times the mean, so that only outliers take the
+  // workgroup-per-bucket path
+  const uint32_t seq_max = std::max<uint32_t>(
+      kSeqMax, (uint32_t)std::ceil(3.0 * per_bucket / (double)(1u << lanes_log2)));
   rt::BufferRef heavy_buf = c.allocate(((size_t)G + 1) * sizeof(uint32_t));
   uint32_t* heavy_count = static_cast<uint32_t*>(heavy_buf->ptr);
   uint32_t* heavy = heavy_count + 1;
   zero_u32(c, heavy_count, 1);
   hipLaunchKernelGGL(k_bucket_sums, dim3((unsigned)((sum_threads + 255) / 256)), dim3(256), 0,
-                     c.stream, task_off, ntask, partials, G, lanes_log2, sums, heavy,
+                     c.stream, task_off, ntask, partials, G, lanes_log2, seq_max, sums, heavy,
                      heavy_count);
   rt::check_launch();
   hipLaunchKernelGGL(k_heavy_sums, dim3(256), dim3(kHeavyThreads), 0, c.stream, task_off, ntask,

@@ -6,6 +6,13 @@
 //   sumcheck  <nv> <2|5> <reps> <seed>
 //             whole sumcheck proof with host Fiat-Shamir: from before
 //             prover_init through every round to the state drop
+//   batch     <reps> <seed>
+//             13 MLEs with nv drawn from {14,16,18,20,22} committed against an
+//             nv-22 SRS, plus 64 independent sumchecks (nv in [10,18], degree
+//             2..5, products (c0, f0..f_{d-1}) + (c1, f_d f0), challenges from
+//             each state's own transcript): serially (one commit / one state at
+//             a time) and batched (one multi_commit; every round of all 64
+//             states in shared launches); outputs must be bit-identical
 //   serve     <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>
 //             a request trace (Poisson arrivals at `rate` per second; 0 = all at
 //             once; auto = 0.8 x the batched capacity measured on a closed-loop
@@ -133,6 +140,154 @@ static int run_sumcheck(size_t nv, int degree, int reps, uint64_t seed) {
   return 0;
 }
 
+struct BatchCase {
+  std::vector<mi355x::Mle> commit_mles;
+  std::vector<mi355x::VirtualPolynomial> vps;
+};
+
+static BatchCase make_batch_case(const mi355x::Mi355xBackend& be, uint64_t seed) {
+  SplitMix64 rng(seed);
+  BatchCase bc;
+  const size_t choices[5] = {14, 16, 18, 20, 22};
+  for (int i = 0; i < 13; ++i) {
+    const size_t nv = choices[rng.below(5)];
+    std::vector<FrBytes> tab(size_t(1) << nv);
+    for (auto& x : tab) x = rng.field().bytes();
+    bc.commit_mles.push_back(be.mle_from_evaluations(nv, tab));
+  }
+  for (int i = 0; i < 64; ++i) {
+    const size_t nv = 10 + rng.below(9);
+    const size_t d = 2 + rng.below(4);
+    std::vector<mi355x::Mle> f;
+    for (size_t k = 0; k <= d; ++k) {
+      std::vector<FrBytes> tab(size_t(1) << nv);
+      for (auto& x : tab) x = rng.field().bytes();
+      f.push_back(be.mle_from_evaluations(nv, tab));
+    }
+    mi355x::VirtualPolynomial vp(nv);
+    vp.add_mle_list(std::vector<mi355x::Mle>(f.begin(), f.begin() + d), rng.field().bytes());
+    vp.add_mle_list({f[d], f[0]}, rng.field().bytes());
+    bc.vps.push_back(std::move(vp));
+  }
+  return bc;
+}
+
+// all messages of the 64 sumchecks; batched: rounds of every live state per call
+static std::vector<std::vector<std::vector<Fr>>> run_sumchecks(const mi355x::Mi355xBackend& be,
+                                                               const BatchCase& bc, bool batched) {
+  const size_t n = bc.vps.size();
+  std::vector<std::vector<std::vector<Fr>>> msgs(n);
+  std::vector<IOPTranscript> tr;
+  for (size_t i = 0; i < n; ++i) {
+    tr.emplace_back("batched sumcheck");
+    const uint64_t id = i;
+    tr.back().append_message("state", reinterpret_cast<const uint8_t*>(&id), 8);
+    tr.back().append_message("aux info", ser_aux(bc.vps[i].aux_info));
+  }
+  if (!batched) {
+    for (size_t i = 0; i < n; ++i) {
+      auto st = be.prover_init(bc.vps[i]);
+      std::optional<FrBytes> ch;
+      for (size_t r = 0; r < bc.vps[i].aux_info.num_variables; ++r) {
+        msgs[i].push_back(from_wire(be.prove_round_and_update_state(st, ch)));
+        tr[i].append_message("prover msg", ser_frs(msgs[i].back()));
+        ch = tr[i].get_and_append_challenge("Internal round").bytes();
+      }
+    }
+    return msgs;
+  }
+  std::vector<mi355x::ProverState> states;
+  for (size_t i = 0; i < n; ++i) states.push_back(be.prover_init(bc.vps[i]));
+  std::vector<std::optional<FrBytes>> ch(n);
+  for (size_t r = 0;; ++r) {
+    std::vector<mi355x::ProverState*> live;
+    std::vector<std::optional<FrBytes>> lch;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < n; ++i)
+      if (r < bc.vps[i].aux_info.num_variables) {
+        live.push_back(&states[i]);
+        lch.push_back(ch[i]);
+        idx.push_back(i);
+      }
+    if (live.empty()) break;
+    const auto out = be.prove_rounds_batch(live, lch);
+    for (size_t k = 0; k < idx.size(); ++k) {
+      const size_t i = idx[k];
+      msgs[i].push_back(from_wire(out[k]));
+      tr[i].append_message("prover msg", ser_frs(msgs[i].back()));
+      ch[i] = tr[i].get_and_append_challenge("Internal round").bytes();
+    }
+  }
+  return msgs;
+}
+
+static int run_batch(int reps, uint64_t seed) {
+  mi355x::Mi355xBackend be(0);
+  const size_t N = 22;
+  SplitMix64 trng(seed ^ 0x5352535f54415553ULL);
+  std::vector<Fr> t;
+  for (size_t i = 0; i < N; ++i) t.push_back(trng.field());
+  auto t0 = Clock::now();
+  const auto pp = be.generate_prover_param(N, to_wire(t));
+  const double srs_ms = ms_since(t0);
+  const BatchCase bc = make_batch_case(be, seed);
+  size_t commit_points = 0;
+  for (const auto& m : bc.commit_mles) commit_points += size_t(1) << m.num_vars();
+  std::vector<double> sc_serial, sc_batched, cm_serial, cm_batched;
+  bool same = true;
+  for (int rep = 0; rep < reps + 1; ++rep) {
+    t0 = Clock::now();
+    std::vector<G1Packet> serial_c;
+    for (const auto& m : bc.commit_mles) serial_c.push_back(be.commit(pp, m));
+    const double a = ms_since(t0);
+    t0 = Clock::now();
+    const auto batched_c = be.multi_commit(pp, bc.commit_mles);
+    const double b = ms_since(t0);
+    t0 = Clock::now();
+    const auto ms = run_sumchecks(be, bc, false);
+    const double c = ms_since(t0);
+    t0 = Clock::now();
+    const auto mb = run_sumchecks(be, bc, true);
+    const double d = ms_since(t0);
+    same = same && serial_c == batched_c && ms == mb;
+    if (rep == 0) continue;  // warmup
+    cm_serial.push_back(a);
+    cm_batched.push_back(b);
+    sc_serial.push_back(c);
+    sc_batched.push_back(d);
+  }
+  // standalone MSMs (config 2 sizes) against the same SRS
+  std::string sizes;
+  SplitMix64 mrng(seed ^ 0x4d534dULL);
+  for (size_t lg : {16, 18, 20, 22}) {
+    std::vector<FrBytes> tab(size_t(1) << lg);
+    for (auto& x : tab) x = mrng.field().bytes();
+    const auto m = be.mle_from_evaluations(lg, tab);
+    be.commit(pp, m);
+    std::vector<double> v;
+    for (int r = 0; r < 5; ++r) {
+      t0 = Clock::now();
+      be.commit(pp, m);
+      v.push_back(ms_since(t0));
+    }
+    const double med = percentile(v, 0.5);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s\"2^%zu\": {\"ms\": %.3f, \"pts_per_s\": %.0f}", sizes.empty() ? "" : ", ",
+                  lg, med, (double)(size_t(1) << lg) / (med * 1e-3));
+    sizes += buf;
+  }
+  const double cs = percentile(cm_serial, 0.5), cb = percentile(cm_batched, 0.5);
+  const double ss = percentile(sc_serial, 0.5), sb = percentile(sc_batched, 0.5);
+  std::printf(
+      "{\"mode\": \"batch\", \"reps\": %d, \"commit_points\": %zu, \"srs_ms\": %.1f, "
+      "\"commit_serial_ms\": %.3f, \"commit_batched_ms\": %.3f, \"sumcheck64_serial_ms\": %.3f, "
+      "\"sumcheck64_batched_ms\": %.3f, \"total_serial_ms\": %.3f, \"total_batched_ms\": %.3f, "
+      "\"speedup\": %.3f, \"bit_identical\": %s, \"msm_sizes\": {%s}}\n",
+      reps, commit_points, srs_ms, cs, cb, ss, sb, cs + ss, cb + sb, (cs + ss) / (cb + sb),
+      same ? "true" : "false", sizes.c_str());
+  return same ? 0 : 1;
+}
+
 struct Summary {
   double proofs_per_s = 0, p50 = 0, p99 = 0, mean = 0;
   size_t errors = 0;
@@ -229,6 +384,8 @@ int main(int argc, char** argv) {
     if (mode == "prove" && argc == 7)
       return run_prove(std::strtoul(argv[2], nullptr, 10), argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
                        std::strtoull(argv[6], nullptr, 0));
+    if (mode == "batch" && argc == 4)
+      return run_batch(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 0));
     if (mode == "serve" && argc == 9)
       return run_serve(std::strtoul(argv[2], nullptr, 10), argv[3], std::strtoul(argv[4], nullptr, 10),
                        std::strtoul(argv[5], nullptr, 10), std::atof(argv[6]), std::strtoul(argv[7], nullptr, 10),
@@ -243,7 +400,8 @@ int main(int argc, char** argv) {
   std::fprintf(stderr,
                "usage: %s prove <nv> <vanilla|jellyfish> <proofs> <warmup> <seed>\n"
                "       %s sumcheck <nv> <2|5> <reps> <seed>\n"
+               "       %s batch <reps> <seed>\n"
                "       %s serve <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>\n",
-               argv[0], argv[0], argv[0]);
+               argv[0], argv[0], argv[0], argv[0]);
   return 2;
 }

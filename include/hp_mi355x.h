@@ -189,6 +189,12 @@ HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(HpMi355xContext *ctx,
                                                const size_t *num_vars,
                                                const uint8_t *const *points,
                                                uint8_t *out_evaluations, size_t count);
+/* Time `reps` folds of a resident table by its lowest variable (value r,
+ * one canonical Fr) with HIP events on the context stream; *ms_per_fold gets
+ * the mean (benchmark of the fold's HBM rate). */
+HpMi355xStatus hp_mi355x_v1_mle_fold_profile(HpMi355xContext *ctx,
+                                             HpMi355xDeviceAddress polynomial, size_t num_vars,
+                                             const uint8_t *r, size_t reps, float *ms_per_fold);
 /* One request of multilinear_kzg_batch: a commit (out_points: one packet)
  * or an open at `point` (out_points: num_vars packets, out_evaluation: 32
  * bytes), with the arguments and checks of multilinear_kzg_commit / _open. */

@@ -4,7 +4,10 @@
 //            recomputes with hashlib
 //   prove    HyperPlonk prove + verify of a seeded mock circuit over the oracle
 //            backend, then verification of tampered proofs
-// Usage: protocol_test kat | protocol_test prove <nv> <vanilla|jellyfish> <seed>
+//   cpu-prove  timed proofs on the oracle backend (CPU baseline)
+// Usage: protocol_test kat | prove <nv> <vanilla|jellyfish> <seed> |
+//        cpu-prove <nv> <vanilla|jellyfish> <reps> <threads>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -192,11 +195,31 @@ static int prove_case(size_t nv, const std::string& gate, uint64_t seed) {
   return failures ? 1 : 0;
 }
 
+// Timed prove on the oracle backend: the CPU baseline of the proof benchmark
+static int cpu_prove(size_t nv, const std::string& gate, int reps, int threads) {
+  const MockCircuit c = gate == "vanilla" ? vanilla_circuit(nv, 1) : jellyfish_circuit(nv, 2);
+  SplitMix64 trng(0x5352535f54415553ULL);
+  std::vector<Fr> t;
+  for (size_t i = 0; i < nv; ++i) t.push_back(trng.field());
+  oracle_cpu::OracleBackend be(threads);
+  const auto pp = be.generate_prover_param(nv, to_wire(t));
+  auto [pk, vk] = preprocess(be, c.index, pp, MultilinearVerifierParam::from_trapdoor(t));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < reps; ++i) (void)prove(be, pk, c.public_inputs, c.witnesses);
+  const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::printf("{\"nv\": %zu, \"gate\": \"%s\", \"reps\": %d, \"threads\": %d, \"proofs_per_s\": %.4f, "
+              "\"ms_per_proof\": %.2f}\n",
+              nv, gate.c_str(), reps, threads, reps / s, 1e3 * s / reps);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc >= 2 && std::string(argv[1]) == "kat") {
     kat();
     return failures ? 1 : 0;
   }
+  if (argc >= 6 && std::string(argv[1]) == "cpu-prove")
+    return cpu_prove(std::strtoul(argv[2], nullptr, 10), argv[3], std::atoi(argv[4]), std::atoi(argv[5]));
   if (argc >= 5 && std::string(argv[1]) == "prove")
     return prove_case(std::strtoul(argv[2], nullptr, 10), argv[3], std::strtoull(argv[4], nullptr, 0));
   std::fprintf(stderr, "usage: %s kat | prove <nv> <vanilla|jellyfish> <seed>\n", argv[0]);
