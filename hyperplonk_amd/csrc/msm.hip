@@ -315,29 +315,79 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
   if (live && lane == 0) store_xyzz(sums, b, s);
 }
 
-// Heavy buckets (many partials): one workgroup per bucket, strided sums then
-// an LDS tree.  Grid-strides over the heavy list.
+// Heavy buckets (many partials).  The first kHeavySplitSlots of the heavy
+// list are spread over heavy_split(count) <= kHeavySplit workgroups each
+// (about 4 partials per thread: strided sums, LDS tree, one block partial per
+// workgroup), then k_heavy_final adds the block partials of each with a
+// wavefront butterfly; any further heavy bucket is summed by one workgroup
+// (k_heavy_sums, grid-strided over the list).
 constexpr unsigned kHeavyThreads = 256;
+constexpr unsigned kHeavySplit = 16;
+constexpr unsigned kHeavySplitSlots = 1024;
+
+__device__ __forceinline__ uint32_t heavy_split(uint32_t cnt) {
+  return min(kHeavySplit, max(1u, cnt / (4 * kHeavyThreads)));
+}
+
+__device__ G1Xyzz block_tree_sum(G1Xyzz s) {
+  __shared__ G1Xyzz lds[kHeavyThreads];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (unsigned d = kHeavyThreads / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) {
+      s = add(s, lds[threadIdx.x + d]);
+      lds[threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(kHeavyThreads)
+    k_heavy_partial(const uint32_t* task_off, const uint32_t* ntask, const G1Xyzz* partials,
+                    const uint32_t* heavy, const uint32_t* heavy_count, G1Xyzz* block_sums) {
+  const uint32_t buckets = min(*heavy_count, kHeavySplitSlots);
+  const uint32_t total = buckets * kHeavySplit;
+  // part-major: the first `buckets` work items are part 0 of every heavy bucket
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint32_t h = w % buckets, part = w / buckets;
+    const uint32_t b = heavy[h];
+    const uint32_t first = task_off[b], cnt = ntask[b];
+    const uint32_t split = heavy_split(cnt);
+    if (part >= split) continue;  // uniform per block
+    G1Xyzz s = G1Xyzz::identity();
+    for (uint32_t k = part * kHeavyThreads + threadIdx.x; k < cnt; k += split * kHeavyThreads)
+      s = add(s, load_xyzz(partials, first + k));
+    s = block_tree_sum(s);
+    if (threadIdx.x == 0) store_xyzz(block_sums, h * kHeavySplit + part, s);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(64) k_heavy_final(const uint32_t* ntask, const uint32_t* heavy,
+                                                    const uint32_t* heavy_count,
+                                                    const G1Xyzz* block_sums, G1Xyzz* sums) {
+  const uint32_t total = min(*heavy_count, kHeavySplitSlots);
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t h = blockIdx.x; h < total; h += gridDim.x) {
+    const uint32_t split = heavy_split(ntask[heavy[h]]);
+    G1Xyzz s = lane < split ? load_xyzz(block_sums, h * kHeavySplit + lane) : G1Xyzz::identity();
+    for (unsigned m = 1; m < split; m <<= 1) s = add(s, shfl_xor_xyzz(s, m));
+    if (lane == 0) store_xyzz(sums, heavy[h], s);
+  }
+}
+
 __global__ void __launch_bounds__(kHeavyThreads)
     k_heavy_sums(const uint32_t* task_off, const uint32_t* ntask, const G1Xyzz* partials,
                  const uint32_t* heavy, const uint32_t* heavy_count, G1Xyzz* sums) {
-  __shared__ G1Xyzz lds[kHeavyThreads];
   const uint32_t total = *heavy_count;
-  for (uint32_t h = blockIdx.x; h < total; h += gridDim.x) {
+  for (uint32_t h = kHeavySplitSlots + blockIdx.x; h < total; h += gridDim.x) {
     const uint32_t b = heavy[h];
     const uint32_t first = task_off[b], cnt = ntask[b];
     G1Xyzz s = G1Xyzz::identity();
     for (uint32_t k = threadIdx.x; k < cnt; k += kHeavyThreads)
       s = add(s, load_xyzz(partials, first + k));
-    lds[threadIdx.x] = s;
-    __syncthreads();
-    for (unsigned d = kHeavyThreads / 2; d > 0; d >>= 1) {
-      if (threadIdx.x < d) {
-        s = add(s, lds[threadIdx.x + d]);
-        lds[threadIdx.x] = s;
-      }
-      __syncthreads();
-    }
+    s = block_tree_sum(s);
     if (threadIdx.x == 0) store_xyzz(sums, b, s);
     __syncthreads();
   }
@@ -736,9 +786,18 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
                      c.stream, task_off, ntask, partials, G, lanes_log2, seq_max, sums, heavy,
                      heavy_count);
   rt::check_launch();
+  rt::BufferRef heavy_parts = c.allocate((size_t)kHeavySplitSlots * kHeavySplit * sizeof(G1Xyzz));
+  G1Xyzz* block_sums = static_cast<G1Xyzz*>(heavy_parts->ptr);
+  hipLaunchKernelGGL(k_heavy_partial, dim3(1024), dim3(kHeavyThreads), 0, c.stream, task_off, ntask,
+                     partials, heavy, heavy_count, block_sums);
+  rt::check_launch();
+  hipLaunchKernelGGL(k_heavy_final, dim3(256), dim3(64), 0, c.stream, ntask, heavy, heavy_count,
+                     block_sums, sums);
+  rt::check_launch();
   hipLaunchKernelGGL(k_heavy_sums, dim3(256), dim3(kHeavyThreads), 0, c.stream, task_off, ntask,
                      partials, heavy, heavy_count, sums);
   rt::check_launch();
+  keep.push_back(std::move(heavy_parts));
   // level 1 of the weighted bucket reduction: (A_k, R_k) per 256 buckets
   const size_t groups = G / kRed1;
   mark(5);

@@ -327,6 +327,8 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   const auto vk = MultilinearVerifierParam::from_trapdoor(t);
   serve::ServeConfig cfg;
   cfg.slots = slots;
+  if (const char* e = std::getenv("HP_SERVE_KZG_HOLD_US")) cfg.kzg_hold = std::chrono::microseconds(std::atol(e));
+  if (const char* e = std::getenv("HP_SERVE_WINDOW_US")) cfg.window = std::chrono::microseconds(std::atol(e));
   cfg.gen_threads = std::max<unsigned>(2, std::min<unsigned>(16, std::thread::hardware_concurrency()) - 2);
   double rate = 0;
   double capacity = 0;
@@ -366,13 +368,14 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
       "\"capacity_probe_per_s\": %.3f, \"slots\": %zu, \"srs_num_vars\": %zu, \"srs_ms\": %.1f, "
       "\"batched\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
       "\"errors\": %zu, \"max_concurrent\": %zu, \"ticks\": %zu, \"ops\": %zu, \"kzg_calls\": %zu, "
-      "\"round_calls\": %zu, \"eval_calls\": %zu, \"device_ms\": {\"gather\": %.1f, \"generic\": %.1f, "
+      "\"round_calls\": %zu, \"eval_calls\": %zu, \"kzg_held\": %zu, \"kzg_hold_us\": %ld, \"device_ms\": {\"gather\": %.1f, \"generic\": %.1f, "
       "\"kzg\": %.1f, \"rounds\": %.1f, \"evals\": %.1f}, \"request_wait_ms\": %.1f, \"request_total_ms\": %.1f}, "
       "\"serial\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
       "\"errors\": %zu}, \"identical_proofs\": %zu, \"verified\": %zu, \"verify_checked\": %zu}\n",
       trace.size(), jelly_n, nv_min, nv_max, rate, capacity, slots, N, srs_ms, b.proofs_per_s, b.p50, b.p99,
       b.mean, b.errors, brep.max_concurrent, brep.stats.ticks, brep.stats.ops, brep.stats.kzg_calls,
-      brep.stats.round_calls, brep.stats.eval_calls, brep.stats.gather_ms, brep.stats.generic_ms,
+      brep.stats.round_calls, brep.stats.eval_calls, brep.stats.kzg_held, (long)cfg.kzg_hold.count(),
+      brep.stats.gather_ms, brep.stats.generic_ms,
       brep.stats.kzg_ms, brep.stats.round_ms, brep.stats.eval_ms, brep.stats.waiting_ms, busy_ms(brep), s.proofs_per_s, s.p50, s.p99, s.mean, s.errors, identical,
       verified, checked);
   return identical == trace.size() && verified == checked ? 0 : 1;

@@ -43,8 +43,12 @@ using mi355x::Mi355xBackend;
 
 class Scheduler {
  public:
-  Scheduler(const Mi355xBackend& be, std::chrono::microseconds window)
-      : be_(be), window_(window), thread_([this] { loop(); }) {}
+  // window: how long a tick waits for requests still on the host;
+  // kzg_hold: how long commits/opens may be held back to merge with those of
+  // other requests while the others still have cheaper work (rounds, tables)
+  Scheduler(const Mi355xBackend& be, std::chrono::microseconds window,
+            std::chrono::microseconds kzg_hold = std::chrono::microseconds(0))
+      : be_(be), window_(window), kzg_hold_(kzg_hold), thread_([this] { loop(); }) {}
   ~Scheduler() {
     {
       std::lock_guard<std::mutex> g(mu_);
@@ -104,7 +108,7 @@ class Scheduler {
   }
 
   struct Stats {
-    size_t ticks = 0, ops = 0, kzg_calls = 0, round_calls = 0, eval_calls = 0;
+    size_t ticks = 0, ops = 0, kzg_calls = 0, round_calls = 0, eval_calls = 0, kzg_held = 0;
     double gather_ms = 0, generic_ms = 0, kzg_ms = 0, round_ms = 0, eval_ms = 0;  // device thread
     double waiting_ms = 0;  // request threads, summed: from submit to result
   };
@@ -128,6 +132,7 @@ class Scheduler {
     std::vector<Mi355xBackend::KzgResult> kzg_results;
     std::exception_ptr error;
     bool done = false;
+    std::chrono::steady_clock::time_point since;
   };
 
   static double ms(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
@@ -137,6 +142,7 @@ class Scheduler {
   void submit(Pending& p) {
     const auto t = std::chrono::steady_clock::now();
     std::unique_lock<std::mutex> lk(mu_);
+    p.since = t;
     queue_.push_back(&p);
     cv_submit_.notify_all();
     cv_done_.wait(lk, [&] { return p.done; });
@@ -153,8 +159,8 @@ class Scheduler {
       const auto g0 = std::chrono::steady_clock::now();
       cv_submit_.wait_until(lk, g0 + window_, [&] { return queue_.size() >= active_; });
       stats_.gather_ms += ms(g0, std::chrono::steady_clock::now());
-      std::vector<Pending*> batch(queue_.begin(), queue_.end());
-      queue_.clear();
+      std::vector<Pending*> batch;
+      hold_back_kzg(&batch);
       ++stats_.ticks;
       stats_.ops += batch.size();
       lk.unlock();
@@ -163,6 +169,31 @@ class Scheduler {
       for (Pending* p : batch) p->done = true;
       cv_done_.notify_all();
     }
+  }
+
+  // Take the queue into *batch, except commits/opens that are worth holding
+  // back: fewer than half of the active requests wait on one, others have
+  // other work in this tick, and none has waited kzg_hold yet.
+  void hold_back_kzg(std::vector<Pending*>* batch) {
+    size_t kzg = 0;
+    auto oldest = std::chrono::steady_clock::time_point::max();
+    for (Pending* p : queue_)
+      if (p->kind == Kind::Kzg) {
+        ++kzg;
+        oldest = std::min(oldest, p->since);
+      }
+    const bool hold = kzg_hold_.count() > 0 && kzg > 0 && kzg < queue_.size() &&
+                      2 * kzg < active_ &&
+                      std::chrono::steady_clock::now() - oldest < kzg_hold_;
+    std::deque<Pending*> rest;
+    for (Pending* p : queue_) {
+      if (hold && p->kind == Kind::Kzg)
+        rest.push_back(p);
+      else
+        batch->push_back(p);
+    }
+    queue_.swap(rest);
+    if (hold) ++stats_.kzg_held;
   }
 
   template <class F>
@@ -241,7 +272,7 @@ class Scheduler {
   }
 
   const Mi355xBackend& be_;
-  std::chrono::microseconds window_;
+  std::chrono::microseconds window_, kzg_hold_;
   mutable std::mutex mu_;
   std::condition_variable cv_submit_, cv_done_;
   std::deque<Pending*> queue_;
@@ -438,6 +469,7 @@ struct ServeConfig {
   size_t gen_threads = 8;
   size_t ahead = 64;
   std::chrono::microseconds window{300};
+  std::chrono::microseconds kzg_hold{3000};
   bool batched = true;  // false: serial FIFO, one request at a time, plain backend
 };
 
@@ -480,7 +512,7 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
       o.done_s = since(Clock::now());
     }
   } else {
-    Scheduler sched(be, cfg.window);
+    Scheduler sched(be, cfg.window, cfg.kzg_hold);
     BatchingBackend bb(be, sched);
     std::mutex mu;
     std::condition_variable cv;
