@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--cpu-sample-log", type=int, default=18)
     ap.add_argument("--extra-sizes", default="16,18")
     ap.add_argument("--skip-workloads", action="store_true", help="MSM headline only")
+    ap.add_argument("--no-tune", action="store_true", help="use the built-in window rule, no autotuning")
     ap.add_argument("--serve-full", action="store_true",
                     help="serve the full 2000-request trace (nv 10..20) instead of the short one")
     return ap.parse_args()
@@ -147,6 +148,13 @@ def main():
     n = 1 << log_n
     be = Mi355xBackend(0)
     rng = SplitMix64Stream(0xB1512381)
+    tuning = None
+    if not args.no_tune:
+        # window bits and task cap for the headline size, timed on this device
+        s = time.time()
+        tuning = be.msm_autotune([log_n])[0]
+        tuning["seconds"] = round(time.time() - s, 2)
+        print(f"[bench] autotune 2^{log_n}: {tuning}", file=sys.stderr, flush=True)
     t = fr_from_bytes(rng.fr_bytes(log_n).tobytes())
     t0 = time.time()
     pp = be.generate_prover_param(log_n, t)
@@ -257,6 +265,7 @@ def main():
         "extra_sizes": extra,
         "workloads": extra_workloads,
         "srs_prepare_s": round(srs_seconds, 3),
+        "msm_tuning": tuning,
         "result_prefix": result[:8].hex(),
     }
     print(json.dumps(line))

@@ -54,6 +54,13 @@ SIGNATURES = {
         _vp, _vp, _vp, _sz, ctypes.POINTER(ctypes.c_float), _sz, ctypes.POINTER(ctypes.c_uint32)
     ],
     "hp_mi355x_v1_multilinear_kzg_batch": [_vp, _vp, _vp, _sz],
+    "hp_mi355x_v1_msm_autotune": [
+        _vp, ctypes.POINTER(_sz), _sz, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+        ctypes.POINTER(ctypes.c_float),
+    ],
+    "hp_mi355x_v1_msm_set_tuning": [
+        _vp, ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), _sz
+    ],
     "hp_mi355x_v1_mle_fold_profile": [_vp, _vp, _sz, _u8p, _sz, ctypes.POINTER(ctypes.c_float)],
     "hp_mi355x_v1_mle_evaluate_batch": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_vp), _u8p, _sz],
     "hp_mi355x_v1_sumcheck_prover_state_round_batch": [

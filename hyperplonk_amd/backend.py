@@ -192,6 +192,24 @@ class Mi355xBackend:
         check(self.lib.hp_mi355x_v1_mle_evaluate_batch(self.context, addrs, nvs, pts, out, n), "evaluate batch")
         return fr_from_bytes(out.raw[: FR_BYTES * n])
 
+    def msm_autotune(self, log_sizes: Sequence[int]) -> List[dict]:
+        """Autotune window bits and task cap per layer size (extension); applies to
+        parameters prepared afterwards."""
+        n = len(log_sizes)
+        sizes = (ctypes.c_size_t * max(n, 1))(*log_sizes)
+        w = (ctypes.c_uint32 * max(n, 1))()
+        cap = (ctypes.c_uint32 * max(n, 1))()
+        ms = (ctypes.c_float * max(n, 1))()
+        check(self.lib.hp_mi355x_v1_msm_autotune(self.context, sizes, n, w, cap, ms), "msm autotune")
+        return [{"log_n": log_sizes[i], "window": w[i], "cap": cap[i], "ms": ms[i]} for i in range(n)]
+
+    def msm_set_tuning(self, table: Sequence[dict]) -> None:
+        n = len(table)
+        sizes = (ctypes.c_size_t * max(n, 1))(*[t["log_n"] for t in table])
+        w = (ctypes.c_uint32 * max(n, 1))(*[t["window"] for t in table])
+        cap = (ctypes.c_uint32 * max(n, 1))(*[t["cap"] for t in table])
+        check(self.lib.hp_mi355x_v1_msm_set_tuning(self.context, sizes, w, cap, n), "msm set tuning")
+
     def fold_profile(self, mle: Mle, r: int, reps: int = 20) -> float:
         """Mean ms of one fold of `mle` by its lowest variable (extension)."""
         self._check_mle(mle)

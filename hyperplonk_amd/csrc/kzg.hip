@@ -413,3 +413,40 @@ extern "C" HpMi355xStatus hp_mi355x_v1_msm_profile(HpMi355xContext* ctx,
     return rt::ok();
   });
 }
+
+// Autotune window bits and lane task cap per layer size on this device and
+// install them in the context (layers prepared afterwards use them).
+extern "C" HpMi355xStatus hp_mi355x_v1_msm_autotune(HpMi355xContext* ctx, const size_t* log_sizes,
+                                                    size_t count, uint32_t* out_window,
+                                                    uint32_t* out_cap, float* out_ms) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (count == 0) return rt::ok();
+    if (log_sizes == nullptr || out_window == nullptr || out_cap == nullptr || out_ms == nullptr)
+      return rt::invalid_parameters();
+    for (size_t i = 0; i < count; ++i)
+      if (log_sizes[i] < 1 || log_sizes[i] > 24) return rt::invalid_parameters();
+    for (size_t i = 0; i < count; ++i)
+      msm::autotune(c, log_sizes[i], &out_window[i], &out_cap[i], &out_ms[i]);
+    return rt::ok();
+  });
+}
+
+// Install a saved tuning table (window 0 / cap 0: the built-in rule).
+extern "C" HpMi355xStatus hp_mi355x_v1_msm_set_tuning(HpMi355xContext* ctx, const size_t* log_sizes,
+                                                      const uint32_t* window, const uint32_t* cap,
+                                                      size_t count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (count == 0) return rt::ok();
+    if (log_sizes == nullptr || window == nullptr || cap == nullptr) return rt::invalid_parameters();
+    for (size_t i = 0; i < count; ++i) {
+      if (log_sizes[i] > 40) return rt::invalid_parameters();
+      if (window[i] != 0 && (window[i] < 4 || window[i] > 20)) return rt::invalid_parameters();
+      if (cap[i] > 255) return rt::invalid_parameters();
+    }
+    for (size_t i = 0; i < count; ++i) {
+      c.tuned_window[log_sizes[i]] = static_cast<uint8_t>(window[i]);
+      c.tuned_cap[log_sizes[i]] = static_cast<uint8_t>(cap[i]);
+    }
+    return rt::ok();
+  });
+}

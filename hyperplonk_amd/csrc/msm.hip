@@ -26,6 +26,7 @@
 // bridge uses unsigned windows with c = ell - ell/3 + 2
 // (backend/src/ascend910b3/native/msm_plan.cpp:119-123).
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -80,6 +81,10 @@ struct Job {
   const G1Affine* bases;  // W window-shifted copies of the layer, window-major
   uint32_t n, c, windows, buckets;
   uint32_t bucket_base, chunks, cc_base, pad;
+  // bucket sums: 2^lanes_log2 lanes per bucket, threads [sum_base, sum_base +
+  // buckets << lanes_log2); buckets with more than seq_max partials per lane
+  // take the heavy path
+  uint32_t lanes_log2, seq_max, sum_base, padded_buckets;  // padded: multiple of 256
 };
 struct SortItem {
   uint32_t job, b0, chunk, pad;
@@ -283,25 +288,37 @@ __device__ __forceinline__ G1Xyzz shfl_xor_xyzz(const G1Xyzz& v, unsigned mask) 
   return r;
 }
 
-// Bucket sum of its task partials by a group of 2^lanes_log2 lanes: each lane
-// adds every group-th partial, then a butterfly over the group.  Buckets with
-// more than seq_max partials per lane (outliers, several times the mean) are
-// appended to `heavy` instead and summed by k_heavy_sums.
+// Bucket sum of its task partials by a group of 2^lanes_log2 lanes (per
+// job: larger MSMs have more partials per bucket): each lane adds every
+// group-th partial, then a butterfly over the group.  Buckets with more than
+// seq_max partials per lane (outliers, several times the job's mean) are
+// appended to `heavy` instead and summed by the heavy kernels.
 constexpr uint32_t kSeqMax = 8;
 __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
                                                      const uint32_t* ntask,
-                                                     const G1Xyzz* partials, uint32_t buckets,
-                                                     uint32_t lanes_log2, uint32_t seq_max,
+                                                     const G1Xyzz* partials, const Job* jobs,
+                                                     uint32_t njobs, uint32_t total_threads,
                                                      G1Xyzz* sums, uint32_t* heavy,
                                                      uint32_t* heavy_count) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  // job owning this thread (last with sum_base <= gid); wave-uniform since
+  // every job's thread range is a multiple of 64 lanes
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (jobs[mid].sum_base <= gid) lo = mid; else hi = mid;
+  }
+  const Job& job = jobs[lo];
+  const uint32_t lanes_log2 = job.lanes_log2;
   const uint32_t group = 1u << lanes_log2;
-  const uint32_t b = gid >> lanes_log2, lane = gid & (group - 1);
-  bool live = b < buckets;  // whole groups share liveness
+  const uint32_t local = gid - job.sum_base;
+  const uint32_t lb = local >> lanes_log2, lane = local & (group - 1);
+  const uint32_t b = job.bucket_base + lb;
+  bool live = gid < total_threads && lb < job.padded_buckets;  // whole groups share liveness
   G1Xyzz s = G1Xyzz::identity();
   if (live) {
     const uint32_t first = task_off[b], cnt = ntask[b];
-    if (cnt > seq_max * group) {
+    if (cnt > job.seq_max * group) {
       if (lane == 0) heavy[atomicAdd(heavy_count, 1u)] = b;
       live = false;
     } else {
@@ -577,20 +594,28 @@ void zero_u32(rt::Context& c, uint32_t* p, size_t n) {
 
 }  // namespace
 
+// Window bits.  All windows share one bucket set, so a window whose digits
+// span few values (the top window when c does not divide the scalar width
+// well) concentrates n entries in a handful of buckets.  c = 16 (16 full
+// windows for 255-bit scalars) and c = 13 (top window of 8 bits) avoid that,
+// and measure fastest from 2^14 and from 2^11 points on (autotuner sweep on
+// MI355X); smaller MSMs keep ~64 digits per bucket.
 Plan choose_plan(size_t n, uint32_t override_c) {
   Plan p;
   uint32_t c = override_c;
   if (c == 0) {
-    // B = 2^(c-1) buckets for n W digits: ~512 digits per bucket for large
-    // MSMs (fewer windows, one cheap bucket reduction), ~64 for small ones
-    // (more buckets keep enough lanes busy).
-    const double per_bucket = n >= (size_t(1) << 18) ? 512.0 : 64.0;
-    c = 16;
-    for (int it = 0; it < 3; ++it) {
-      const uint32_t w = (256 + c - 1) / c;
-      const double lg = std::log2(std::max<double>(1.0, (double)n * w / per_bucket));
-      const int cc = (int)std::lround(lg) + 1;
-      c = (uint32_t)std::min(20, std::max(4, cc));
+    if (n >= (size_t(1) << 14)) {
+      c = 16;
+    } else if (n >= (size_t(1) << 11)) {
+      c = 13;
+    } else {
+      c = 16;
+      for (int it = 0; it < 3; ++it) {
+        const uint32_t w = (256 + c - 1) / c;
+        const double lg = std::log2(std::max<double>(1.0, (double)n * w / 64.0));
+        const int cc = (int)std::lround(lg) + 1;
+        c = (uint32_t)std::min(20, std::max(4, cc));
+      }
     }
   }
   p.c = c;
@@ -599,10 +624,32 @@ Plan choose_plan(size_t n, uint32_t override_c) {
   return p;
 }
 
+// Entries per lane task by the largest MSM of a batch (autotuner sweep).
+uint32_t default_cap(size_t largest_n) {
+  if (largest_n <= (size_t(1) << 12)) return 2;
+  if (largest_n <= (size_t(1) << 16)) return 4;
+  if (largest_n <= (size_t(1) << 18)) return 8;
+  return 16;
+}
+
+size_t log2_floor(size_t n) {
+  size_t l = 0;
+  while ((size_t(2) << l) <= n) ++l;
+  return l;
+}
+
+Plan plan_for(const rt::Context& c, size_t n, uint32_t override_c) {
+  if (override_c == 0 && n > 0) {
+    const uint32_t t = c.tuned_window[log2_floor(n)];
+    if (t) return choose_plan(n, t);
+  }
+  return choose_plan(n, override_c);
+}
+
 void prepare_layer_from_device(rt::Context& c, Layer& layer, const G1Affine* points, size_t n,
                                uint32_t override_c) {
   layer.n = n;
-  layer.plan = choose_plan(n, override_c);
+  layer.plan = plan_for(c, n, override_c);
   const uint32_t W = layer.plan.windows;
   layer.bases = c.allocate(W * n * sizeof(G1Affine));
   G1Affine* bases = static_cast<G1Affine*>(layer.bases->ptr);
@@ -678,6 +725,7 @@ void Batch::add(const Layer& layer, const Fr* scalars) {
   gbuckets_ += (B + kRed1 - 1) / kRed1 * kRed1;
   cc_total_ += (size_t)B * e.chunks;
   m_total_ += layer.n * layer.plan.windows;
+  largest_n_ = std::max(largest_n_, layer.n);
   // positions and counter indices are 32-bit on the device
   if (cc_total_ >= (size_t(1) << 32) || m_total_ >= (size_t(1) << 31)) rt::fail_params();
   entries_.push_back(e);
@@ -691,14 +739,35 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   };
   const uint32_t njobs = (uint32_t)entries_.size();
   if (njobs == 0) return;
+  // lane tasks: entries per lane by the largest MSM (2 .. 16), or the
+  // context's tuned cap, or an explicit one
+  uint32_t cap = default_cap(largest_n_);
+  if (cap_override_)
+    cap = cap_override_;
+  else if (largest_n_ && c.tuned_cap[log2_floor(largest_n_)])
+    cap = c.tuned_cap[log2_floor(largest_n_)];
   // job table and sort work items, uploaded together
   std::vector<Job> jobs(njobs);
   std::vector<SortItem> items;
+  size_t sum_threads_total = 0;
   for (uint32_t j = 0; j < njobs; ++j) {
     const Entry& e = entries_[j];
     const Plan& p = e.layer->plan;
     jobs[j] = {e.scalars, static_cast<const G1Affine*>(e.layer->bases->ptr), (uint32_t)e.layer->n,
-               p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base, 0u};
+               p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base, 0u,
+               0u, 0u, 0u, 0u};
+    // bucket-sum shape of this MSM as if alone: lanes per bucket while each
+    // lane keeps >= 2 partials and the job's threads stay <= 131072; heavy
+    // threshold three times its mean partials per lane
+    const double per_bucket = ((double)e.layer->n * p.windows / cap + 1.0) / p.buckets;
+    uint32_t l = 0;
+    while (l < 6 && ((size_t)p.buckets << (l + 1)) <= 131072 && (double)(2u << l) < per_bucket) ++l;
+    jobs[j].lanes_log2 = l;
+    jobs[j].seq_max =
+        std::max<uint32_t>(kSeqMax, (uint32_t)std::ceil(3.0 * per_bucket / (double)(1u << l)));
+    jobs[j].padded_buckets = (p.buckets + kRed1 - 1) / kRed1 * kRed1;
+    jobs[j].sum_base = sum_threads_total;
+    sum_threads_total += (((size_t)jobs[j].padded_buckets << l) + 63) / 64 * 64;
     for (uint32_t b0 = 0; b0 < p.buckets; b0 += kRangeBuckets)
       for (uint32_t ch = 0; ch < e.chunks; ++ch) items.push_back({j, b0, ch, 0u});
   }
@@ -743,9 +812,6 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
                      cc_offsets, totals, starts, counts, owner);
   rt::check_launch();
   mark(2);
-  // lane tasks: ~64 entries per lane at large sizes, down to 4 for small
-  // batches so that enough lanes run
-  const uint32_t cap = (uint32_t)std::max<size_t>(4, std::min<size_t>(64, m_total_ / 262144));
   hipLaunchKernelGGL(k_task_counts, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, G, cap,
                      ntask);
   rt::check_launch();
@@ -767,24 +833,14 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   mark(4);
   // lanes per bucket: enough groups for ~2 wavefronts per SIMD, and no more
   // lanes than about half the expected partials per bucket
-  const double per_bucket = (double)(m_total_ / cap + 1) / std::max<uint32_t>(G, 1);
-  uint32_t lanes_log2 = 0;
-  while (lanes_log2 < 6 && ((size_t)G << (lanes_log2 + 1)) <= 131072 &&
-         (double)(2u << lanes_log2) < per_bucket)
-    ++lanes_log2;
-  const size_t sum_threads = (size_t)G << lanes_log2;
-  // sequential partials per lane before a bucket counts as heavy: at least
-  // kSeqMax and three times the mean, so that only outliers take the
-  // workgroup-per-bucket path
-  const uint32_t seq_max = std::max<uint32_t>(
-      kSeqMax, (uint32_t)std::ceil(3.0 * per_bucket / (double)(1u << lanes_log2)));
+  if (sum_threads_total >= (size_t(1) << 32)) rt::fail_params();
   rt::BufferRef heavy_buf = c.allocate(((size_t)G + 1) * sizeof(uint32_t));
   uint32_t* heavy_count = static_cast<uint32_t*>(heavy_buf->ptr);
   uint32_t* heavy = heavy_count + 1;
   zero_u32(c, heavy_count, 1);
-  hipLaunchKernelGGL(k_bucket_sums, dim3((unsigned)((sum_threads + 255) / 256)), dim3(256), 0,
-                     c.stream, task_off, ntask, partials, G, lanes_log2, seq_max, sums, heavy,
-                     heavy_count);
+  hipLaunchKernelGGL(k_bucket_sums, dim3((unsigned)((sum_threads_total + 255) / 256)), dim3(256), 0,
+                     c.stream, task_off, ntask, partials, djobs, njobs, (uint32_t)sum_threads_total,
+                     sums, heavy, heavy_count);
   rt::check_launch();
   rt::BufferRef heavy_parts = c.allocate((size_t)kHeavySplitSlots * kHeavySplit * sizeof(G1Xyzz));
   G1Xyzz* block_sums = static_cast<G1Xyzz*>(heavy_parts->ptr);
@@ -893,6 +949,116 @@ void generate_points(rt::Context& c, const Fr* scalars, size_t n, G1Affine* out,
                      table, out, static_cast<G1Xyzz*>(scratch->ptr),
                      static_cast<Fq*>(prefix->ptr));
   rt::check_launch();
+}
+
+namespace {
+
+// Median device time of `reps` single-MSM batches (after one warm-up).
+float time_msm(rt::Context& c, const Layer& layer, const Fr* scalars, uint32_t cap, int reps) {
+  Batch b;
+  b.add(layer, scalars);
+  b.set_cap(cap);
+  rt::BufferRef res = c.allocate(std::max<size_t>(b.result_slots(), 1) * sizeof(G1Xyzz));
+  hipEvent_t e0, e1;
+  rt::hip_check(hipEventCreate(&e0));
+  rt::hip_check(hipEventCreate(&e1));
+  b.enqueue(c, static_cast<G1Xyzz*>(res->ptr));
+  std::vector<float> t;
+  for (int r = 0; r < reps; ++r) {
+    rt::hip_check(hipEventRecord(e0, c.stream));
+    b.enqueue(c, static_cast<G1Xyzz*>(res->ptr));
+    rt::hip_check(hipEventRecord(e1, c.stream));
+    rt::hip_check(hipEventSynchronize(e1));
+    float ms = 0;
+    rt::hip_check(hipEventElapsedTime(&ms, e0, e1));
+    t.push_back(ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
+// canonical uniform Fr values from a SplitMix64 stream (rejection of 255-bit draws)
+std::vector<uint8_t> random_fr(size_t n, uint64_t seed) {
+  static const uint64_t r[4] = {0xffffffff00000001ULL, 0x53bda402fffe5bfeULL, 0x3339d80809a1d805ULL,
+                                0x73eda753299d7d48ULL};
+  std::vector<uint8_t> out(n * rt::kFrBytes);
+  uint64_t s = seed;
+  auto next = [&] {
+    s += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  };
+  for (size_t i = 0; i < n;) {
+    uint64_t w[4];
+    for (int k = 0; k < 4; ++k) w[k] = next();
+    w[3] &= 0x7FFFFFFFFFFFFFFFULL;
+    bool less = false;
+    for (int k = 3; k >= 0; --k)
+      if (w[k] != r[k]) {
+        less = w[k] < r[k];
+        break;
+      }
+    if (!less) continue;
+    std::memcpy(out.data() + i * rt::kFrBytes, w, rt::kFrBytes);
+    ++i;
+  }
+  return out;
+}
+
+}  // namespace
+
+void autotune(rt::Context& c, size_t log_n, uint32_t* best_c, uint32_t* best_cap, float* best_ms) {
+  const size_t n = size_t(1) << log_n;
+  rt::BufferRef table = make_fixed_table(c);
+  const auto ks = random_fr(n, 0x7475e3ULL + log_n);
+  rt::BufferRef kdev = rt::upload_fr_table(c, ks.data(), n, log_n);
+  rt::BufferRef pts = c.allocate(n * sizeof(G1Affine));
+  generate_points(c, static_cast<const Fr*>(kdev->ptr), n, static_cast<G1Affine*>(pts->ptr),
+                  static_cast<const G1Affine*>(table->ptr));
+  const auto ss = random_fr(n, 0x5ca1a75ULL + log_n);
+  rt::BufferRef sdev = rt::upload_fr_table(c, ss.data(), n, log_n);
+  const Fr* scalars = static_cast<const Fr*>(sdev->ptr);
+  const uint32_t rule = choose_plan(n, 0).c;
+  std::vector<uint32_t> cands;
+  for (int d = -2; d <= 2; ++d) {
+    const int cc = (int)rule + d;
+    if (cc >= 4 && cc <= 20) cands.push_back((uint32_t)cc);
+  }
+  for (uint32_t extra : {8u, 16u})
+    if (extra + 3 >= rule && extra <= rule + 3 &&
+        std::find(cands.begin(), cands.end(), extra) == cands.end())
+      cands.push_back(extra);
+  float best = 1e30f;
+  uint32_t bc = rule;
+  Layer best_layer;
+  for (uint32_t cc : cands) {
+    Layer layer;
+    prepare_layer_from_device(c, layer, static_cast<const G1Affine*>(pts->ptr), n, cc);
+    const float ms = time_msm(c, layer, scalars, default_cap(n), 3);
+    if (ms < best) {
+      best = ms;
+      bc = cc;
+      best_layer = std::move(layer);
+    }
+  }
+  uint32_t bcap = 0;
+  float best2 = 1e30f;
+  for (uint32_t cap : {2u, 4u, 8u, 16u, 32u, 64u}) {
+    const float ms = time_msm(c, best_layer, scalars, cap, 3);
+    if (ms < best2) {
+      best2 = ms;
+      bcap = cap;
+    }
+  }
+  c.tuned_window[log_n] = static_cast<uint8_t>(bc);
+  c.tuned_cap[log_n] = static_cast<uint8_t>(bcap);
+  *best_c = bc;
+  *best_cap = bcap;
+  *best_ms = std::min(best, best2);
 }
 
 }  // namespace msm

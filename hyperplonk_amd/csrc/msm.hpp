@@ -20,6 +20,12 @@ struct Plan {
 };
 
 Plan choose_plan(size_t n, uint32_t override_c = 0);
+// The plan for a layer of n points on this context: override, tuned, or rule.
+Plan plan_for(const rt::Context& c, size_t n, uint32_t override_c);
+// Time candidate window sizes and lane task caps for a layer of 2^log_n
+// random points (HIP events, median of 3 MSMs each); installs and returns the
+// fastest in the context's tuning table.
+void autotune(rt::Context& c, size_t log_n, uint32_t* best_c, uint32_t* best_cap, float* best_ms);
 
 struct Layer {
   size_t n = 0;               // points per window copy
@@ -50,6 +56,7 @@ class Batch {
  public:
   void add(const Layer& layer, const Fr* scalars);
   size_t size() const { return entries_.size(); }
+  void set_cap(uint32_t cap) { cap_override_ = cap; }
   size_t result_slots() const;
   void enqueue(rt::Context& c, G1Xyzz* result, StageEvents* events = nullptr) const;
   G1Xyzz finish(size_t job, const G1Xyzz* host_slots) const;
@@ -63,6 +70,8 @@ class Batch {
   };
   std::vector<Entry> entries_;
   uint32_t gbuckets_ = 0;
+  uint32_t cap_override_ = 0;
+  size_t largest_n_ = 0;
   size_t cc_total_ = 0, m_total_ = 0;
 };
 

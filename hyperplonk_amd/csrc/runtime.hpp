@@ -102,6 +102,10 @@ struct Context {
   std::unordered_map<const void*, std::shared_ptr<void>> resources;
   bool poisoned = false;
   int32_t first_error = 0;
+  // MSM tuning by log2 of the layer size (0 = the built-in rule): window bits
+  // for layers prepared afterwards, lane task cap for batches
+  uint8_t tuned_window[64] = {};
+  uint8_t tuned_cap[64] = {};
   // Pinned host staging for uploads and small results.
   uint8_t* pinned = nullptr;
   size_t pinned_bytes = 0;

@@ -195,6 +195,19 @@ HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(HpMi355xContext *ctx,
 HpMi355xStatus hp_mi355x_v1_mle_fold_profile(HpMi355xContext *ctx,
                                              HpMi355xDeviceAddress polynomial, size_t num_vars,
                                              const uint8_t *r, size_t reps, float *ms_per_fold);
+/* Autotune the MSM for layers of 2^log_sizes[i] points (1..24) on this device:
+ * candidate window bits around the built-in rule and lane task caps, timed
+ * with HIP events over random points and scalars; the fastest go into the
+ * context's tuning table (used by layers prepared afterwards) and into
+ * out_window / out_cap / out_ms (count entries each). */
+HpMi355xStatus hp_mi355x_v1_msm_autotune(HpMi355xContext *ctx, const size_t *log_sizes,
+                                         size_t count, uint32_t *out_window, uint32_t *out_cap,
+                                         float *out_ms);
+/* Install a saved tuning table: window bits (4..20, or 0 for the built-in rule)
+ * and lane task cap (0 for the rule) per log2 layer size. */
+HpMi355xStatus hp_mi355x_v1_msm_set_tuning(HpMi355xContext *ctx, const size_t *log_sizes,
+                                           const uint32_t *window, const uint32_t *cap,
+                                           size_t count);
 /* One request of multilinear_kzg_batch: a commit (out_points: one packet)
  * or an open at `point` (out_points: num_vars packets, out_evaluation: 32
  * bytes), with the arguments and checks of multilinear_kzg_commit / _open. */

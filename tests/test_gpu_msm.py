@@ -211,3 +211,28 @@ def test_kzg_batch_matches_single_calls(backend, rng_factory, srs20):
     with pytest.raises(InvalidParameters):
         backend.kzg_batch(pp, [("commit", mles[0]), ("open", mles[2], [1, 2])])
     assert backend.kzg_batch(pp, []) == []
+
+
+def test_autotune_changes_plans_not_results(backend, oracle, rng_factory):
+    """The autotuner only picks window bits and task caps: commits through
+    parameters prepared after tuning still equal the oracle's MSM."""
+    table = backend.msm_autotune([6, 9])
+    assert [t["log_n"] for t in table] == [6, 9]
+    for t in table:
+        assert 4 <= t["window"] <= 20 and t["cap"] in (2, 4, 8, 16, 32, 64) and t["ms"] > 0
+    try:
+        rng = rng_factory(930)
+        N = 10
+        t = fr_from_bytes(rng.fr_bytes(N).tobytes())
+        pp = backend.generate_prover_param(N, t)
+        for nv in (6, 9):
+            tab = rng.fr_bytes(1 << nv)
+            m = backend.mle_from_evaluations(nv, tab)
+            layer = backend.download_layer(pp, N - nv)
+            want = oracle.msm_pippenger_packets(layer, tab.tobytes(), 1 << nv, 8)
+            assert backend.commit_packet(pp, m) == want
+        # an explicit table: windows outside 4..20 are rejected
+        with pytest.raises(InvalidParameters):
+            backend.msm_set_tuning([{"log_n": 6, "window": 3, "cap": 4}])
+    finally:
+        backend.msm_set_tuning([{"log_n": 6, "window": 0, "cap": 0}, {"log_n": 9, "window": 0, "cap": 0}])
