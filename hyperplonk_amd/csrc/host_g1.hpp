@@ -70,30 +70,26 @@ inline Fq64 sub(const Fq64& a, const Fq64& b) {
   return r;
 }
 
-// CIOS Montgomery product (q < 2^382 leaves room for the lazy top word).
+// CIOS Montgomery product in the "no final carry" form: q's top limb is below
+// 2^63 - 1, so every intermediate fits in six words (the carries A and C of
+// the product and reduction rows never overflow a word).
 inline Fq64 mul(const Fq64& a, const Fq64& b) {
-  uint64_t t[8] = {0};
+  uint64_t t[6] = {0};
   for (int i = 0; i < 6; ++i) {
-    unsigned __int128 c = 0;
-    for (int j = 0; j < 6; ++j) {
-      c += (unsigned __int128)a.v[j] * b.v[i] + t[j];
-      t[j] = (uint64_t)c;
-      c >>= 64;
-    }
-    c += t[6];
-    t[6] = (uint64_t)c;
-    t[7] = (uint64_t)(c >> 64);
+    unsigned __int128 x = (unsigned __int128)a.v[0] * b.v[i] + t[0];
+    uint64_t A = (uint64_t)(x >> 64);
+    t[0] = (uint64_t)x;
     const uint64_t m = t[0] * kQInv;
-    c = (unsigned __int128)m * kQ[0] + t[0];
-    c >>= 64;
+    unsigned __int128 y = (unsigned __int128)m * kQ[0] + t[0];
+    uint64_t C = (uint64_t)(y >> 64);
     for (int j = 1; j < 6; ++j) {
-      c += (unsigned __int128)m * kQ[j] + t[j];
-      t[j - 1] = (uint64_t)c;
-      c >>= 64;
+      x = (unsigned __int128)a.v[j] * b.v[i] + t[j] + A;
+      A = (uint64_t)(x >> 64);
+      y = (unsigned __int128)m * kQ[j] + (uint64_t)x + C;
+      C = (uint64_t)(y >> 64);
+      t[j - 1] = (uint64_t)y;
     }
-    c += t[6];
-    t[5] = (uint64_t)c;
-    t[6] = t[7] + (uint64_t)(c >> 64);
+    t[5] = C + A;
   }
   Fq64 r;
   std::memcpy(r.v, t, sizeof r.v);

@@ -10,7 +10,7 @@
 //                scatters base references (w n + i | sign) into
 //                bucket-contiguous order.
 //   2. tasks     each bucket is cut into tasks of <= task_cap entries.
-//   3. accumulate one lane per task (<= 64 entries of one bucket): XYZZ +=
+//   3. accumulate one lane per task (<= cap entries of one bucket): XYZZ +=
 //                affine (8M + 2S); 64 independent accumulators per wavefront.
 //   4. buckets   per-bucket sum of its task partials.
 //   5. reduce    sum_b (b + 1) S_b: blocks of 256 buckets run an LDS suffix
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(kHeavyThreads)
   }
 }
 
-// Level 1: blocks of 256 buckets.  U_t = sum_{t' >= t} S_t' (suffix scan),
+// Level 1: blocks of kRed1 = 256 buckets.  U_t = sum_{t' >= t} S_t' (suffix scan),
 // A = sum_t U_t = sum_t (t + 1) S_t, R = U_0.
 constexpr unsigned kRed1 = 256;
 __global__ void __launch_bounds__(kRed1) k_reduce_buckets(const G1Xyzz* sums, uint32_t buckets,
@@ -854,7 +854,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
                      partials, heavy, heavy_count, sums);
   rt::check_launch();
   keep.push_back(std::move(heavy_parts));
-  // level 1 of the weighted bucket reduction: (A_k, R_k) per 256 buckets
+  // level 1 of the weighted bucket reduction: (A_k, R_k) per kRed1 buckets
   const size_t groups = G / kRed1;
   mark(5);
   hipLaunchKernelGGL(k_reduce_buckets, dim3((unsigned)groups), dim3(kRed1), 0, c.stream, sums, G,
@@ -868,7 +868,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   }
 }
 
-// Job j: sum_k A_k + 256 * sum_k k R_k over its level-1 groups, on the host.
+// Job j: sum_k A_k + kRed1 * sum_k k R_k over its level-1 groups, on the host.
 G1Xyzz Batch::finish(size_t job, const G1Xyzz* slots) const {
   const size_t all = gbuckets_ / kRed1;
   const size_t g0 = entries_[job].bucket_base / kRed1;
@@ -882,7 +882,7 @@ G1Xyzz Batch::finish(size_t job, const G1Xyzz* slots) const {
     run = host::add(run, host::from_device(R[k]));
     acc = host::add(acc, run);
   }
-  for (int s = 0; s < 8; ++s) acc = host::dbl(acc);  // 256 buckets per group
+  for (int s = 0; s < 8; ++s) acc = host::dbl(acc);  // kRed1 = 256 buckets per group
   return host::to_device(host::add(total, acc));
 }
 
