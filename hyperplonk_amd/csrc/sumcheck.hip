@@ -19,6 +19,9 @@
 // (backend/src/ascend910b3/native/bridge_sumcheck.cpp:210-331 create,
 // :519-550 round).
 #include <algorithm>
+#include <memory>
+#include <map>
+#include <deque>
 #include <cstdlib>
 #include <cstring>
 #include <unordered_map>
@@ -427,49 +430,191 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
   return j;
 }
 
-template <int D>
-void launch_pack(rt::Context& c, sc::JobPack& pack, unsigned grid, bool items) {
-  if (items)
-    hipLaunchKernelGGL(sc::k_round_items<D>, dim3(grid), dim3(sc::kThreads), 0, c.stream, pack);
-  else
-    hipLaunchKernelGGL(sc::k_round_pairs<D>, dim3(grid), dim3(sc::kThreads), 0, c.stream, pack);
-  rt::check_launch();
-}
+struct PackLaunch {
+  uint32_t degree;
+  bool items;
+  unsigned grid;
+  sc::JobPack pack;
+};
 
-// Launch every job (degree <= kMaxJobDegree): one kernel per (degree,
-// decomposition) and per kJobsPerLaunch jobs.
-void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
+// Group jobs (degree <= kMaxJobDegree) into launches: one per (degree,
+// decomposition) and per kJobsPerLaunch jobs; sets each job's block_begin.
+std::vector<PackLaunch> plan_packs(const std::vector<sc::RoundJob>& jobs) {
   auto key = [&](size_t i) {
     return std::make_pair(std::max<uint32_t>(jobs[i].degree, 1), jobs[i].items_mode);
   };
   std::vector<size_t> order(jobs.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key(a) < key(b); });
+  std::vector<PackLaunch> out;
   size_t i = 0;
   while (i < order.size()) {
     const auto k = key(order[i]);
-    sc::JobPack pack{};
-    unsigned grid = 0;
-    while (i < order.size() && key(order[i]) == k && pack.count < (uint32_t)sc::kJobsPerLaunch) {
+    PackLaunch l{};
+    l.degree = k.first;
+    l.items = k.second != 0;
+    while (i < order.size() && key(order[i]) == k && l.pack.count < (uint32_t)sc::kJobsPerLaunch) {
       sc::RoundJob j = jobs[order[i]];
-      j.block_begin = grid;
-      grid += j.blocks;
-      pack.job[pack.count++] = j;
+      j.block_begin = l.grid;
+      l.grid += j.blocks;
+      l.pack.job[l.pack.count++] = j;
       ++i;
     }
-    const bool items = k.second != 0;
-    switch (k.first) {
-      case 1: launch_pack<1>(c, pack, grid, items); break;
-      case 2: launch_pack<2>(c, pack, grid, items); break;
-      case 3: launch_pack<3>(c, pack, grid, items); break;
-      case 4: launch_pack<4>(c, pack, grid, items); break;
-      case 5: launch_pack<5>(c, pack, grid, items); break;
-      case 6: launch_pack<6>(c, pack, grid, items); break;
-      case 7: launch_pack<7>(c, pack, grid, items); break;
-      case 8: launch_pack<8>(c, pack, grid, items); break;
-      default: rt::fail_state(-3);
+    out.push_back(l);
+  }
+  return out;
+}
+
+template <int D>
+const void* kernel_for(bool items) {
+  return items ? reinterpret_cast<const void*>(&sc::k_round_items<D>)
+               : reinterpret_cast<const void*>(&sc::k_round_pairs<D>);
+}
+
+const void* kernel_of(uint32_t d, bool items) {
+  switch (d) {
+    case 1: return kernel_for<1>(items);
+    case 2: return kernel_for<2>(items);
+    case 3: return kernel_for<3>(items);
+    case 4: return kernel_for<4>(items);
+    case 5: return kernel_for<5>(items);
+    case 6: return kernel_for<6>(items);
+    case 7: return kernel_for<7>(items);
+    case 8: return kernel_for<8>(items);
+    default: rt::fail_state(-3);
+  }
+}
+
+void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
+  for (PackLaunch& l : plan_packs(jobs)) {
+    void* args[] = {&l.pack};
+    rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.items), dim3(l.grid), dim3(sc::kThreads), args,
+                                  0, c.stream));
+  }
+}
+
+// ---- round launches as hipGraphs, replayed with new challenges ----------
+//
+// A batch round whose launch shapes (per launch: degree class, decomposition,
+// grid, job count) were seen before replays the graph built for them: its
+// kernel nodes get the new job packs (tables, challenges) as parameters and
+// the graph's copy node brings the messages back into its pinned buffer.
+// Graphs hold no memset nodes; the per-job tickets are reset by the kernels.
+struct RoundGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphNode_t> kernels;
+  rt::BufferRef out;
+  uint8_t* host = nullptr;
+  size_t bytes = 0;
+  ~RoundGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+struct RoundGraphCache {
+  std::map<std::vector<uint32_t>, std::unique_ptr<RoundGraph>> graphs;
+  std::deque<std::vector<uint32_t>> order;  // oldest first
+  static constexpr size_t kMax = 512;
+};
+
+int g_round_graph_key;
+
+// Off by default: measured on MI355X, a graph launch plus node-parameter
+// updates costs more than the one to a few kernel launches of a round
+// (n = 20 degree-5 proof 6.34 vs 6.02 ms; nv-12 proof 8.24 vs 7.80 ms).
+// HP_MI355X_ROUND_GRAPHS=1 turns it on.
+bool round_graphs_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("HP_MI355X_ROUND_GRAPHS");
+    return e != nullptr && std::atoi(e) != 0;
+  }();
+  return on;
+}
+
+RoundGraphCache& graph_cache(rt::Context& c) {
+  auto it = c.resources.find(&g_round_graph_key);
+  if (it == c.resources.end())
+    it = c.resources.emplace(&g_round_graph_key, std::make_shared<RoundGraphCache>()).first;
+  return *static_cast<RoundGraphCache*>(it->second.get());
+}
+
+hipKernelNodeParams node_params(PackLaunch& l, void** args) {
+  hipKernelNodeParams p{};
+  p.func = const_cast<void*>(kernel_of(l.degree, l.items));
+  p.gridDim = dim3(l.grid);
+  p.blockDim = dim3(sc::kThreads);
+  p.sharedMemBytes = 0;
+  p.kernelParams = args;
+  p.extra = nullptr;
+  return p;
+}
+
+// Run the jobs (outputs at offsets out_offset[i] of a total_bytes buffer)
+// through a cached or new graph and copy the messages to `host_out`.
+void run_jobs_graph(rt::Context& c, std::vector<sc::RoundJob>& jobs,
+                    const std::vector<size_t>& out_offset, size_t total_bytes, uint8_t* host_out) {
+  // launch shapes, independent of addresses
+  std::vector<uint32_t> key;
+  {
+    std::vector<PackLaunch> shape = plan_packs(jobs);
+    for (const PackLaunch& l : shape) {
+      key.push_back(l.degree);
+      key.push_back(l.items ? 1u : 0u);
+      key.push_back(l.grid);
+      key.push_back(l.pack.count);
+      for (uint32_t j = 0; j < l.pack.count; ++j) key.push_back(l.pack.job[j].degree);
+    }
+    key.push_back(static_cast<uint32_t>(total_bytes));
+  }
+  RoundGraphCache& cache = graph_cache(c);
+  auto it = cache.graphs.find(key);
+  RoundGraph* g = it == cache.graphs.end() ? nullptr : it->second.get();
+  bool fresh = false;
+  if (g == nullptr) {
+    if (cache.graphs.size() >= RoundGraphCache::kMax) {
+      cache.graphs.erase(cache.order.front());
+      cache.order.pop_front();
+    }
+    auto ng = std::make_unique<RoundGraph>();
+    ng->bytes = total_bytes;
+    ng->out = c.allocate(total_bytes);
+    rt::hip_check(hipHostMalloc(reinterpret_cast<void**>(&ng->host), total_bytes, hipHostMallocDefault));
+    g = ng.get();
+    cache.graphs.emplace(key, std::move(ng));
+    cache.order.push_back(key);
+    fresh = true;
+  }
+  Fr* out_base = static_cast<Fr*>(g->out->ptr);
+  for (size_t i = 0; i < jobs.size(); ++i) jobs[i].out = out_base + out_offset[i];
+  std::vector<PackLaunch> packs = plan_packs(jobs);
+  if (fresh) {
+    rt::hip_check(hipGraphCreate(&g->graph, 0));
+    hipGraphNode_t prev = nullptr;
+    for (PackLaunch& l : packs) {
+      void* args[] = {&l.pack};
+      hipKernelNodeParams p = node_params(l, args);
+      hipGraphNode_t node;
+      rt::hip_check(hipGraphAddKernelNode(&node, g->graph, prev ? &prev : nullptr, prev ? 1 : 0, &p));
+      g->kernels.push_back(node);
+      prev = node;
+    }
+    hipGraphNode_t copy;
+    rt::hip_check(hipGraphAddMemcpyNode1D(&copy, g->graph, &prev, 1, g->host, g->out->ptr, total_bytes,
+                                          hipMemcpyDeviceToHost));
+    rt::hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  } else {
+    for (size_t k = 0; k < packs.size(); ++k) {
+      void* args[] = {&packs[k].pack};
+      hipKernelNodeParams p = node_params(packs[k], args);
+      rt::hip_check(hipGraphExecKernelNodeSetParams(g->exec, g->kernels[k], &p));
     }
   }
+  rt::hip_check(hipGraphLaunch(g->exec, c.stream));
+  rt::hip_check(hipStreamSynchronize(c.stream));
+  std::memcpy(host_out, g->host, total_bytes);
 }
 
 // Checks of one round call before any side effect (bridge_sumcheck.cpp:519-550).
@@ -680,6 +825,14 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
     try {
       const size_t width = s.max_degree + 1;
       const Fr ch = challenge_of(s, challenge_or_null);
+      if (s.max_degree <= kMaxJobDegree && round_graphs_enabled()) {
+        std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr)};
+        std::vector<uint8_t> out(width * rt::kFrBytes);
+        run_jobs_graph(c, jobs, {0}, out.size(), out.data());
+        std::memcpy(out_evaluations, out.data(), out.size());
+        s.round += 1;
+        return rt::ok();
+      }
       if (s.max_degree <= kMaxJobDegree) {
         std::vector<sc::RoundJob> jobs{make_job(s, ch, static_cast<Fr*>(s.result->ptr))};
         launch_jobs(c, jobs);
@@ -725,6 +878,26 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
       total += ss[i]->max_degree + 1;
     }
     try {
+      bool all_jobs = true;
+      for (size_t i = 0; i < count; ++i) all_jobs = all_jobs && ss[i]->max_degree <= kMaxJobDegree;
+      if (all_jobs && round_graphs_enabled()) {
+        std::vector<sc::RoundJob> jobs;
+        std::vector<size_t> offs;
+        for (size_t i = 0; i < count; ++i) {
+          sc::State& s = *ss[i];
+          const Fr ch = challenge_of(s, challenges_or_null ? challenges_or_null[i] : nullptr);
+          jobs.push_back(make_job(s, ch, nullptr));
+          offs.push_back(offset[i]);
+        }
+        std::vector<uint8_t> host(total * rt::kFrBytes);
+        run_jobs_graph(c, jobs, offs, host.size(), host.data());
+        for (size_t i = 0; i < count; ++i) {
+          std::memcpy(out_evaluations[i], host.data() + offset[i] * rt::kFrBytes,
+                      (ss[i]->max_degree + 1) * rt::kFrBytes);
+          ss[i]->round += 1;
+        }
+        return rt::ok();
+      }
       rt::BufferRef outs = c.allocate(total * rt::kFrBytes);
       Fr* out_base = static_cast<Fr*>(outs->ptr);
       std::vector<sc::RoundJob> jobs;

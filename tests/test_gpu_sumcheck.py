@@ -276,3 +276,24 @@ def test_round_batch_checks_before_work(backend):
         backend.prove_rounds_batch([a, b], [None, 5])
     # nothing advanced: both still answer round zero
     assert backend.prove_rounds_batch([a, b], [None, None]) == [[14, 20], [21, 30]]
+
+
+@pytest.mark.parametrize("degree", ["2", "5"])
+def test_round_graph_replay_gives_same_messages(degree):
+    """Rounds replayed from hipGraphs (HP_MI355X_ROUND_GRAPHS=1) produce the
+    same messages as direct launches: whole n = 12 sumchecks, 1 + 2 runs each
+    (the later runs replay the graphs built by the first)."""
+    import json
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prog = os.path.join(root, "_build", "prove_bench")
+    digests = []
+    for graphs in ("0", "1"):
+        env = dict(os.environ, HP_MI355X_ROUND_GRAPHS=graphs)
+        res = subprocess.run([prog, "sumcheck", "12", degree, "2", "3"], capture_output=True, text=True,
+                             timeout=120, env=env)
+        assert res.returncode == 0, res.stderr
+        digests.append(json.loads(res.stdout.strip().splitlines()[-1])["messages_sha3_8"])
+    assert digests[0] == digests[1]
