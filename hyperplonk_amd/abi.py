@@ -50,6 +50,8 @@ SIGNATURES = {
     "hp_mi355x_v1_mle_download": [_vp, _vp, _sz, _u8p, _sz],
     "hp_mi355x_v1_multilinear_kzg_params_generate": [_vp, _vp, _u8p],
     "hp_mi355x_v1_multilinear_kzg_params_download_layer": [_vp, _vp, _sz, _u8p, _sz],
+    "hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed": [_vp, _vp, _sz, _u8p, _sz],
+    "hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed": [_vp, _vp, _sz, _u8p, _sz],
     "hp_mi355x_v1_msm_profile": [
         _vp, _vp, _vp, _sz, ctypes.POINTER(ctypes.c_float), _sz, ctypes.POINTER(ctypes.c_uint32)
     ],

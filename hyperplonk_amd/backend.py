@@ -326,6 +326,34 @@ class Mi355xBackend:
         )
         return buf.raw
 
+    def prepare_prover_param_compressed(self, num_vars: int, powers_of_g: Iterable[bytes]) -> PreparedProverParam:
+        """Extension: N+1 layers in the compressed encoding of canonical SRS
+        files (48 bytes a point), decompressed on the device."""
+        layers = list(powers_of_g)
+        if len(layers) != num_vars + 1:
+            raise InvalidParameters("prover parameters need num_vars + 1 layers")
+        pp = self._new_params(num_vars)
+        for i, layer in enumerate(layers):
+            data = bytes(layer)
+            check(
+                self.lib.hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed(
+                    self.context, pp.raw, i, data, len(data) // 48
+                ),
+                f"upload compressed layer {i}",
+            )
+        return pp
+
+    def download_layer_compressed(self, pp: PreparedProverParam, layer: int) -> bytes:
+        n = 1 << (pp.num_vars - layer)
+        buf = ctypes.create_string_buffer(n * 48)
+        check(
+            self.lib.hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed(
+                self.context, pp.raw, layer, buf, n
+            ),
+            "download_layer_compressed",
+        )
+        return buf.raw
+
     def commit_packet(self, pp: PreparedProverParam, polynomial: Mle) -> bytes:
         self._check_mle(polynomial)
         out = ctypes.create_string_buffer(G1_BYTES)

@@ -162,6 +162,36 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer(
   });
 }
 
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed(
+    HpMi355xContext* ctx, HpMi355xKzgParams* params, size_t layer_index, const uint8_t* compressed,
+    size_t point_count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || compressed == nullptr || layer_index > p->num_vars) return rt::invalid_parameters();
+    if (point_count != (size_t(1) << (p->num_vars - layer_index))) return rt::invalid_parameters();
+    msm::Layer layer;
+    if (!msm::upload_layer_compressed(c, layer, compressed, point_count, p->window_override))
+      return rt::invalid_parameters();
+    c.sync();
+    p->layers[layer_index] = std::move(layer);
+    p->ready[layer_index] = true;
+    return rt::ok();
+  });
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed(
+    HpMi355xContext* ctx, const HpMi355xKzgParams* params, size_t layer_index, uint8_t* compressed,
+    size_t point_count) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p || compressed == nullptr || layer_index > p->num_vars) return rt::invalid_parameters();
+    if (point_count != (size_t(1) << (p->num_vars - layer_index))) return rt::invalid_parameters();
+    if (!p->ready[layer_index]) return rt::invalid_state();
+    msm::download_layer_compressed(c, p->layers[layer_index], compressed);
+    return rt::ok();
+  });
+}
+
 extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_destroy(HpMi355xContext* ctx,
                                                                       HpMi355xKzgParams* params) {
   return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {

@@ -500,6 +500,91 @@ __global__ void k_points_from_mont(const G1Affine* pts, G1Affine* out, size_t n)
   }
 }
 
+// ---- compressed encoding (SRS files) ---------------------------------------
+// 48 bytes per point = 12 little-endian u32 words w[0..11] of the big-endian
+// x, so canonical limb j is bswap(w[11 - j]) and the flag byte is the low
+// byte of w[0] (top bits of limb 11).
+
+// canonical y: y > -y  <=>  2 y >= q (y != 0)
+__device__ __forceinline__ bool fq_largest(const Fq& canonical) {
+  uint32_t d[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i)
+    d[i] = (canonical.v[i] << 1) | (i ? canonical.v[i - 1] >> 31 : 0u);
+  return geq_modulus<FqParams>(d);
+}
+
+__global__ void k_decompress(const uint4* in, size_t n, G1Affine* out, unsigned* bad) {
+  // (q + 1) / 4, limb-wise from the modulus (q = 3 mod 4, so q + 1 does not carry)
+  uint32_t e[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    const uint32_t lo = i == 0 ? FqParams::MOD[0] + 1u : FqParams::MOD[i];
+    e[i] = (lo >> 2) | (i < 11 ? FqParams::MOD[i + 1] << 30 : 0u);
+  }
+  Fq four = Fq::zero();
+  four.v[0] = 4;
+  four = to_mont(four);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint4 a = in[3 * i], b = in[3 * i + 1], c = in[3 * i + 2];
+    const uint32_t w[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+    Fq x;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) x.v[j] = __builtin_bswap32(w[11 - j]);
+    const uint32_t flags = x.v[11] >> 29;  // bit 2 compressed, 1 infinity, 0 largest
+    x.v[11] &= 0x1fffffffu;
+    if (!(flags & 4u)) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    if (flags & 2u) {  // infinity: every other bit clear
+      if ((flags & 1u) || !x.is_zero()) atomicOr(bad, 1u);
+      store_affine(out, i, G1Affine::identity());
+      continue;
+    }
+    if (geq_modulus<FqParams>(x.v)) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    x = to_mont(x);
+    const Fq rhs = sqr(x) * x + four;
+    Fq y = Fq::one();
+    for (int l = 11; l >= 0; --l)
+      for (int b2 = 31; b2 >= 0; --b2) {
+        y = sqr(y);
+        if ((e[l] >> b2) & 1u) y = y * rhs;
+      }
+    if (sqr(y) != rhs) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    if (fq_largest(from_mont(y)) != bool(flags & 1u)) y = neg(y);
+    store_affine(out, i, {x, y});
+  }
+}
+
+__global__ void k_compress(const G1Affine* pts, size_t n, uint4* out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const G1Affine p = load_affine(pts, i);
+    uint32_t w[12];
+    if (p.is_identity()) {
+#pragma unroll
+      for (int j = 0; j < 12; ++j) w[j] = 0;
+      w[0] = 0xC0u;
+    } else {
+      const Fq x = from_mont(p.x);
+#pragma unroll
+      for (int j = 0; j < 12; ++j) w[11 - j] = __builtin_bswap32(x.v[j]);
+      w[0] |= fq_largest(from_mont(p.y)) ? 0xA0u : 0x80u;
+    }
+    out[3 * i] = make_uint4(w[0], w[1], w[2], w[3]);
+    out[3 * i + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    out[3 * i + 2] = make_uint4(w[8], w[9], w[10], w[11]);
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -918,6 +1003,43 @@ void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets) {
     else
       std::memcpy(out + 1, q, 96);
   }
+}
+
+bool upload_layer_compressed(rt::Context& c, Layer& layer, const uint8_t* compressed, size_t n,
+                             uint32_t override_c) {
+  constexpr size_t kBytes = 48;
+  rt::BufferRef packed = c.allocate(std::max<size_t>(n, 1) * kBytes);
+  const size_t chunk = (size_t(32) << 20) / kBytes;
+  for (size_t done = 0; done < n; done += chunk) {
+    const size_t cnt = std::min(chunk, n - done);
+    uint8_t* stage = c.staging(cnt * kBytes);
+    std::memcpy(stage, compressed + kBytes * done, cnt * kBytes);
+    rt::hip_check(hipMemcpyAsync(static_cast<uint8_t*>(packed->ptr) + kBytes * done, stage,
+                                 cnt * kBytes, hipMemcpyHostToDevice, c.stream));
+    c.staging_used();
+  }
+  rt::BufferRef raw = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Affine));
+  rt::BufferRef flag = c.allocate(sizeof(unsigned));
+  zero_u32(c, static_cast<uint32_t*>(flag->ptr), 1);
+  hipLaunchKernelGGL(k_decompress, dim3(grid_for(n)), dim3(256), 0, c.stream,
+                     static_cast<const uint4*>(packed->ptr), n, static_cast<G1Affine*>(raw->ptr),
+                     static_cast<unsigned*>(flag->ptr));
+  rt::check_launch();
+  unsigned bad = 0;
+  rt::download_bytes(c, flag->ptr, sizeof(unsigned), reinterpret_cast<uint8_t*>(&bad));
+  if (bad) return false;
+  prepare_layer_from_device(c, layer, static_cast<const G1Affine*>(raw->ptr), n, override_c);
+  return true;
+}
+
+void download_layer_compressed(rt::Context& c, const Layer& layer, uint8_t* compressed) {
+  const size_t n = layer.n;
+  rt::BufferRef tmp = c.allocate(std::max<size_t>(n, 1) * 48);
+  hipLaunchKernelGGL(k_compress, dim3(grid_for(n)), dim3(256), 0, c.stream,
+                     static_cast<const G1Affine*>(layer.bases->ptr), n,
+                     static_cast<uint4*>(tmp->ptr));
+  rt::check_launch();
+  rt::download_bytes(c, tmp->ptr, n * 48, compressed);
 }
 
 rt::BufferRef make_fixed_table(rt::Context& c) {

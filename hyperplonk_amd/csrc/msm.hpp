@@ -82,6 +82,15 @@ void affine_to_packet(const G1Affine& p, uint8_t* out97);
 // Copy layer points (window 0) back as packets.
 void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets);
 
+// Compressed G1 encoding of SRS files (48 bytes per point: x big-endian, flag
+// bits 0x80 compressed, 0x40 infinity, 0x20 y > -y on the first byte).
+// upload_layer_compressed decompresses on the device (square root of
+// x^3 + 4) and returns false for a missing compression flag, inconsistent
+// infinity bits, x >= q or x off the curve; no subgroup check.
+bool upload_layer_compressed(rt::Context& c, Layer& layer, const uint8_t* compressed, size_t n,
+                             uint32_t override_c = 0);
+void download_layer_compressed(rt::Context& c, const Layer& layer, uint8_t* compressed);
+
 // Byte table d * 2^(8 w) * g (w < 32, d < 256) of the standard generator.
 rt::BufferRef make_fixed_table(rt::Context& c);
 // Device fixed-base generation of g * s_i for a Montgomery scalar table.

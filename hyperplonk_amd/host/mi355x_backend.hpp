@@ -308,6 +308,37 @@ class Mi355xBackend {
     return pp;
   }
 
+  // The same from layers in the compressed encoding of SRS files (48 bytes a
+  // point, decompressed on the device); layers[i] holds 2^(num_vars - i) points.
+  PreparedProverParam prepare_prover_param_compressed(
+      size_t num_vars, const std::vector<const std::vector<uint8_t>*>& layers) const {
+    const auto lock = ctx_->guard();
+    if (layers.size() != num_vars + 1) invalid("prover parameters need num_vars + 1 layers");
+    PreparedProverParam pp = new_params(num_vars);
+    for (size_t i = 0; i < layers.size(); ++i)
+      check(hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed(
+                ctx_->raw(), pp.state_->raw, i, layers[i]->data(), layers[i]->size() / 48),
+            "upload_layer_compressed");
+    return pp;
+  }
+
+  // Every layer of prepared parameters, as 97-byte packets or compressed.
+  std::vector<std::vector<uint8_t>> download_prover_param(const PreparedProverParam& pp,
+                                                          bool compressed) const {
+    const auto lock = ctx_->guard();
+    std::vector<std::vector<uint8_t>> out;
+    for (size_t i = 0; i <= pp.num_vars(); ++i) {
+      const size_t n = size_t(1) << (pp.num_vars() - i);
+      out.emplace_back(n * (compressed ? 48 : 97));
+      check(compressed ? hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed(
+                             ctx_->raw(), pp.raw(), i, out.back().data(), n)
+                       : hp_mi355x_v1_multilinear_kzg_params_download_layer(
+                             ctx_->raw(), pp.raw(), i, out.back().data(), n),
+            "download_layer");
+    }
+    return out;
+  }
+
   PreparedProverParam generate_prover_param(size_t num_vars, const std::vector<Fr>& t) const {
     const auto lock = ctx_->guard();
     if (t.size() != num_vars) invalid("trapdoor length must equal num_vars");

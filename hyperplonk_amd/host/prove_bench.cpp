@@ -19,6 +19,11 @@
 //             prefix) served with continuous batching and again serially
 //             (FIFO, one proof at a time); latency = completion - arrival;
 //             every proof of the two runs must be identical
+//   srs       <nv> <path>
+//             device SRS generation from a seeded trapdoor, compression on the
+//             device and the canonical file write (load_or_generate_srs), then
+//             the file read back and its layers decompressed and prepared on
+//             the device; the prepared points must equal the generated ones
 // Prints one JSON object per run on stdout.
 #include <algorithm>
 #include <chrono>
@@ -30,6 +35,7 @@
 #include "circuit.hpp"
 #include "mi355x_backend.hpp"
 #include "serve.hpp"
+#include "srs_file.hpp"
 
 using namespace hyperplonk;
 using namespace hyperplonk::host;
@@ -119,25 +125,27 @@ static int run_sumcheck(size_t nv, int degree, int reps, uint64_t seed) {
   }
   std::vector<double> ms;
   std::string digest;
+  bool identical = true;
   for (int r = 0; r < reps + 1; ++r) {
     const auto t0 = Clock::now();
     IOPTranscript tr("sumcheck bench");
     tr.append_message("seed", reinterpret_cast<const uint8_t*>(&seed), 8);
     const IOPProof proof = sumcheck_prove(be, vp, tr);
     const double t = ms_since(t0);
-    if (r == 0) {  // first run is warmup
-      Writer w;
-      for (const auto& msg : proof.proofs) w.frs(msg);
-      digest = hex8(w.buf);
-      continue;
-    }
+    Writer w;
+    for (const auto& msg : proof.proofs) w.frs(msg);
+    const std::string d = hex8(w.buf);
+    identical = identical && (digest.empty() || d == digest);
+    digest = d;  // the last run's messages (runs after the first may replay cached launches)
+    if (r == 0) continue;  // first run is warmup
     ms.push_back(t);
   }
   std::printf(
       "{\"mode\": \"sumcheck\", \"nv\": %zu, \"degree\": %d, \"reps\": %d, \"median_ms\": %.4f, "
-      "\"min_ms\": %.4f, \"messages_sha3_8\": \"%s\"}\n",
-      nv, degree, reps, percentile(ms, 0.5), percentile(ms, 0.0), digest.c_str());
-  return 0;
+      "\"min_ms\": %.4f, \"messages_sha3_8\": \"%s\", \"runs_identical\": %s}\n",
+      nv, degree, reps, percentile(ms, 0.5), percentile(ms, 0.0), digest.c_str(),
+      identical ? "true" : "false");
+  return identical ? 0 : 1;
 }
 
 struct BatchCase {
@@ -330,6 +338,7 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   if (const char* e = std::getenv("HP_SERVE_KZG_HOLD_US")) cfg.kzg_hold = std::chrono::microseconds(std::atol(e));
   if (const char* e = std::getenv("HP_SERVE_WINDOW_US")) cfg.window = std::chrono::microseconds(std::atol(e));
   cfg.gen_threads = std::max<unsigned>(2, std::min<unsigned>(16, std::thread::hardware_concurrency()) - 2);
+  cfg.progress_every = count >= 500 ? 100 : 0;
   double rate = 0;
   double capacity = 0;
   if (rate_arg == "auto") {
@@ -381,6 +390,34 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   return identical == trace.size() && verified == checked ? 0 : 1;
 }
 
+static int run_srs(size_t nv, const std::string& path) {
+  mi355x::Mi355xBackend be(0);
+  SplitMix64 trng(0x5352535f54415553ULL ^ nv);
+  std::vector<Fr> t;
+  for (size_t i = 0; i < nv; ++i) t.push_back(trng.field());
+  std::remove(path.c_str());
+  // warm the device (context, kernels) outside the timed setups
+  (void)be.generate_prover_param(2, to_wire(std::vector<Fr>(t.begin(), t.begin() + 2)));
+  auto t0 = Clock::now();
+  const auto gpp = be.generate_prover_param(nv, to_wire(t));
+  const double gen_device_ms = ms_since(t0);
+  const SrsSetup made = load_or_generate_srs(be, nv, t, path);
+  const SrsSetup loaded = load_or_generate_srs(be, nv, t, path);
+  t0 = Clock::now();
+  const auto lpp = be.prepare_prover_param_compressed(nv, loaded.srs.prover_layers(nv));
+  const double prepare_ms = ms_since(t0);
+  const bool same = be.download_prover_param(lpp, true) == be.download_prover_param(gpp, true);
+  const size_t bytes = serialize_srs(loaded.srs).size();
+  std::remove(path.c_str());
+  std::printf(
+      "{\"mode\": \"srs\", \"num_vars\": %zu, \"points\": %zu, \"file_bytes\": %zu, "
+      "\"generate_device_ms\": %.1f, \"generate_and_write_ms\": %.1f, \"read_ms\": %.1f, "
+      "\"prepare_from_file_ms\": %.1f, \"load_total_ms\": %.1f, \"identical\": %s}\n",
+      nv, (size_t(2) << nv) - 1, bytes, gen_device_ms, made.setup_ms, loaded.load_ms, prepare_ms,
+      loaded.load_ms + prepare_ms, same ? "true" : "false");
+  return same ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
@@ -393,6 +430,7 @@ int main(int argc, char** argv) {
       return run_serve(std::strtoul(argv[2], nullptr, 10), argv[3], std::strtoul(argv[4], nullptr, 10),
                        std::strtoul(argv[5], nullptr, 10), std::atof(argv[6]), std::strtoul(argv[7], nullptr, 10),
                        std::strtoull(argv[8], nullptr, 0));
+    if (mode == "srs" && argc == 4) return run_srs(std::strtoul(argv[2], nullptr, 10), argv[3]);
     if (mode == "sumcheck" && argc == 6)
       return run_sumcheck(std::strtoul(argv[2], nullptr, 10), std::atoi(argv[3]), std::atoi(argv[4]),
                           std::strtoull(argv[5], nullptr, 0));
@@ -404,7 +442,8 @@ int main(int argc, char** argv) {
                "usage: %s prove <nv> <vanilla|jellyfish> <proofs> <warmup> <seed>\n"
                "       %s sumcheck <nv> <2|5> <reps> <seed>\n"
                "       %s batch <reps> <seed>\n"
-               "       %s serve <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>\n",
-               argv[0], argv[0], argv[0], argv[0]);
+               "       %s serve <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>\n"
+               "       %s srs <nv> <path>\n",
+               argv[0], argv[0], argv[0], argv[0], argv[0]);
   return 2;
 }

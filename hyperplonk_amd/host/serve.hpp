@@ -471,6 +471,7 @@ struct ServeConfig {
   std::chrono::microseconds window{300};
   std::chrono::microseconds kzg_hold{3000};
   bool batched = true;  // false: serial FIFO, one request at a time, plain backend
+  size_t progress_every = 0;  // > 0: a line on stderr every that many completions
 };
 
 struct ServeReport {
@@ -493,6 +494,15 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
   pipe.prefill(std::min(cfg.ahead, trace.size()));  // inputs of the first requests exist before the clock starts
   const auto t0 = Clock::now();
   auto since = [&](Clock::time_point t) { return std::chrono::duration<double>(t - t0).count(); };
+  std::atomic<size_t> completed{0};
+  auto note_done = [&] {
+    const size_t k = ++completed;
+    if (cfg.progress_every && (k % cfg.progress_every == 0 || k == trace.size())) {
+      std::fprintf(stderr, "serve %s: %zu/%zu done at %.1f s\n", cfg.batched ? "batched" : "serial", k,
+                   trace.size(), since(Clock::now()));
+      std::fflush(stderr);
+    }
+  };
   auto wait_arrival = [&](const Request& r) {
     const auto at = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(r.arrival_s));
     std::this_thread::sleep_until(at);
@@ -510,6 +520,7 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
         o.error = e.what();
       }
       o.done_s = since(Clock::now());
+      note_done();
     }
   } else {
     Scheduler sched(be, cfg.window, cfg.kzg_hold);
@@ -539,6 +550,7 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
         }
         job.second.reset();
         o.done_s = since(Clock::now());
+        note_done();
         sched.leave();
         {
           std::lock_guard<std::mutex> g(mu);

@@ -229,6 +229,20 @@ HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext *ctx,
                                                   const HpMi355xKzgParams *params,
                                                   const HpMi355xKzgRequest *requests,
                                                   size_t count);
+/* Canonical SRS files (MultilinearUniversalParams serialize_compressed, the
+ * format hyperplonk/examples/full_prove_bench.rs:665-760 loads and writes):
+ * fill one layer from point_count compressed G1 points (48 bytes each:
+ * big-endian x, flag bits 0x80 compressed / 0x40 infinity / 0x20 y > -y on
+ * the first byte), decompressed on the device; INVALID_PARAMETERS for a
+ * missing compression flag, x >= q or x off the curve (no subgroup check, as
+ * deserialize_compressed_unchecked in hyperplonk/benches/bench.rs:53-56). */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed(
+    HpMi355xContext *ctx, HpMi355xKzgParams *params, size_t layer_index,
+    const uint8_t *compressed, size_t point_count);
+/* Copy one prepared layer back in the compressed encoding (48 bytes a point). */
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed(
+    HpMi355xContext *ctx, const HpMi355xKzgParams *params, size_t layer_index,
+    uint8_t *compressed, size_t point_count);
 #ifdef __cplusplus
 }
 #endif

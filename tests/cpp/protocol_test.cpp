@@ -5,8 +5,11 @@
 //   prove    HyperPlonk prove + verify of a seeded mock circuit over the oracle
 //            backend, then verification of tampered proofs
 //   cpu-prove  timed proofs on the oracle backend (CPU baseline)
+//   srs      SRS file encoding: compressed G1/G2 known answers, square roots
+//            against Euler's criterion, a small universal SRS written, read
+//            back and decompressed point by point, malformed files rejected
 // Usage: protocol_test kat | prove <nv> <vanilla|jellyfish> <seed> |
-//        cpu-prove <nv> <vanilla|jellyfish> <reps> <threads>
+//        cpu-prove <nv> <vanilla|jellyfish> <reps> <threads> | srs <dir>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -14,6 +17,7 @@
 #include <string>
 
 #include "../../hyperplonk_amd/host/circuit.hpp"
+#include "../../hyperplonk_amd/host/srs_file.hpp"
 #include "oracle_backend.hpp"
 
 using namespace hyperplonk;
@@ -213,6 +217,183 @@ static int cpu_prove(size_t nv, const std::string& gate, int reps, int threads) 
   return 0;
 }
 
+template <class F>
+static bool throws(F&& f) {
+  try {
+    f();
+  } catch (const SrsFileError&) {
+    return true;
+  }
+  return false;
+}
+
+static void srs_checks(const std::string& dir) {
+  const G1Aff g1 = g1_generator();
+  const G2Aff g2 = g2_generator();
+  // compressed generators (the encodings the BLS12-381 serialization
+  // standard publishes for them)
+  const auto c1 = g1_compressed(g1_to_packet(g1));
+  const auto c2 = g2_compressed(g2);
+  CHECK(hex(c1.data(), 48) ==
+        "97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac586c55e83ff97a1aeffb3af00adb22c6bb");
+  CHECK(hex(c2.data(), 96) ==
+        "93e02b6052719f607dacd3a088274f65596bd0d09920b61ab5da61bbdc7f5049334cf11213945d57e5ac7d055d042b7e"
+        "024aa2b2f08f0a91260805272dc51051c6e47ad4fa403b02b4510b647ae3d1770bac0326a805bbefd48056c8c121bdb8");
+  std::printf("g2gen %s\n", hex(c2.data(), 96).c_str());
+  auto same1 = [](const G1Aff& a, const G1Aff& b) {
+    return a.infinity == b.infinity && (a.infinity || (a.x == b.x && a.y == b.y));
+  };
+  auto same2 = [](const G2Aff& a, const G2Aff& b) {
+    return a.infinity == b.infinity && (a.infinity || (a.x == b.x && a.y == b.y));
+  };
+  CHECK(same1(g1_decompress(c1.data()), g1));
+  CHECK(same2(g2_decompress(c2.data()), g2));
+  // round trips of random multiples and their negations; the sign flag is
+  // the only difference between P and -P
+  SplitMix64 rng(99);
+  for (int i = 0; i < 8; ++i) {
+    const Fr k = rng.field();
+    const G1Aff p = to_aff(jac_mul(to_jac(g1), k.canonical()));
+    const G1Aff n = to_aff(jac_neg(to_jac(p)));
+    const auto cp = g1_compressed(g1_to_packet(p)), cn = g1_compressed(g1_to_packet(n));
+    CHECK(same1(g1_decompress(cp.data()), p));
+    CHECK(same1(g1_decompress(cn.data()), n));
+    CHECK((cp[0] ^ cn[0]) == 0x20);
+    CHECK(std::equal(cp.begin() + 1, cp.end(), cn.begin() + 1));
+    const G2Aff q = to_aff(jac_mul(to_jac(g2), k.canonical()));
+    const G2Aff qn = to_aff(jac_neg(to_jac(q)));
+    const auto cq = g2_compressed(q), cqn = g2_compressed(qn);
+    CHECK(same2(g2_decompress(cq.data()), q));
+    CHECK(same2(g2_decompress(cqn.data()), qn));
+    CHECK((cq[0] ^ cqn[0]) == 0x20);
+  }
+  // square roots against Euler's criterion: a in Fq2 is a square iff its
+  // norm c0^2 + c1^2 is a square in Fq
+  const Limbs<6> half = srs_detail::shifted_q(-1, 1);
+  int squares = 0, non_squares = 0;
+  for (int i = 0; i < 24; ++i) {
+    uint8_t b[96];
+    for (auto& v : b) v = static_cast<uint8_t>(rng.next());
+    const Fq2 a{Fq::from_le_bytes_mod_order(b, 48), Fq::from_le_bytes_mod_order(b + 48, 48)};
+    const bool euler = (a.c0.square() + a.c1.square()).pow(half) == Fq::one();
+    Fq2 r;
+    const bool found = srs_detail::fq2_sqrt(a, &r);
+    CHECK(found == euler);
+    if (found) CHECK(r.square() == a);
+    const Fq x = a.c0;
+    Fq s;
+    const bool fq_found = srs_detail::fq_sqrt(x, &s);
+    CHECK(fq_found == (x.pow(half) == Fq::one()));
+    if (fq_found) CHECK(s.square() == x);
+    (found ? squares : non_squares)++;
+  }
+  CHECK(squares > 0 && non_squares > 0);
+  // an x off the curve: x^3 + 4 not a square
+  std::array<uint8_t, 48> off = c1;
+  for (uint8_t v = 1;; ++v) {
+    off[47] = v;
+    const Fq x = srs_detail::get_be(off.data());
+    Fq y;
+    if (!srs_detail::fq_sqrt(x.square() * x + Fq::from_u64(4), &y)) break;
+  }
+  CHECK(throws([&] { g1_decompress(off.data()); }));
+  std::array<uint8_t, 48> unflagged = c1;
+  unflagged[0] &= 0x7f;
+  CHECK(throws([&] { g1_decompress(unflagged.data()); }));
+  std::array<uint8_t, 48> inf{};
+  inf[0] = 0xC0;
+  CHECK(g1_decompress(inf.data()).infinity);
+  inf[5] = 1;
+  CHECK(throws([&] { g1_decompress(inf.data()); }));
+
+  // a small universal SRS: layer i = { g * eq(t[i..], x) }, last layer [g]
+  const size_t nv = 3;
+  std::vector<Fr> t;
+  for (size_t i = 0; i < nv; ++i) t.push_back(rng.field());
+  SrsFile s;
+  s.num_vars = nv;
+  std::vector<std::vector<G1Aff>> points;
+  for (size_t i = 0; i <= nv; ++i) {
+    const std::vector<Fr> ti(t.begin() + i, t.end());
+    std::vector<G1Aff> layer;
+    std::vector<uint8_t> bytes;
+    for (size_t x = 0; x < (size_t(1) << (nv - i)); ++x) {
+      std::vector<Fr> bits;
+      for (size_t k = 0; k < nv - i; ++k) bits.push_back((x >> k) & 1 ? Fr::one() : Fr::zero());
+      const Fr e = ti.empty() ? Fr::one() : eq_eval(bits, ti);
+      layer.push_back(to_aff(jac_mul(to_jac(g1), e.canonical())));
+      const auto c = g1_compressed(g1_to_packet(layer.back()));
+      bytes.insert(bytes.end(), c.begin(), c.end());
+    }
+    points.push_back(layer);
+    s.layers.push_back(bytes);
+  }
+  const MultilinearVerifierParam vp = MultilinearVerifierParam::from_trapdoor(t);
+  s.g = vp.g;
+  s.h = vp.h;
+  s.h_mask = vp.h_mask;
+  const std::vector<uint8_t> bytes = serialize_srs(s);
+  // 8 + 8 + sum(8 + 48 n) + 48 + 96 + 8 + 96 nv
+  CHECK(bytes.size() == 16 + (8 * 4 + 48 * 15) + 48 + 96 + 8 + 96 * nv);
+  std::printf("srs_nv3 %zu %s\n", bytes.size(), hex(sha3_256(bytes.data(), bytes.size()).data(), 8).c_str());
+  const SrsFile r = deserialize_srs(bytes);
+  CHECK(r.num_vars == nv && r.layers == s.layers && same1(r.g, s.g) && same2(r.h, s.h));
+  for (size_t i = 0; i < nv; ++i) CHECK(same2(r.h_mask[i], s.h_mask[i]));
+  for (size_t i = 0; i <= nv; ++i)
+    for (size_t x = 0; x < points[i].size(); ++x)
+      CHECK(same1(g1_decompress(r.layers[i].data() + 48 * x), points[i][x]));
+  CHECK(same1(points[nv][0], g1));
+  // the trimmed verifier key keeps the last masks; prover layers start at N - nv
+  CHECK(r.verifier_param(2).h_mask.size() == 2 && same2(r.verifier_param(2).h_mask[0], s.h_mask[1]));
+  CHECK(r.prover_layers(2).size() == 3 && r.prover_layers(2)[0] == &r.layers[1]);
+  // file round trip; no overwrite
+  const std::string path = dir + "/srs_nv3.bin";
+  std::remove(path.c_str());
+  write_srs_file(path, s);
+  CHECK(throws([&] { write_srs_file(path, s); }));
+  CHECK(serialize_srs(read_srs_file(path)) == bytes);
+  std::remove(path.c_str());
+  // malformed files
+  auto bad = [&](std::vector<uint8_t> b) { return throws([&] { deserialize_srs(b); }); };
+  CHECK(bad(std::vector<uint8_t>(bytes.begin(), bytes.end() - 1)));
+  {
+    auto b = bytes;
+    b.push_back(0);
+    CHECK(bad(b));
+  }
+  {
+    auto b = bytes;
+    b[8] = 3;  // layer count != num_vars + 1
+    CHECK(bad(b));
+  }
+  {
+    auto b = bytes;
+    b[16] = 7;  // first layer length
+    CHECK(bad(b));
+  }
+  {
+    auto b = bytes;
+    const size_t g_at = 16 + (8 * 4 + 48 * 15);
+    b[g_at] &= 0x7f;  // g not flagged compressed
+    CHECK(bad(b));
+  }
+  {
+    // first mask with an x whose x^3 + 4 (1 + u) has a non-square norm
+    auto b = bytes;
+    const size_t mask_at = 16 + (8 * 4 + 48 * 15) + 48 + 96 + 8;
+    bool placed = false;
+    for (uint8_t v = 1; v < 200 && !placed; ++v) {
+      b[mask_at + 95] = v;
+      const Fq2 x{srs_detail::get_be(b.data() + mask_at + 48), srs_detail::get_be(b.data() + mask_at)};
+      const Fq2 rhs = x.square() * x + Fq2{Fq::from_u64(4), Fq::from_u64(4)};
+      placed = (rhs.c0.square() + rhs.c1.square()).pow(half) != Fq::one();
+    }
+    CHECK(placed);
+    CHECK(bad(b));
+  }
+  std::printf("SRS %s\n", failures ? "FAIL" : "PASS");
+}
+
 int main(int argc, char** argv) {
   if (argc >= 2 && std::string(argv[1]) == "kat") {
     kat();
@@ -220,8 +401,12 @@ int main(int argc, char** argv) {
   }
   if (argc >= 6 && std::string(argv[1]) == "cpu-prove")
     return cpu_prove(std::strtoul(argv[2], nullptr, 10), argv[3], std::atoi(argv[4]), std::atoi(argv[5]));
+  if (argc >= 3 && std::string(argv[1]) == "srs") {
+    srs_checks(argv[2]);
+    return failures ? 1 : 0;
+  }
   if (argc >= 5 && std::string(argv[1]) == "prove")
     return prove_case(std::strtoul(argv[2], nullptr, 10), argv[3], std::strtoull(argv[4], nullptr, 0));
-  std::fprintf(stderr, "usage: %s kat | prove <nv> <vanilla|jellyfish> <seed>\n", argv[0]);
+  std::fprintf(stderr, "usage: %s kat | prove <nv> <vanilla|jellyfish> <seed> | srs <dir>\n", argv[0]);
   return 2;
 }

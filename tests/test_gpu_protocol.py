@@ -20,3 +20,11 @@ def test_gpu_proofs_match_oracle_proofs():
     assert res.returncode == 0, res.stdout + res.stderr
     assert "PROTOCOL_GPU PASS" in res.stdout
     assert res.stdout.count("MATCH") == 4 and "MISMATCH" not in res.stdout
+
+
+def test_srs_file_round_trip_on_device(tmp_path):
+    assert os.path.exists(BIN), "build the test programs first (make -C tests/cpp)"
+    res = subprocess.run([BIN, "srs", "10", str(tmp_path)], capture_output=True, text=True, timeout=600)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "SRS_GPU PASS" in res.stdout

@@ -6,7 +6,11 @@ Pins: Keccak-f[1600] through SHA3-256 against hashlib at lengths around the
 rate; the Merlin construction through merlin's published simple-transcript
 challenge; from_le_bytes_mod_order through Python integers; the compressed G1
 encoding through the standard compressed BLS12-381 generator; the pairing
-through bilinearity, non-degeneracy and order r (checked inside the program)."""
+through bilinearity, non-degeneracy and order r (checked inside the program).
+SRS files: the compressed G2 generator against its standard encoding, square
+roots in Fq and Fq2 against Euler's criterion, decompression round trips of
+random points and their negations, a small SRS written, read and decompressed
+point by point, and malformed files refused."""
 import hashlib
 import os
 import subprocess
@@ -17,6 +21,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "_build", "protocol_test")
 R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 MERLIN_SIMPLE = "d5a21972d0d5fe320c0d263fac7fffb8145aa640af6e9bca177c03c7efcf0615"
+G2_GEN_COMPRESSED = (
+    "93e02b6052719f607dacd3a088274f65596bd0d09920b61ab5da61bbdc7f5049334cf11213945d57e5ac7d055d042b7e"
+    "024aa2b2f08f0a91260805272dc51051c6e47ad4fa403b02b4510b647ae3d1770bac0326a805bbefd48056c8c121bdb8"
+)
 G1_GEN_COMPRESSED = (
     "97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac586c55e83ff97a1aeffb3af00adb22c6bb"
 )
@@ -67,3 +75,14 @@ def test_prove_verify_on_oracle_backend(nv, gate, seed):
     res = subprocess.run([BIN, "prove", str(nv), gate, str(seed)], capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "PROVE PASS" in res.stdout
+
+
+def test_srs_file_encoding(tmp_path):
+    build()
+    res = subprocess.run([BIN, "srs", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "SRS PASS" in res.stdout
+    rows = {l.split()[0]: l.split()[1:] for l in res.stdout.splitlines() if l.strip()}
+    assert rows["g2gen"] == [G2_GEN_COMPRESSED]
+    # num_vars 3: 16 header bytes, 4 layers of 8 + 48 n bytes, g, h, masks
+    assert int(rows["srs_nv3"][0]) == 16 + 4 * 8 + 48 * 15 + 48 + 96 + 8 + 96 * 3
