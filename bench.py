@@ -18,7 +18,9 @@ workload): `fold` (HBM rate of the fix_variables fold, HIP events), `proofs`
 (plain one-at-a-time HyperPlonk prove, vanilla and jellyfish, nv 12, with the
 oracle prover as its CPU baseline), `sumcheck` (n = 20 whole proofs, degree 2
 and 5), `batch` (13 uneven commits + 64 sumchecks, serial vs batched, and the
-standalone MSM sizes 2^16..2^22) and `serving` (a Poisson request trace served
+standalone MSM sizes 2^16..2^22), `srs` (device SRS generation with the
+canonical file write, then the file read back and decompressed on the device,
+nv 20) and `serving` (a Poisson request trace served
 with continuous batching vs serially: proofs/s and p50/p99 latency; the full
 2000-request, nv 10..20 trace with --serve-full).
 """
@@ -26,6 +28,7 @@ import argparse
 import json
 import os
 import subprocess
+import tempfile
 import sys
 import time
 
@@ -73,11 +76,12 @@ def run_json(cmd, timeout):
     print(f"[bench] {' '.join(os.path.basename(c) if i == 0 else c for i, c in enumerate(cmd))}",
           file=sys.stderr, flush=True)
     try:
-        res = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        # stderr (progress lines, errors) passes through; stdout carries the JSON
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout)
     except subprocess.TimeoutExpired:
         return {"error": f"timeout after {timeout}s"}
     if res.returncode != 0:
-        return {"error": f"exit {res.returncode}: {res.stderr.strip()[-300:]}"}
+        return {"error": f"exit {res.returncode} (its stderr is in the log)"}
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
@@ -117,6 +121,9 @@ def workloads(args, be, rng):
         "degree5_n20": run_json([PROVE_BENCH, "sumcheck", "20", "5", "7", "7"], 300),
     }
     out["batch"] = run_json([PROVE_BENCH, "batch", "3", "4"], 600)
+    # SRS: device generation + canonical file write, then file read + device decompression (nv 20)
+    with tempfile.TemporaryDirectory() as tmp:
+        out["srs"] = run_json([PROVE_BENCH, "srs", "20", os.path.join(tmp, "srs_nv20.bin")], 300)
     if args.serve_full:
         out["serving"] = run_json([PROVE_BENCH, "serve", "2000", "auto", "10", "20", "0.3", "16", "5"], 3000)
     else:

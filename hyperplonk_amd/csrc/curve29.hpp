@@ -1,0 +1,110 @@
+// G1 group law of the MSM bucket stages on Fq29 (lazy reduction).
+//
+// XYZZ coordinates as in curve.hpp (x = X / ZZ, y = Y / ZZZ, ZZ^3 = ZZZ^2,
+// identity: ZZ = 0 with all-zero limbs).  Every operation keeps
+//   X < 10 p,  Y < 4 p,  ZZ, ZZZ < 2 p   (normalised limbs),
+// which fixes the multiple of p each subtraction adds:
+//   madd-2008-s:  P = U2 + 16p - X1 (< 18p), R = S2 + 4p - Y1 (< 6p),
+//                 X3 = R^2 + 8p - (PPP + 2Q) (< 10p),
+//                 Y3 = R (Q + 16p - X3) + 2p - Y1 PPP (< 4p);
+//   add-2008-s:   P = U2 + 2p - U1, R = S2 + 2p - S1 (< 4p), X3, Y3 as above;
+//   dbl-2008-s-1: M = 3 X^2 (< 6p), X3 = M^2 + 4p - 2S (< 6p),
+//                 Y3 = M (S + 16p - X3) + 2p - W Y (< 4p).
+// Products stay below p (1 + 2^-15) for inputs below 32 p.  Exceptional
+// cases (identity operands, P == Q, P == -Q) are exact, tested with
+// is_zero_mod, so the group element does not depend on the order of
+// accumulation.
+#pragma once
+#include "curve.hpp"
+#include "field29.hpp"
+
+namespace hp {
+namespace f29 {
+
+struct Xyzz29 {
+  Fq29 X, Y, ZZ, ZZZ;
+  HP_HD bool is_identity() const { return all_zero(ZZ); }
+  HP_HD static Xyzz29 identity() { return {one(), one(), zero(), zero()}; }
+};
+
+struct Aff29 {
+  Fq29 x, y;
+};
+
+HP_HD Xyzz29 from_affine(const Aff29& a) { return {a.x, a.y, one(), one()}; }
+
+// 2 (x, y), y != 0 (mdbl-2008-s-1)
+HP_HD Xyzz29 dbl_affine(const Aff29& a) {
+  const Fq29 U = add(a.y, a.y);
+  const Fq29 V = sqr(U);
+  const Fq29 W = mul(U, V);
+  const Fq29 S = mul(a.x, V);
+  const Fq29 xx = sqr(a.x);
+  const Fq29 M = add_twice(xx, xx);
+  const Fq29 X3 = sub4p(sqr(M), add(S, S));
+  const Fq29 Y3 = sub2p(mul(M, sub16p(S, X3)), mul(W, a.y));
+  return {X3, Y3, V, W};
+}
+
+HP_HD Xyzz29 dbl(const Xyzz29& p) {
+  if (p.is_identity() || is_zero_mod(p.Y)) return Xyzz29::identity();
+  const Fq29 U = add(p.Y, p.Y);
+  const Fq29 V = sqr(U);
+  const Fq29 W = mul(U, V);
+  const Fq29 S = mul(p.X, V);
+  const Fq29 xx = sqr(p.X);
+  const Fq29 M = add_twice(xx, xx);
+  const Fq29 X3 = sub4p(sqr(M), add(S, S));
+  const Fq29 Y3 = sub2p(mul(M, sub16p(S, X3)), mul(W, p.Y));
+  return {X3, Y3, mul(V, p.ZZ), mul(W, p.ZZZ)};
+}
+
+// P + Q, Q affine and finite with canonical coordinates (madd-2008-s)
+HP_HD Xyzz29 add_mixed(const Xyzz29& p, const Aff29& q) {
+  if (p.is_identity()) return from_affine(q);
+  const Fq29 U2 = mul(q.x, p.ZZ);
+  const Fq29 S2 = mul(q.y, p.ZZZ);
+  const Fq29 P = sub16p(U2, p.X);
+  const Fq29 R = sub4p(S2, p.Y);
+  if (is_zero_mod(P)) {
+    if (is_zero_mod(R)) return dbl_affine(q);
+    return Xyzz29::identity();
+  }
+  const Fq29 PP = sqr(P);
+  const Fq29 PPP = mul(P, PP);
+  const Fq29 Qv = mul(p.X, PP);
+  const Fq29 X3 = sub8p(sqr(R), add_twice(PPP, Qv));
+  const Fq29 Y3 = sub2p(mul(R, sub16p(Qv, X3)), mul(p.Y, PPP));
+  return {X3, Y3, mul(p.ZZ, PP), mul(p.ZZZ, PPP)};
+}
+
+// P + Q, both XYZZ (add-2008-s)
+HP_HD Xyzz29 add(const Xyzz29& p, const Xyzz29& q) {
+  if (q.is_identity()) return p;
+  if (p.is_identity()) return q;
+  const Fq29 U1 = mul(p.X, q.ZZ);
+  const Fq29 U2 = mul(q.X, p.ZZ);
+  const Fq29 S1 = mul(p.Y, q.ZZZ);
+  const Fq29 S2 = mul(q.Y, p.ZZZ);
+  const Fq29 P = sub2p(U2, U1);
+  const Fq29 R = sub2p(S2, S1);
+  if (is_zero_mod(P)) {
+    if (is_zero_mod(R)) return dbl(p);
+    return Xyzz29::identity();
+  }
+  const Fq29 PP = sqr(P);
+  const Fq29 PPP = mul(P, PP);
+  const Fq29 Qv = mul(U1, PP);
+  const Fq29 X3 = sub8p(sqr(R), add_twice(PPP, Qv));
+  const Fq29 Y3 = sub2p(mul(R, sub16p(Qv, X3)), mul(S1, PPP));
+  return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
+}
+
+// Conversions to the 32-bit-limb XYZZ form (identity: ZZ = ZZZ = 0).
+HP_HD G1Xyzz to_xyzz(const Xyzz29& p) {
+  if (p.is_identity()) return G1Xyzz::identity();
+  return {from29(p.X), from29(p.Y), from29(p.ZZ), from29(p.ZZZ)};
+}
+
+}  // namespace f29
+}  // namespace hp

@@ -34,6 +34,7 @@
 #include "mle.hpp"
 #include "host_g1.hpp"
 #include "msm.hpp"
+#include "curve29.hpp"
 
 namespace hp {
 namespace msm {
@@ -256,34 +257,71 @@ __global__ void k_build_tasks(const uint32_t* counts, const uint32_t* starts,
   }
 }
 
+using f29::Fq29;
+using f29::Xyzz29;
+
+// Xyzz29 = 56 words = 14 x 16 bytes; word w is limb w % 14 of coordinate w / 14.
+__device__ __forceinline__ uint32_t& xyzz29_word(Xyzz29& v, int w) {
+  f29::Fq29& c = w < 14 ? v.X : w < 28 ? v.Y : w < 42 ? v.ZZ : v.ZZZ;
+  return c.l[w % 14];
+}
+__device__ __forceinline__ uint32_t xyzz29_word(const Xyzz29& v, int w) {
+  const f29::Fq29& c = w < 14 ? v.X : w < 28 ? v.Y : w < 42 ? v.ZZ : v.ZZZ;
+  return c.l[w % 14];
+}
+
+__device__ __forceinline__ void store_xyzz29(Xyzz29* base, size_t i, const Xyzz29& v) {
+  uint4* q = reinterpret_cast<uint4*>(base + i);
+#pragma unroll
+  for (int k = 0; k < 14; ++k)
+    q[k] = make_uint4(xyzz29_word(v, 4 * k), xyzz29_word(v, 4 * k + 1),
+                      xyzz29_word(v, 4 * k + 2), xyzz29_word(v, 4 * k + 3));
+}
+
+__device__ __forceinline__ Xyzz29 load_xyzz29(const Xyzz29* base, size_t i) {
+  const uint4* q = reinterpret_cast<const uint4*>(base + i);
+  Xyzz29 v;
+#pragma unroll
+  for (int k = 0; k < 14; ++k) {
+    const uint4 a = q[k];
+    xyzz29_word(v, 4 * k) = a.x;
+    xyzz29_word(v, 4 * k + 1) = a.y;
+    xyzz29_word(v, 4 * k + 2) = a.z;
+    xyzz29_word(v, 4 * k + 3) = a.w;
+  }
+  return v;
+}
+
 // One lane per task: each task is <= task_cap consecutive entries of one
 // bucket, so a wavefront holds 64 independent XYZZ accumulators and no lane
-// waits on a cross-lane reduction.
+// waits on a cross-lane reduction.  Bases hold packed Fq29 coordinates.
 __global__ void __launch_bounds__(256) k_accumulate(const Task* tasks, const uint32_t* ntasks,
                                                     const uint32_t* sorted, const Job* jobs,
-                                                    G1Xyzz* partials) {
+                                                    Xyzz29* partials) {
   const uint32_t task = blockIdx.x * blockDim.x + threadIdx.x;
   if (task >= *ntasks) return;
   const Task t = tasks[task];
   const G1Affine* bases = jobs[t.job].bases;
-  G1Xyzz acc = G1Xyzz::identity();
+  Xyzz29 acc = Xyzz29::identity();
   for (uint32_t e = 0; e < t.len; ++e) {
     const uint32_t v = sorted[t.start + e];
-    G1Affine p = load_affine(bases, v & 0x7fffffffu);
-    if (v >> 31) p.y = neg(p.y);
-    acc = add_mixed(acc, p);
+    const G1Affine raw = load_affine(bases, v & 0x7fffffffu);
+    if (raw.is_identity()) continue;
+    f29::Aff29 p{f29::unpack(raw.x.v), f29::unpack(raw.y.v)};
+    if (v >> 31) p.y = f29::neg_canonical(p.y);
+    acc = f29::add_mixed(acc, p);
   }
-  store_xyzz(partials, task, acc);
+  store_xyzz29(partials, task, acc);
 }
 
-__device__ __forceinline__ G1Xyzz shfl_xor_xyzz(const G1Xyzz& v, unsigned mask) {
-  G1Xyzz r;
+__device__ __forceinline__ Xyzz29 shfl_xor_xyzz(const Xyzz29& v, unsigned mask) {
+  Xyzz29 r;
 #pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    r.X.v[i] = __shfl_xor(v.X.v[i], mask, 64);
-    r.Y.v[i] = __shfl_xor(v.Y.v[i], mask, 64);
-    r.ZZ.v[i] = __shfl_xor(v.ZZ.v[i], mask, 64);
-    r.ZZZ.v[i] = __shfl_xor(v.ZZZ.v[i], mask, 64);
+  for (int i = 0; i < f29::L; ++i) {
+    r.X.l[i] = __shfl_xor(v.X.l[i], mask, 64);
+    r.Y.l[i] = __shfl_xor(v.Y.l[i], mask, 64);
+    r.ZZ.l[i] = __shfl_xor(v.ZZ.l[i], mask, 64);
+    r.ZZZ.l[i] = __shfl_xor(v.ZZZ.l[i], mask, 64);
   }
   return r;
 }
@@ -296,9 +334,9 @@ __device__ __forceinline__ G1Xyzz shfl_xor_xyzz(const G1Xyzz& v, unsigned mask) 
 constexpr uint32_t kSeqMax = 8;
 __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
                                                      const uint32_t* ntask,
-                                                     const G1Xyzz* partials, const Job* jobs,
+                                                     const Xyzz29* partials, const Job* jobs,
                                                      uint32_t njobs, uint32_t total_threads,
-                                                     G1Xyzz* sums, uint32_t* heavy,
+                                                     Xyzz29* sums, uint32_t* heavy,
                                                      uint32_t* heavy_count) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   // job owning this thread (last with sum_base <= gid); wave-uniform since
@@ -315,21 +353,22 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
   const uint32_t lb = local >> lanes_log2, lane = local & (group - 1);
   const uint32_t b = job.bucket_base + lb;
   bool live = gid < total_threads && lb < job.padded_buckets;  // whole groups share liveness
-  G1Xyzz s = G1Xyzz::identity();
+  Xyzz29 s = Xyzz29::identity();
   if (live) {
     const uint32_t first = task_off[b], cnt = ntask[b];
     if (cnt > job.seq_max * group) {
       if (lane == 0) heavy[atomicAdd(heavy_count, 1u)] = b;
       live = false;
     } else {
-      for (uint32_t k = lane; k < cnt; k += group) s = add(s, load_xyzz(partials, first + k));
+      for (uint32_t k = lane; k < cnt; k += group)
+        s = f29::add(s, load_xyzz29(partials, first + k));
     }
   }
   for (uint32_t m = 1; m < group; m <<= 1) {
-    const G1Xyzz o = shfl_xor_xyzz(s, m);
-    s = add(s, o);
+    const Xyzz29 o = shfl_xor_xyzz(s, m);
+    s = f29::add(s, o);
   }
-  if (live && lane == 0) store_xyzz(sums, b, s);
+  if (live && lane == 0) store_xyzz29(sums, b, s);
 }
 
 // Heavy buckets (many partials).  The first kHeavySplitSlots of the heavy
@@ -346,13 +385,13 @@ __device__ __forceinline__ uint32_t heavy_split(uint32_t cnt) {
   return min(kHeavySplit, max(1u, cnt / (4 * kHeavyThreads)));
 }
 
-__device__ G1Xyzz block_tree_sum(G1Xyzz s) {
-  __shared__ G1Xyzz lds[kHeavyThreads];
+__device__ Xyzz29 block_tree_sum(Xyzz29 s) {
+  __shared__ Xyzz29 lds[kHeavyThreads];
   lds[threadIdx.x] = s;
   __syncthreads();
   for (unsigned d = kHeavyThreads / 2; d > 0; d >>= 1) {
     if (threadIdx.x < d) {
-      s = add(s, lds[threadIdx.x + d]);
+      s = f29::add(s, lds[threadIdx.x + d]);
       lds[threadIdx.x] = s;
     }
     __syncthreads();
@@ -361,8 +400,8 @@ __device__ G1Xyzz block_tree_sum(G1Xyzz s) {
 }
 
 __global__ void __launch_bounds__(kHeavyThreads)
-    k_heavy_partial(const uint32_t* task_off, const uint32_t* ntask, const G1Xyzz* partials,
-                    const uint32_t* heavy, const uint32_t* heavy_count, G1Xyzz* block_sums) {
+    k_heavy_partial(const uint32_t* task_off, const uint32_t* ntask, const Xyzz29* partials,
+                    const uint32_t* heavy, const uint32_t* heavy_count, Xyzz29* block_sums) {
   const uint32_t buckets = min(*heavy_count, kHeavySplitSlots);
   const uint32_t total = buckets * kHeavySplit;
   // part-major: the first `buckets` work items are part 0 of every heavy bucket
@@ -372,71 +411,96 @@ __global__ void __launch_bounds__(kHeavyThreads)
     const uint32_t first = task_off[b], cnt = ntask[b];
     const uint32_t split = heavy_split(cnt);
     if (part >= split) continue;  // uniform per block
-    G1Xyzz s = G1Xyzz::identity();
+    Xyzz29 s = Xyzz29::identity();
     for (uint32_t k = part * kHeavyThreads + threadIdx.x; k < cnt; k += split * kHeavyThreads)
-      s = add(s, load_xyzz(partials, first + k));
+      s = f29::add(s, load_xyzz29(partials, first + k));
     s = block_tree_sum(s);
-    if (threadIdx.x == 0) store_xyzz(block_sums, h * kHeavySplit + part, s);
+    if (threadIdx.x == 0) store_xyzz29(block_sums, h * kHeavySplit + part, s);
     __syncthreads();
   }
 }
 
 __global__ void __launch_bounds__(64) k_heavy_final(const uint32_t* ntask, const uint32_t* heavy,
                                                     const uint32_t* heavy_count,
-                                                    const G1Xyzz* block_sums, G1Xyzz* sums) {
+                                                    const Xyzz29* block_sums, Xyzz29* sums) {
   const uint32_t total = min(*heavy_count, kHeavySplitSlots);
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t h = blockIdx.x; h < total; h += gridDim.x) {
     const uint32_t split = heavy_split(ntask[heavy[h]]);
-    G1Xyzz s = lane < split ? load_xyzz(block_sums, h * kHeavySplit + lane) : G1Xyzz::identity();
-    for (unsigned m = 1; m < split; m <<= 1) s = add(s, shfl_xor_xyzz(s, m));
-    if (lane == 0) store_xyzz(sums, heavy[h], s);
+    Xyzz29 s = lane < split ? load_xyzz29(block_sums, h * kHeavySplit + lane) : Xyzz29::identity();
+    for (unsigned m = 1; m < split; m <<= 1) s = f29::add(s, shfl_xor_xyzz(s, m));
+    if (lane == 0) store_xyzz29(sums, heavy[h], s);
   }
 }
 
 __global__ void __launch_bounds__(kHeavyThreads)
-    k_heavy_sums(const uint32_t* task_off, const uint32_t* ntask, const G1Xyzz* partials,
-                 const uint32_t* heavy, const uint32_t* heavy_count, G1Xyzz* sums) {
+    k_heavy_sums(const uint32_t* task_off, const uint32_t* ntask, const Xyzz29* partials,
+                 const uint32_t* heavy, const uint32_t* heavy_count, Xyzz29* sums) {
   const uint32_t total = *heavy_count;
   for (uint32_t h = kHeavySplitSlots + blockIdx.x; h < total; h += gridDim.x) {
     const uint32_t b = heavy[h];
     const uint32_t first = task_off[b], cnt = ntask[b];
-    G1Xyzz s = G1Xyzz::identity();
+    Xyzz29 s = Xyzz29::identity();
     for (uint32_t k = threadIdx.x; k < cnt; k += kHeavyThreads)
-      s = add(s, load_xyzz(partials, first + k));
+      s = f29::add(s, load_xyzz29(partials, first + k));
     s = block_tree_sum(s);
-    if (threadIdx.x == 0) store_xyzz(sums, b, s);
+    if (threadIdx.x == 0) store_xyzz29(sums, b, s);
     __syncthreads();
   }
 }
 
 // Level 1: blocks of kRed1 = 256 buckets.  U_t = sum_{t' >= t} S_t' (suffix scan),
-// A = sum_t U_t = sum_t (t + 1) S_t, R = U_0.
+// A = sum_t U_t = sum_t (t + 1) S_t, R = U_0; written in the 32-bit-limb form
+// for the host.
 constexpr unsigned kRed1 = 256;
-__global__ void __launch_bounds__(kRed1) k_reduce_buckets(const G1Xyzz* sums, uint32_t buckets,
+__global__ void __launch_bounds__(kRed1) k_reduce_buckets(const Xyzz29* sums, uint32_t buckets,
                                                           G1Xyzz* pair_a, G1Xyzz* pair_r) {
-  __shared__ G1Xyzz u[kRed1];
+  __shared__ Xyzz29 u[kRed1];
   const uint32_t t = threadIdx.x;
   const uint32_t b = blockIdx.x * kRed1 + t;
-  G1Xyzz v = b < buckets ? load_xyzz(sums, b) : G1Xyzz::identity();
+  Xyzz29 v = b < buckets ? load_xyzz29(sums, b) : Xyzz29::identity();
   u[t] = v;
   __syncthreads();
   for (unsigned d = 1; d < kRed1; d <<= 1) {
-    if (t + d < kRed1) v = add(v, u[t + d]);
+    if (t + d < kRed1) v = f29::add(v, u[t + d]);
     __syncthreads();
     u[t] = v;
     __syncthreads();
   }
-  if (t == 0) store_xyzz(pair_r, blockIdx.x, v);
+  if (t == 0) store_xyzz(pair_r, blockIdx.x, f29::to_xyzz(v));
   // tree sum of U
   for (unsigned d = kRed1 / 2; d > 0; d >>= 1) {
     if (t < d) {
-      v = add(v, u[t + d]);
+      v = f29::add(v, u[t + d]);
       u[t] = v;
     }
     __syncthreads();
   }
-  if (t == 0) store_xyzz(pair_a, blockIdx.x, v);
+  if (t == 0) store_xyzz(pair_a, blockIdx.x, f29::to_xyzz(v));
+}
+
+// Prepared bases: 32-bit-limb Montgomery -> packed Fq29 (in place) and back.
+__global__ void k_bases_to29(G1Affine* pts, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    G1Affine p = load_affine(pts, i);
+    if (p.is_identity()) continue;
+    p.x = f29::to29_packed(p.x);
+    p.y = f29::to29_packed(p.y);
+    store_affine(pts, i, p);
+  }
+}
+
+__global__ void k_bases_from29(const G1Affine* pts, G1Affine* out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    G1Affine p = load_affine(pts, i);
+    if (!p.is_identity()) {
+      p.x = f29::from29_packed(p.x);
+      p.y = f29::from29_packed(p.y);
+    }
+    store_affine(out, i, p);
+  }
 }
 
 // ---- SRS preparation --------------------------------------------------------
@@ -751,7 +815,23 @@ void prepare_layer_from_device(rt::Context& c, Layer& layer, const G1Affine* poi
       rt::check_launch();
     }
   }
+  // the bucket stages read packed Fq29 coordinates
+  hipLaunchKernelGGL(k_bases_to29, dim3(grid_for(W * n)), dim3(256), 0, c.stream, bases, W * n);
+  rt::check_launch();
 }
+
+namespace {
+// window 0 of a layer back in the 32-bit-limb Montgomery form
+rt::BufferRef layer_points(rt::Context& c, const Layer& layer) {
+  const size_t n = layer.n;
+  rt::BufferRef out = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Affine));
+  hipLaunchKernelGGL(k_bases_from29, dim3(grid_for(n)), dim3(256), 0, c.stream,
+                     static_cast<const G1Affine*>(layer.bases->ptr),
+                     static_cast<G1Affine*>(out->ptr), n);
+  rt::check_launch();
+  return out;
+}
+}  // namespace
 
 bool upload_layer(rt::Context& c, Layer& layer, const uint8_t* packets, size_t n,
                   uint32_t override_c) {
@@ -907,14 +987,14 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, starts,
                      owner, task_off, G, cap, tasks);
   rt::check_launch();
-  rt::BufferRef partials_buf = c.allocate(max_tasks * sizeof(G1Xyzz));
-  G1Xyzz* partials = static_cast<G1Xyzz*>(partials_buf->ptr);
+  rt::BufferRef partials_buf = c.allocate(max_tasks * sizeof(Xyzz29));
+  Xyzz29* partials = static_cast<Xyzz29*>(partials_buf->ptr);
   mark(3);
   hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((max_tasks + 255) / 256)), dim3(256), 0,
                      c.stream, tasks, totals + 1, sorted, djobs, partials);
   rt::check_launch();
-  rt::BufferRef sums_buf = c.allocate((size_t)G * sizeof(G1Xyzz));
-  G1Xyzz* sums = static_cast<G1Xyzz*>(sums_buf->ptr);
+  rt::BufferRef sums_buf = c.allocate((size_t)G * sizeof(Xyzz29));
+  Xyzz29* sums = static_cast<Xyzz29*>(sums_buf->ptr);
   mark(4);
   // lanes per bucket: enough groups for ~2 wavefronts per SIMD, and no more
   // lanes than about half the expected partials per bucket
@@ -927,8 +1007,8 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
                      c.stream, task_off, ntask, partials, djobs, njobs, (uint32_t)sum_threads_total,
                      sums, heavy, heavy_count);
   rt::check_launch();
-  rt::BufferRef heavy_parts = c.allocate((size_t)kHeavySplitSlots * kHeavySplit * sizeof(G1Xyzz));
-  G1Xyzz* block_sums = static_cast<G1Xyzz*>(heavy_parts->ptr);
+  rt::BufferRef heavy_parts = c.allocate((size_t)kHeavySplitSlots * kHeavySplit * sizeof(Xyzz29));
+  Xyzz29* block_sums = static_cast<Xyzz29*>(heavy_parts->ptr);
   hipLaunchKernelGGL(k_heavy_partial, dim3(1024), dim3(kHeavyThreads), 0, c.stream, task_off, ntask,
                      partials, heavy, heavy_count, block_sums);
   rt::check_launch();
@@ -986,10 +1066,9 @@ void to_packet(const G1Xyzz& p, uint8_t* out) { host::to_packet(host::from_devic
 
 void download_layer(rt::Context& c, const Layer& layer, uint8_t* packets) {
   const size_t n = layer.n;
-  rt::BufferRef tmp = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Affine));
+  rt::BufferRef tmp = layer_points(c, layer);
   hipLaunchKernelGGL(k_points_from_mont, dim3(grid_for(n)), dim3(256), 0, c.stream,
-                     static_cast<const G1Affine*>(layer.bases->ptr),
-                     static_cast<G1Affine*>(tmp->ptr), n);
+                     static_cast<const G1Affine*>(tmp->ptr), static_cast<G1Affine*>(tmp->ptr), n);
   rt::check_launch();
   std::vector<uint8_t> raw(n * sizeof(G1Affine));
   rt::download_bytes(c, tmp->ptr, raw.size(), raw.data());
@@ -1034,10 +1113,10 @@ bool upload_layer_compressed(rt::Context& c, Layer& layer, const uint8_t* compre
 
 void download_layer_compressed(rt::Context& c, const Layer& layer, uint8_t* compressed) {
   const size_t n = layer.n;
+  rt::BufferRef pts = layer_points(c, layer);
   rt::BufferRef tmp = c.allocate(std::max<size_t>(n, 1) * 48);
   hipLaunchKernelGGL(k_compress, dim3(grid_for(n)), dim3(256), 0, c.stream,
-                     static_cast<const G1Affine*>(layer.bases->ptr), n,
-                     static_cast<uint4*>(tmp->ptr));
+                     static_cast<const G1Affine*>(pts->ptr), n, static_cast<uint4*>(tmp->ptr));
   rt::check_launch();
   rt::download_bytes(c, tmp->ptr, n * 48, compressed);
 }

@@ -1,0 +1,184 @@
+// Host-side checks of the radix-2^29 Fq arithmetic and its group law
+// (hyperplonk_amd/csrc/field29.hpp, curve29.hpp; the same source the MSM
+// kernels compile), against the 32-bit-limb arithmetic of field.hpp /
+// curve.hpp and against exact integer identities:
+//   * pack / unpack round trips; conversions Fq <-> Fq29 round trip;
+//   * products of lazy operands (up to 31 p) against the Fq product;
+//   * add / add_twice / sub_kp / neg against Fq;
+//   * is_zero_mod on every k p (k < 32) and on k p +- 1, canonical;
+//   * mixed and general XYZZ additions, doublings and every exceptional case
+//     (identity operands, P + P, P + (-P)) against curve.hpp, compared as
+//     affine points.
+// Prints "FIELD29 PASS" or the failing checks.
+#include <cstdio>
+#include <cstdint>
+#include <random>
+
+#include "../../hyperplonk_amd/csrc/curve29.hpp"
+
+using namespace hp;
+using namespace hp::f29;
+
+static int failures = 0;
+#define CHECK(c)                                                  \
+  do {                                                            \
+    if (!(c)) {                                                   \
+      std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c);     \
+      ++failures;                                                 \
+    }                                                             \
+  } while (0)
+
+static std::mt19937_64 rng(2024);
+
+static Fq rand_fq() {  // canonical, Montgomery form of a random value
+  Fq x;
+  for (;;) {
+    for (int i = 0; i < 12; ++i) x.v[i] = (uint32_t)rng();
+    x.v[11] &= 0x1fffffffu;
+    if (!geq_modulus<FqParams>(x.v)) break;
+  }
+  return to_mont(x);
+}
+
+static Fq29 to29(const Fq& x) { return unpack(to29_packed(x).v); }
+static Fq back(const Fq29& a) { return from29(a); }
+
+// a + k p as limbs (a canonical, k small): a lazy representative of a
+static Fq29 lazy(const Fq29& a, uint32_t k) {
+  Fq29 kp = zero();
+  uint64_t c = 0;
+  for (int i = 0; i < L; ++i) {
+    c += (uint64_t)k * Consts::P[i] + a.l[i];
+    kp.l[i] = i < L - 1 ? ((uint32_t)c & kMask) : (uint32_t)c;
+    c >>= kBits;
+  }
+  return kp;
+}
+
+static G1Affine rand_point(const G1Affine& g) {
+  G1Xyzz acc = G1Xyzz::from_affine(g);
+  const int steps = 5 + (int)(rng() % 40);
+  for (int i = 0; i < steps; ++i) acc = (rng() & 1) ? dbl(acc) : add_mixed(acc, g);
+  return to_affine(acc);
+}
+
+static bool same_point(const G1Xyzz& a, const G1Xyzz& b) {
+  const G1Affine x = to_affine(a), y = to_affine(b);
+  return x.x == y.x && x.y == y.y;
+}
+
+static Aff29 aff29(const G1Affine& a) { return {to29(a.x), to29(a.y)}; }
+
+int main() {
+  // field
+  for (int it = 0; it < 2000; ++it) {
+    const Fq a = rand_fq(), b = rand_fq();
+    const Fq29 a29 = to29(a), b29 = to29(b);
+    CHECK(back(a29) == a);
+    uint32_t w[12];
+    pack(a29, w);
+    Fq29 u = unpack(w);
+    bool same = true;
+    for (int i = 0; i < L; ++i) same = same && u.l[i] == a29.l[i];
+    CHECK(same);
+    const uint32_t ka = (uint32_t)(rng() % 16), kb = (uint32_t)(rng() % 16);
+    CHECK(back(mul(lazy(a29, ka), lazy(b29, kb))) == a * b);
+    CHECK(back(sqr(lazy(a29, ka))) == a * a);
+    CHECK(back(add(lazy(a29, ka), lazy(b29, kb))) == a + b);
+    CHECK(back(add_twice(lazy(a29, ka), b29)) == a + b + b);
+    CHECK(back(sub2p(a29, lazy(b29, 1))) == a - b);
+    CHECK(back(sub4p(lazy(a29, 3), lazy(b29, 3))) == a - b);
+    CHECK(back(sub8p(a29, lazy(b29, 7))) == a - b);
+    CHECK(back(sub16p(lazy(a29, 15), lazy(b29, 15))) == a - b);
+    CHECK(back(neg_canonical(a29)) == neg(a));
+    // product bound: below p (1 + 2^-15) for inputs below 32 p
+    const Fq29 m = mul(lazy(a29, 31), lazy(b29, 31));
+    CHECK(m.l[L - 1] <= Consts::P[L - 1]);
+  }
+  for (uint32_t k = 0; k < 32; ++k) {
+    const Fq29 z = lazy(zero(), k);
+    CHECK(is_zero_mod(z));
+    Fq29 plus = z;
+    plus.l[0] += 1;  // k p + 1 (no carry: low limb of k p is never 2^29 - 1 here)
+    CHECK(plus.l[0] <= kMask && !is_zero_mod(plus));
+    // same low limb as k p, another limb off: the full comparison decides
+    for (int limb = 1; limb < L; ++limb) {
+      Fq29 t = z;
+      t.l[limb] ^= 1u;
+      CHECK(!is_zero_mod(t));
+    }
+    if (k > 0) {
+      const Fq29 one_raw = lazy(zero(), 0);
+      Fq29 o = one_raw;
+      o.l[0] = 1;
+      CHECK(!is_zero_mod(sub_kp(z, o, Consts::P2)) || k == 0);  // k p - 1 + 2p
+    }
+    const Fq29 c = canonical(lazy(to29(rand_fq()), k));
+    CHECK(!is_zero_mod(c) || all_zero(c));
+  }
+  for (int it = 0; it < 200; ++it) {
+    const Fq a = rand_fq();
+    const uint32_t k = (uint32_t)(rng() % 31);
+    const Fq29 c = canonical(lazy(to29(a), k));
+    bool same = true;
+    const Fq29 ref = to29(a);
+    for (int i = 0; i < L; ++i) same = same && c.l[i] == ref.l[i];
+    CHECK(same);
+    CHECK(!is_zero_mod(lazy(to29(a), k)));
+  }
+  // group law
+  G1Affine g;
+  {
+    static const uint32_t gx[12] = {0xdb22c6bbu, 0xfb3af00au, 0xf97a1aefu, 0x6c55e83fu,
+                                    0x171bac58u, 0xa14e3a3fu, 0x9774b905u, 0xc3688c4fu,
+                                    0x4fa9ac0fu, 0x2695638cu, 0x3197d794u, 0x17f1d3a7u};
+    static const uint32_t gy[12] = {0x46c5e7e1u, 0x0caa2329u, 0xa2888ae4u, 0xd03cc744u,
+                                    0x2c04b3edu, 0x00db18cbu, 0xd5d00af6u, 0xfcf5e095u,
+                                    0x741d8ae4u, 0xa09e30edu, 0xe3aaa0f1u, 0x08b3f481u};
+    for (int i = 0; i < 12; ++i) {
+      g.x.v[i] = gx[i];
+      g.y.v[i] = gy[i];
+    }
+    g.x = to_mont(g.x);
+    g.y = to_mont(g.y);
+  }
+  for (int it = 0; it < 200; ++it) {
+    // a chain of mixed additions with repeated and negated points
+    std::vector<G1Affine> pts;
+    for (int i = 0; i < 6; ++i) pts.push_back(rand_point(g));
+    G1Xyzz ref = G1Xyzz::identity();
+    Xyzz29 acc = Xyzz29::identity();
+    for (int s = 0; s < 24; ++s) {
+      G1Affine q = pts[rng() % pts.size()];
+      const int kind = (int)(rng() % 6);
+      if (kind == 0 && !ref.is_identity()) q = to_affine(ref);         // P + P
+      if (kind == 1 && !ref.is_identity()) q = neg(to_affine(ref));    // P + (-P)
+      ref = add_mixed(ref, q);
+      Aff29 q29 = aff29(q);
+      if (rng() & 1) {  // the negated form used for negative digits
+        q29.y = neg_canonical(to29(neg(q.y)));
+      }
+      acc = f29::add_mixed(acc, q29);
+      CHECK(same_point(to_xyzz(acc), ref));
+    }
+    // general additions and doublings of accumulated points
+    Xyzz29 b29 = Xyzz29::identity();
+    G1Xyzz bref = G1Xyzz::identity();
+    for (int s = 0; s < 6; ++s) {
+      const G1Affine q = pts[rng() % pts.size()];
+      bref = add_mixed(bref, q);
+      b29 = f29::add_mixed(b29, aff29(q));
+    }
+    CHECK(same_point(to_xyzz(f29::add(acc, b29)), add(ref, bref)));
+    CHECK(same_point(to_xyzz(f29::add(acc, acc)), add(ref, ref)));
+    CHECK(same_point(to_xyzz(f29::dbl(acc)), dbl(ref)));
+    CHECK(same_point(to_xyzz(f29::add(acc, Xyzz29::identity())), ref));
+    CHECK(same_point(to_xyzz(f29::add(Xyzz29::identity(), b29)), bref));
+    // P + (-P) in XYZZ form
+    Xyzz29 nacc = acc;
+    nacc.Y = sub4p(zero(), acc.Y);
+    CHECK(f29::add(acc, nacc).is_identity() || ref.is_identity());
+  }
+  std::printf("FIELD29 %s\n", failures ? "FAIL" : "PASS");
+  return failures ? 1 : 0;
+}
