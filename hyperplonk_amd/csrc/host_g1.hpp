@@ -139,6 +139,8 @@ inline G1Xyzz to_device(const Xyzz64& p) {
   return r;
 }
 
+inline Xyzz64 neg(const Xyzz64& p) { return {p.X, sub(Fq64{}, p.Y), p.ZZ, p.ZZZ}; }
+
 inline Xyzz64 dbl(const Xyzz64& p) {
   if (p.is_identity() || is_zero(p.Y)) return Xyzz64{{}, {}, {}, {}};
   const Fq64 U = add(p.Y, p.Y), V = mul(U, U), W = mul(U, V), S = mul(p.X, V);

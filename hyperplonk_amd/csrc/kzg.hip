@@ -70,8 +70,7 @@ void run_batch(rt::Context& c, const msm::Batch& batch, uint8_t* out97) {
   batch.enqueue(c, static_cast<G1Xyzz*>(res->ptr));
   std::vector<G1Xyzz> host(slots);
   rt::download_bytes(c, res->ptr, slots * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
-  for (size_t j = 0; j < batch.size(); ++j)
-    msm::to_packet(batch.finish(j, host.data()), out97 + rt::kG1Bytes * j);
+  batch.finish_packets(host.data(), out97);
 }
 
 }  // namespace

@@ -1033,22 +1033,68 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   }
 }
 
-// Job j: sum_k A_k + kRed1 * sum_k k R_k over its level-1 groups, on the host.
-G1Xyzz Batch::finish(size_t job, const G1Xyzz* slots) const {
-  const size_t all = gbuckets_ / kRed1;
-  const size_t g0 = entries_[job].bucket_base / kRed1;
-  const size_t groups = (entries_[job].layer->plan.buckets + kRed1 - 1) / kRed1;
-  const G1Xyzz* A = slots + g0;
-  const G1Xyzz* R = slots + all + g0;
-  host::Xyzz64 total = host::from_device(A[0]);
-  for (size_t k = 1; k < groups; ++k) total = host::add(total, host::from_device(A[k]));
-  host::Xyzz64 run{}, acc{};
-  for (size_t k = groups; k-- > 1;) {
-    run = host::add(run, host::from_device(R[k]));
-    acc = host::add(acc, run);
+// Host end of every MSM of the batch: MSM j is sum_k A_k + kRed1 sum_k k R_k
+// over its level-1 groups k.  The groups are cut into chunks of kFinishChunk
+// (c covers [c L, c L + L)), summed on the worker pool, then combined per MSM:
+//   sum_k k R_k = sum_c (W_c - S_c) + L sum_c c S_c,
+// with S_c = sum R_k and W_c = sum (k - c L + 1) R_k over chunk c.
+namespace {
+constexpr size_t kFinishChunk = 16;
+
+struct ChunkSums {
+  host::Xyzz64 a{}, s{}, w{};  // zero ZZ: the identity
+};
+
+ChunkSums chunk_sums(const G1Xyzz* A, const G1Xyzz* R, size_t lo, size_t hi) {
+  ChunkSums c;
+  for (size_t k = lo; k < hi; ++k) c.a = host::add(c.a, host::from_device(A[k]));
+  for (size_t k = hi; k-- > lo;) {
+    c.s = host::add(c.s, host::from_device(R[k]));
+    c.w = host::add(c.w, c.s);
   }
-  for (int s = 0; s < 8; ++s) acc = host::dbl(acc);  // kRed1 = 256 buckets per group
-  return host::to_device(host::add(total, acc));
+  return c;
+}
+
+host::Xyzz64 combine(const ChunkSums* cs, size_t n) {
+  host::Xyzz64 a{}, t1{}, run{}, t2{};
+  for (size_t c = 0; c < n; ++c) {
+    a = host::add(a, cs[c].a);
+    t1 = host::add(t1, host::add(cs[c].w, host::neg(cs[c].s)));
+  }
+  for (size_t c = n; c-- > 1;) {
+    run = host::add(run, cs[c].s);
+    t2 = host::add(t2, run);
+  }
+  for (size_t m = 1; m < kFinishChunk; m <<= 1) t2 = host::dbl(t2);
+  host::Xyzz64 acc = host::add(t1, t2);
+  for (unsigned m = 1; m < kRed1; m <<= 1) acc = host::dbl(acc);
+  return host::add(a, acc);
+}
+}  // namespace
+
+void Batch::finish_packets(const G1Xyzz* slots, uint8_t* out97) const {
+  const size_t all = gbuckets_ / kRed1, njobs = entries_.size();
+  struct Item {
+    size_t job, lo, hi;
+  };
+  std::vector<Item> items;
+  std::vector<size_t> first(njobs + 1);
+  for (size_t j = 0; j < njobs; ++j) {
+    first[j] = items.size();
+    const size_t groups = (entries_[j].layer->plan.buckets + kRed1 - 1) / kRed1;
+    for (size_t lo = 0; lo < groups; lo += kFinishChunk)
+      items.push_back({j, lo, std::min(groups, lo + kFinishChunk)});
+  }
+  first[njobs] = items.size();
+  std::vector<ChunkSums> sums(items.size());
+  rt::parallel_for(items.size(), [&](size_t i) {
+    const size_t g0 = entries_[items[i].job].bucket_base / kRed1;
+    sums[i] = chunk_sums(slots + g0, slots + all + g0, items[i].lo, items[i].hi);
+  });
+  rt::parallel_for(njobs, [&](size_t j) {
+    host::to_packet(combine(sums.data() + first[j], first[j + 1] - first[j]),
+                    out97 + rt::kG1Bytes * j);
+  });
 }
 
 void affine_to_packet(const G1Affine& a, uint8_t* out) {

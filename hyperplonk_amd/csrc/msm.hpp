@@ -30,7 +30,7 @@ void autotune(rt::Context& c, size_t log_n, uint32_t* best_c, uint32_t* best_cap
 struct Layer {
   size_t n = 0;               // points per window copy
   Plan plan;
-  rt::BufferRef bases;        // W * n affine Montgomery points, window-major
+  rt::BufferRef bases;        // W * n affine points, window-major, packed Fq29 coordinates
 };
 
 // Validate 97-byte packets on the host (tags, canonical coordinates) and
@@ -51,7 +51,8 @@ struct StageEvents {
 // together: every stage is one launch over all of them in one global bucket
 // space, so many small MSMs cost about as much as one.  enqueue() leaves
 // result_slots() XYZZ values in `result` (device); after they are copied to
-// the host, finish(j) returns MSM j.  The layers must outlive the launch.
+// the host, finish_packets() writes every MSM's point.  The layers must outlive
+// the launch.
 class Batch {
  public:
   void add(const Layer& layer, const Fr* scalars);
@@ -59,7 +60,8 @@ class Batch {
   void set_cap(uint32_t cap) { cap_override_ = cap; }
   size_t result_slots() const;
   void enqueue(rt::Context& c, G1Xyzz* result, StageEvents* events = nullptr) const;
-  G1Xyzz finish(size_t job, const G1Xyzz* host_slots) const;
+  // After the result slots are copied to the host: one 97-byte packet per MSM.
+  void finish_packets(const G1Xyzz* host_slots, uint8_t* out97) const;
 
  private:
   struct Entry {

@@ -7,7 +7,11 @@
 // freed while work may still read them, and context_destroy drains the stream
 // before returning memory to the driver.
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
+#include <thread>
 #include <unordered_set>
 
 #include "kernels_common.hpp"
@@ -210,6 +214,82 @@ void download_fr(Context& c, const Fr* src, size_t count, uint8_t* host) {
                      static_cast<Fr*>(tmp->ptr), count);
   check_launch();
   download_bytes(c, tmp->ptr, count * kFrBytes, host);
+}
+
+namespace {
+// A batch of items: workers and the caller claim indices until none is left.
+struct ForkJoin {
+  const std::function<void(size_t)>* body;
+  size_t n;
+  std::atomic<size_t> next{0}, done{0};
+  std::mutex mu;
+  std::condition_variable cv;
+  void run() {
+    for (size_t i; (i = next.fetch_add(1)) < n;) {
+      (*body)(i);
+      if (done.fetch_add(1) + 1 == n) {
+        std::lock_guard<std::mutex> g(mu);
+        cv.notify_all();
+      }
+    }
+  }
+};
+
+class Pool {
+ public:
+  static Pool& get() {
+    static Pool* pool = new Pool();  // workers live until the process ends
+    return *pool;
+  }
+  size_t workers() const { return threads_.size(); }
+  void submit(const std::shared_ptr<ForkJoin>& job, size_t helpers) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (size_t k = 0; k < helpers; ++k) queue_.push_back(job);
+    }
+    cv_.notify_all();
+  }
+
+ private:
+  Pool() {
+    const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
+    const unsigned n = std::min(7u, hw - 1);
+    for (unsigned k = 0; k < n; ++k) threads_.emplace_back([this] { loop(); });
+    for (auto& t : threads_) t.detach();
+  }
+  void loop() {
+    for (;;) {
+      std::shared_ptr<ForkJoin> job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !queue_.empty(); });
+        job = std::move(queue_.front());
+        queue_.pop_front();
+      }
+      job->run();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::shared_ptr<ForkJoin>> queue_;
+  std::vector<std::thread> threads_;
+};
+}  // namespace
+
+void parallel_for(size_t n, const std::function<void(size_t)>& body) {
+  if (n == 0) return;
+  if (n == 1) {
+    body(0);
+    return;
+  }
+  auto job = std::make_shared<ForkJoin>();
+  job->body = &body;
+  job->n = n;
+  Pool& pool = Pool::get();
+  pool.submit(job, std::min(n - 1, pool.workers()));
+  job->run();
+  std::unique_lock<std::mutex> lk(job->mu);
+  job->cv.wait(lk, [&] { return job->done.load() == n; });
 }
 
 bool context_live(HpMi355xContext* ctx) {

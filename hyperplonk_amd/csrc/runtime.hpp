@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -128,6 +129,11 @@ struct Context {
 };
 
 bool context_live(HpMi355xContext* ctx);
+
+// Host fork-join over n independent items on a process-wide pool of worker
+// threads (the calling thread works too); returns when every item is done.
+// Used for the host ends of MSMs (combining bucket-reduction groups).
+void parallel_for(size_t n, const std::function<void(size_t)>& body);
 
 // Launch-error check after kernel launches.
 inline void check_launch() { hip_check(hipGetLastError()); }
