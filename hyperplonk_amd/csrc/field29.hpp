@@ -1,38 +1,39 @@
-// BLS12-381 Fq on 14 limbs of 29 bits, Montgomery form with R = 2^406, lazy
-// reduction: the arithmetic of the MSM bucket stages.
+// BLS12-381 prime fields on limbs of 29 bits, Montgomery form with R = 2^(29 L),
+// lazy reduction: Fq29 (14 limbs, R = 2^406) is the arithmetic of the MSM
+// bucket stages, Fr29 (9 limbs, R = 2^261) that of the sumcheck rounds.
 //
 // Why this radix: a 29 x 29-bit product is below 2^58, so a column of the
-// Montgomery product (at most 28 products plus the carry) fits a 64-bit
+// Montgomery product (at most 2 L products plus the carry) fits a 64-bit
 // accumulator and every limb product is ONE v_mad_u64_u32 with no carry
-// bookkeeping, against a multiply-add plus a carry add on 32-bit limbs.
-// 14^2 + 14^2 = 392 products cost fewer VALU instructions than 2 x 288.
+// bookkeeping, against a multiply-add plus a carry add on 32-bit limbs
+// (2 * 14^2 = 392 products for Fq against 2 x 288 instructions).
 //
-// Lazy reduction: R / p > 2^25, so for inputs below 16 p the product
-// (a b + m p) / R is below p (1 + 2^-17): multiplication never needs a final
-// subtraction, additions only propagate carries, and a subtraction adds a
-// multiple k p of the modulus that bounds the subtrahend.  Values therefore
-// live in [0, 32 p) with normalised limbs (< 2^29, the top limb holds the
-// rest); only equality with zero needs the exact test is_zero_mod.
+// Lazy reduction: (a b + m p) / R < a b / R + p, and R / p is 2^25.3 (Fq) or
+// 70.7 (Fr), so products of operands a few times p land just above p: no
+// final subtraction; additions only propagate carries; a subtraction adds a
+// multiple k p of the modulus that bounds the subtrahend.  Values stay well
+// below R with normalised limbs (< 2^29, the top limb holds the rest); equality
+// with zero needs the exact test is_zero_mod.
 //
-// Storage: prepared SRS points hold x 2^406 mod p (canonical, < p) packed in
-// twelve 32-bit words, the same 96 bytes as the 32-bit-limb form.
+// Storage: Fq29 prepared SRS points hold x 2^406 mod p (canonical, < p) packed
+// in twelve 32-bit words, the same 96 bytes as the 32-bit-limb form.  Fr29 is
+// applied to values stored in the 32-bit-limb Montgomery form A = a 2^256
+// (read as integers): mul(c 2^261, A) = c A, so a prescaled operand keeps
+// results in the stored form (see the sumcheck kernels).
 #pragma once
 #include "field.hpp"
 
 namespace hp {
 namespace f29 {
 
-constexpr int L = 14;
 constexpr uint32_t kBits = 29;
 constexpr uint32_t kMask = (1u << kBits) - 1u;
-constexpr uint32_t kInv = 0x1ffcfffdu;  // -p^-1 mod 2^29
 
-struct Fq29 {
-  uint32_t l[L];
-};
-
-// p, 2p, 4p, 8p, 16p and R mod p (Montgomery one), normalised limbs
-struct Consts {
+struct FqP {
+  static constexpr int L = 14;
+  static constexpr int Words = 12;              // packed form
+  static constexpr uint32_t INV = 0x1ffcfffdu;  // -p^-1 mod 2^29
+  // p, 2p, 4p, 8p, 16p, normalised limbs
   static constexpr uint32_t P[L] = {0x1fffaaab, 0xff7ffff, 0x14ffffee, 0x17fffd62, 0xf6241ea,
                                     0x9507b58,  0xafd9cc3, 0x109e70a2, 0x1764774b, 0x121a5d66,
                                     0x12c6e9ed, 0x12ffcd34, 0x111ea3,  0xd};
@@ -48,55 +49,66 @@ struct Consts {
   static constexpr uint32_t P16[L] = {0x1ffaaab0, 0x1f7fffff, 0xffffee7,  0x1fffd62a, 0x16241eab,
                                       0x1507b587, 0xfd9cc34,  0x9e70a25,  0x164774b8, 0x1a5d66b,
                                       0xc6e9ed9,  0xffcd349,  0x111ea39,  0xd0};
-  static constexpr uint32_t ONE[L] = {0x3a9fb84,  0xba00690,  0x71288f1,  0xf59bcc5,  0x126cb614,
-                                      0x585bf36,  0x1b85ac3d, 0x1cf856fa, 0x1891ecbd, 0x1a7eec05,
-                                      0x155a88f0, 0x741ac6d,  0x1317c30f, 0x9};
-  // 2^406 mod p and 2^362 as 32-bit words: the Montgomery factors that move
-  // a value between R = 2^384 (Fq) and R = 2^406 (Fq29) with one Fq product
-  static constexpr uint32_t TO29[12] = {0x3a9fb84,  0xc57400d2, 0x629c4a23, 0x6147acde,
-                                        0x7e6d26cb, 0x6b0f4b0b, 0xc2b7d6e1, 0x91ecbde7,
-                                        0x4fdd80b8, 0xd56a23c3, 0xf3a0d636, 0x13317c30};
-  static constexpr uint32_t FROM29[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0x400};
-  static constexpr uint64_t kPTop = 6980443811ull;  // p >> 348
+  static constexpr uint64_t kTop = 6980443811ull;  // p >> (29 (L - 2)) = p >> 348
 };
 
-HP_HD Fq29 zero() {
-  Fq29 r;
+struct FrP {
+  static constexpr int L = 9;
+  static constexpr int Words = 8;
+  static constexpr uint32_t INV = 0x1fffffffu;  // -r^-1 mod 2^29 (r = 1 mod 2^32)
+  static constexpr uint32_t P[L] = {0x1,       0x1ffffff8, 0x1f96ffbf, 0x1b4805ff, 0x1d80553b,
+                                    0xc0404d0, 0x1520cce7, 0xa6533af,  0x73eda7};
+  static constexpr uint32_t P2[L] = {0x2,        0x1ffffff0, 0x1f2dff7f, 0x16900bff, 0x1b00aa77,
+                                     0x180809a1, 0xa4199ce,  0x14ca675f, 0xe7db4e};
+  static constexpr uint32_t P4[L] = {0x4,        0x1fffffe0, 0x1e5bfeff, 0xd2017ff,  0x160154ef,
+                                     0x10101343, 0x1483339d, 0x994cebe,  0x1cfb69d};
+  static constexpr uint32_t P8[L] = {0x8,      0x1fffffc0, 0x1cb7fdff, 0x1a402fff, 0xc02a9de,
+                                     0x202687, 0x906673b,  0x13299d7d, 0x39f6d3a};
+  static constexpr uint32_t P16[L] = {0x10,     0x1fffff80, 0x196ffbff, 0x14805fff, 0x180553bd,
+                                      0x404d0e, 0x120cce76, 0x6533afa,  0x73eda75};
+  static constexpr uint64_t kTop = 4078865654035375ull;  // r >> (29 (L - 2)) = r >> 203
+};
+
+template <class P>
+struct F29 {
+  uint32_t l[P::L];
+};
+using Fq29 = F29<FqP>;
+using Fr29 = F29<FrP>;
+
+template <class P>
+HP_HD F29<P> zero_t() {
+  F29<P> r;
 #pragma unroll
-  for (int i = 0; i < L; ++i) r.l[i] = 0;
+  for (int i = 0; i < P::L; ++i) r.l[i] = 0;
   return r;
 }
 
-HP_HD Fq29 one() {
-  Fq29 r;
-#pragma unroll
-  for (int i = 0; i < L; ++i) r.l[i] = Consts::ONE[i];
-  return r;
-}
-
-HP_HD bool all_zero(const Fq29& a) {
+template <class P>
+HP_HD bool all_zero(const F29<P>& a) {
   uint32_t acc = 0;
 #pragma unroll
-  for (int i = 0; i < L; ++i) acc |= a.l[i];
+  for (int i = 0; i < P::L; ++i) acc |= a.l[i];
   return acc == 0;
 }
 
-// (a b + m p) / 2^406 by product scanning; inputs below 16 p with normalised
-// limbs give an output below p (1 + 2^-17), normalised.
-HP_HD Fq29 mul(const Fq29& a, const Fq29& b) {
+// (a b + m p) / R by product scanning: normalised limbs in, normalised out.
+template <class P>
+HP_HD F29<P> mul(const F29<P>& a, const F29<P>& b) {
+  constexpr int L = P::L;
   uint32_t m[L];
-  Fq29 t;
+  F29<P> t;
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < L; ++k) {
 #pragma unroll
     for (int i = 0; i < k; ++i) {
       acc += (uint64_t)a.l[i] * b.l[k - i];
-      acc += (uint64_t)m[i] * Consts::P[k - i];
+      acc += (uint64_t)m[i] * P::P[k - i];
     }
     acc += (uint64_t)a.l[k] * b.l[0];
-    m[k] = ((uint32_t)acc * kInv) & kMask;
-    acc += (uint64_t)m[k] * Consts::P[0];
+    m[k] = ((uint32_t)acc * P::INV) & kMask;
+    acc += (uint64_t)m[k] * P::P[0];
     acc >>= kBits;  // the low 29 bits are zero
   }
 #pragma unroll
@@ -104,7 +116,7 @@ HP_HD Fq29 mul(const Fq29& a, const Fq29& b) {
 #pragma unroll
     for (int i = k - L + 1; i < L; ++i) {
       acc += (uint64_t)a.l[i] * b.l[k - i];
-      acc += (uint64_t)m[i] * Consts::P[k - i];
+      acc += (uint64_t)m[i] * P::P[k - i];
     }
     t.l[k - L] = (uint32_t)acc & kMask;
     acc >>= kBits;
@@ -113,10 +125,15 @@ HP_HD Fq29 mul(const Fq29& a, const Fq29& b) {
   return t;
 }
 
-HP_HD Fq29 sqr(const Fq29& a) { return mul(a, a); }
+template <class P>
+HP_HD F29<P> sqr(const F29<P>& a) {
+  return mul(a, a);
+}
 
-HP_HD Fq29 add(const Fq29& a, const Fq29& b) {
-  Fq29 r;
+template <class P>
+HP_HD F29<P> add(const F29<P>& a, const F29<P>& b) {
+  constexpr int L = P::L;
+  F29<P> r;
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < L - 1; ++i) {
@@ -129,8 +146,10 @@ HP_HD Fq29 add(const Fq29& a, const Fq29& b) {
 }
 
 // a + b + b
-HP_HD Fq29 add_twice(const Fq29& a, const Fq29& b) {
-  Fq29 r;
+template <class P>
+HP_HD F29<P> add_twice(const F29<P>& a, const F29<P>& b) {
+  constexpr int L = P::L;
+  F29<P> r;
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < L - 1; ++i) {
@@ -143,8 +162,10 @@ HP_HD Fq29 add_twice(const Fq29& a, const Fq29& b) {
 }
 
 // a + k p - b for the constant limbs kp of k p >= b (signed carries)
-HP_HD Fq29 sub_kp(const Fq29& a, const Fq29& b, const uint32_t* kp) {
-  Fq29 r;
+template <class P>
+HP_HD F29<P> sub_kp(const F29<P>& a, const F29<P>& b, const uint32_t* kp) {
+  constexpr int L = P::L;
+  F29<P> r;
   int32_t c = 0;
 #pragma unroll
   for (int i = 0; i < L - 1; ++i) {
@@ -156,24 +177,90 @@ HP_HD Fq29 sub_kp(const Fq29& a, const Fq29& b, const uint32_t* kp) {
   return r;
 }
 
-HP_HD Fq29 sub2p(const Fq29& a, const Fq29& b) { return sub_kp(a, b, Consts::P2); }
-HP_HD Fq29 sub4p(const Fq29& a, const Fq29& b) { return sub_kp(a, b, Consts::P4); }
-HP_HD Fq29 sub8p(const Fq29& a, const Fq29& b) { return sub_kp(a, b, Consts::P8); }
-HP_HD Fq29 sub16p(const Fq29& a, const Fq29& b) { return sub_kp(a, b, Consts::P16); }
+template <class P>
+HP_HD F29<P> sub1p(const F29<P>& a, const F29<P>& b) {
+  return sub_kp(a, b, P::P);
+}
+template <class P>
+HP_HD F29<P> sub2p(const F29<P>& a, const F29<P>& b) {
+  return sub_kp(a, b, P::P2);
+}
+template <class P>
+HP_HD F29<P> sub4p(const F29<P>& a, const F29<P>& b) {
+  return sub_kp(a, b, P::P4);
+}
+template <class P>
+HP_HD F29<P> sub8p(const F29<P>& a, const F29<P>& b) {
+  return sub_kp(a, b, P::P8);
+}
+template <class P>
+HP_HD F29<P> sub16p(const F29<P>& a, const F29<P>& b) {
+  return sub_kp(a, b, P::P16);
+}
 // p - a for a canonical a (< p)
-HP_HD Fq29 neg_canonical(const Fq29& a) { return sub_kp(zero(), a, Consts::P); }
+template <class P>
+HP_HD F29<P> neg_canonical(const F29<P>& a) {
+  return sub_kp(zero_t<P>(), a, P::P);
+}
+
+// a - k p when a >= k p, else a
+template <class P>
+HP_HD F29<P> cond_sub(const F29<P>& a, const uint32_t* kp) {
+  constexpr int L = P::L;
+  F29<P> d;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const int32_t x = (int32_t)a.l[i] - (int32_t)kp[i] + c;
+    d.l[i] = i < L - 1 ? ((uint32_t)x & kMask) : (uint32_t)x;
+    c = i < L - 1 ? (x >> kBits) : (x >> 31);
+  }
+  return c == 0 ? d : a;
+}
+
+// a mod p in [0, p) for a < 32 p
+template <class P>
+HP_HD F29<P> canonical(const F29<P>& a) {
+  F29<P> v = cond_sub(a, P::P16);
+  v = cond_sub(v, P::P8);
+  v = cond_sub(v, P::P4);
+  v = cond_sub(v, P::P2);
+  return cond_sub(v, P::P);
+}
+
+// a - k p for k = floor(a / p) - 1 (estimated from the top 58 bits), clamped
+// at 0: a representative below 3 p of a value below 2^(29 L - 1)
+template <class P>
+HP_HD F29<P> reduce_est(const F29<P>& a) {
+  constexpr int L = P::L;
+  const uint64_t top = ((uint64_t)a.l[L - 1] << kBits) | a.l[L - 2];
+  const double q = (double)top * (1.0 / (double)P::kTop);
+  const uint32_t k = q >= 1.0 ? (uint32_t)q - 1u : 0u;
+  F29<P> r;
+  int64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < L - 1; ++i) {
+    const int64_t x = (int64_t)a.l[i] - (int64_t)k * P::P[i] + c;
+    r.l[i] = (uint32_t)x & kMask;
+    c = x >> kBits;
+  }
+  r.l[L - 1] = (uint32_t)((int64_t)a.l[L - 1] - (int64_t)k * P::P[L - 1] + c);
+  return r;
+}
 
 // a == k p exactly (limbs normalised, k < 64)
-HP_HD bool equals_kp(const Fq29& a, uint32_t k) {
+template <class P>
+HP_HD bool equals_kp(const F29<P>& a, uint32_t k) {
+  constexpr int L = P::L;
   uint64_t c = 0;
   uint32_t diff = 0;
 #pragma unroll
   for (int i = 0; i < L - 1; ++i) {
-    c += (uint64_t)k * Consts::P[i];
+    c += (uint64_t)k * P::P[i];
     diff |= ((uint32_t)c & kMask) ^ a.l[i];
     c >>= kBits;
   }
-  c += (uint64_t)k * Consts::P[L - 1];
+  c += (uint64_t)k * P::P[L - 1];
   diff |= (uint32_t)c ^ a.l[L - 1];
   return diff == 0;
 }
@@ -181,56 +268,68 @@ HP_HD bool equals_kp(const Fq29& a, uint32_t k) {
 // a = 0 (mod p) for a < 32 p: the only candidate multiple is k = round(a / p),
 // read off the top 58 bits; the low limb rules out all but ~2^-29 of the
 // non-multiples before the full comparison.
-HP_HD bool is_zero_mod(const Fq29& a) {
-  const uint64_t top = ((uint64_t)a.l[L - 1] << kBits) | a.l[L - 2];  // a >> 348
-  const uint32_t k = (uint32_t)((double)top * (1.0 / (double)Consts::kPTop) + 0.5);
-  if ((((uint64_t)k * Consts::P[0]) & kMask) != a.l[0]) return false;
+template <class P>
+HP_HD bool is_zero_mod(const F29<P>& a) {
+  constexpr int L = P::L;
+  const uint64_t top = ((uint64_t)a.l[L - 1] << kBits) | a.l[L - 2];
+  const uint32_t k = (uint32_t)((double)top * (1.0 / (double)P::kTop) + 0.5);
+  if ((((uint64_t)k * P::P[0]) & kMask) != a.l[0]) return false;
   return equals_kp(a, k);
 }
 
-// a mod p in [0, p) for a < 32 p (conversions only)
-HP_HD Fq29 canonical(const Fq29& a) {
-  Fq29 v = a;
-  const uint32_t* steps[5] = {Consts::P16, Consts::P8, Consts::P4, Consts::P2, Consts::P};
+// Packed value (P::Words 32-bit words, < 2^(32 Words)) <-> limbs.
+template <class P>
+HP_HD F29<P> unpack_t(const uint32_t* w) {
+  F29<P> r;
 #pragma unroll
-  for (int s = 0; s < 5; ++s) {
-    Fq29 d;
-    int32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-      const int32_t x = (int32_t)v.l[i] - (int32_t)steps[s][i] + c;
-      d.l[i] = i < L - 1 ? ((uint32_t)x & kMask) : (uint32_t)x;
-      c = i < L - 1 ? (x >> kBits) : (x >> 31);
-    }
-    if (c == 0) v = d;  // no borrow: v >= k p
-  }
-  return v;
-}
-
-// Packed canonical value (twelve 32-bit words, < 2^384) <-> limbs.
-HP_HD Fq29 unpack(const uint32_t* w) {
-  Fq29 r;
-#pragma unroll
-  for (int i = 0; i < L; ++i) {
+  for (int i = 0; i < P::L; ++i) {
     const int bit = (int)kBits * i, j = bit >> 5, sh = bit & 31;
     uint64_t x = w[j];
-    if (j + 1 < 12) x |= (uint64_t)w[j + 1] << 32;
+    if (j + 1 < P::Words) x |= (uint64_t)w[j + 1] << 32;
     r.l[i] = (uint32_t)(x >> sh) & kMask;
   }
   return r;
 }
 
-HP_HD void pack(const Fq29& a, uint32_t* w) {
+template <class P>
+HP_HD void pack_t(const F29<P>& a, uint32_t* w) {
 #pragma unroll
-  for (int j = 0; j < 12; ++j) w[j] = 0;
+  for (int j = 0; j < P::Words; ++j) w[j] = 0;
 #pragma unroll
-  for (int i = 0; i < L; ++i) {
+  for (int i = 0; i < P::L; ++i) {
     const int bit = (int)kBits * i, j = bit >> 5, sh = bit & 31;
     const uint64_t x = (uint64_t)a.l[i] << sh;
     w[j] |= (uint32_t)x;
-    if (j + 1 < 12) w[j + 1] |= (uint32_t)(x >> 32);
+    if (j + 1 < P::Words) w[j + 1] |= (uint32_t)(x >> 32);
   }
 }
+
+// ---- Fq29 names used by the MSM -------------------------------------------
+constexpr int L = FqP::L;
+
+// R mod p (Montgomery one) and the factors that move a value between
+// R = 2^384 (Fq) and R = 2^406 (Fq29) with one Fq product: 2^406 mod p, 2^362
+struct Consts {
+  static constexpr uint32_t ONE[L] = {0x3a9fb84,  0xba00690,  0x71288f1,  0xf59bcc5,  0x126cb614,
+                                      0x585bf36,  0x1b85ac3d, 0x1cf856fa, 0x1891ecbd, 0x1a7eec05,
+                                      0x155a88f0, 0x741ac6d,  0x1317c30f, 0x9};
+  static constexpr uint32_t TO29[12] = {0x3a9fb84,  0xc57400d2, 0x629c4a23, 0x6147acde,
+                                        0x7e6d26cb, 0x6b0f4b0b, 0xc2b7d6e1, 0x91ecbde7,
+                                        0x4fdd80b8, 0xd56a23c3, 0xf3a0d636, 0x13317c30};
+  static constexpr uint32_t FROM29[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0x400};
+};
+
+HP_HD Fq29 zero() { return zero_t<FqP>(); }
+
+HP_HD Fq29 one() {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < L; ++i) r.l[i] = Consts::ONE[i];
+  return r;
+}
+
+HP_HD Fq29 unpack(const uint32_t* w) { return unpack_t<FqP>(w); }
+HP_HD void pack(const Fq29& a, uint32_t* w) { pack_t(a, w); }
 
 // Fq (Montgomery, R = 2^384) -> packed canonical Fq29 value and back.
 HP_HD Fq to29_packed(const Fq& x) {
@@ -249,11 +348,41 @@ HP_HD Fq from29_packed(const Fq& packed) {
 }
 
 HP_HD Fq from29(const Fq29& a) {
-  Fq packed, c;
+  Fq packed;
   pack(canonical(a), packed.v);
+  return from29_packed(packed);  // x 2^406 * 2^362 / 2^384 = x R384
+}
+
+// ---- Fr29 names used by the sumcheck ----------------------------------------
+// 2^261 mod r as 32-bit words: the Fr (R = 2^256) Montgomery form of 2^5, so
+// c (R = 2^256) * K5 = c 2^261, the prescaled operand of Fr29 products.
+struct FrConsts {
+  static constexpr uint32_t K5[8] = {0xffffffba, 0x45,       0x72d846,   0x1a25272e,
+                                     0x5dbeee8b, 0xfe2eedcd, 0x9eefbe41, 0x4d043f42};
+};
+
+HP_HD Fr29 unpack_fr(const Fr& a) { return unpack_t<FrP>(a.v); }
+
+// canonical packed Fr (the stored form) from a value below 32 r
+HP_HD Fr pack_fr(const Fr29& a) {
+  Fr r;
+  pack_t(canonical(a), r.v);
+  return r;
+}
+
+// packed Fr of a value already below r
+HP_HD Fr pack_fr_canonical(const Fr29& a) {
+  Fr r;
+  pack_t(a, r.v);
+  return r;
+}
+
+// c (stored form, c 2^256) -> c 2^261 in Fr29 limbs
+HP_HD Fr29 prescale_fr(const Fr& c) {
+  Fr k;
 #pragma unroll
-  for (int i = 0; i < 12; ++i) c.v[i] = Consts::FROM29[i];
-  return packed * c;  // x 2^406 * 2^362 / 2^384 = x R384
+  for (int i = 0; i < 8; ++i) k.v[i] = FrConsts::K5[i];
+  return unpack_fr(c * k);
 }
 
 }  // namespace f29

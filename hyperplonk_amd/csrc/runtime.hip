@@ -60,15 +60,26 @@ size_t Allocator::round_up(size_t bytes) {
   return (bytes + granule - 1) / granule * granule;
 }
 
-void* Allocator::acquire(size_t bytes) {
+Allocator::Allocator(hipStream_t stream) : stream_(stream) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+    limit_ = std::min<size_t>(size_t(16) << 30, std::max<size_t>(size_t(1) << 30, total_b / 8));
+  else
+    (void)hipGetLastError();
+}
+
+void* Allocator::acquire(size_t bytes, size_t* capacity) {
   const size_t size = round_up(bytes);
-  auto it = free_.find(size);
-  if (it != free_.end() && !it->second.empty()) {
+  auto it = free_.lower_bound(size);
+  while (it != free_.end() && it->second.empty()) ++it;
+  if (it != free_.end() && it->first <= 2 * size) {
     void* p = it->second.back();
     it->second.pop_back();
-    cached_ -= size;
+    cached_ -= it->first;
+    *capacity = it->first;
     return p;
   }
+  if (cached_ > limit_) trim();
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, size);
   if (e != hipSuccess) {
@@ -80,14 +91,14 @@ void* Allocator::acquire(size_t bytes) {
       fail_oom();
     }
   }
+  *capacity = size;
   return p;
 }
 
-void Allocator::release(void* ptr, size_t bytes) {
+void Allocator::release(void* ptr, size_t capacity) {
   if (!ptr) return;
-  const size_t size = round_up(bytes);
-  free_[size].push_back(ptr);
-  cached_ += size;
+  free_[capacity].push_back(ptr);
+  cached_ += capacity;
 }
 
 void Allocator::trim() {
@@ -101,7 +112,7 @@ void Allocator::trim() {
 Allocator::~Allocator() { trim(); }
 
 Buffer::~Buffer() {
-  if (ctx && ptr && ctx->alloc) ctx->alloc->release(ptr, bytes);
+  if (ctx && ptr && ctx->alloc) ctx->alloc->release(ptr, capacity);
 }
 
 BufferRef Context::allocate(size_t bytes, size_t nv) {
@@ -109,7 +120,7 @@ BufferRef Context::allocate(size_t bytes, size_t nv) {
   b->ctx = this;
   b->bytes = bytes;
   b->num_vars = nv;
-  b->ptr = alloc->acquire(bytes == 0 ? 1 : bytes);
+  b->ptr = alloc->acquire(bytes == 0 ? 1 : bytes, &b->capacity);
   return b;
 }
 

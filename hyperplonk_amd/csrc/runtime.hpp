@@ -65,12 +65,18 @@ bool canonical_fr(const uint8_t* bytes);
 bool canonical_fq(const uint8_t* bytes);
 bool canonical_fr_array(const uint8_t* bytes, size_t count);
 
+// Caching device allocator.  Freed blocks are kept by size class and reused
+// best-fit (a cached block up to twice the request); the cache is bounded
+// (`limit`, an eighth of the device's memory, at most 16 GiB) so that blocks
+// of sizes that never recur (batch-dependent scratch) cannot fill the device
+// and starve the runtime's own allocations (kernel scratch, queues).
 class Allocator {
  public:
-  explicit Allocator(hipStream_t stream) : stream_(stream) {}
+  explicit Allocator(hipStream_t stream);
   ~Allocator();
-  void* acquire(size_t bytes);
-  void release(void* ptr, size_t bytes);
+  // returns a block of at least `bytes`; *capacity gets its size class
+  void* acquire(size_t bytes, size_t* capacity);
+  void release(void* ptr, size_t capacity);
   void trim();  // synchronises, then frees every cached block
   size_t cached_bytes() const { return cached_; }
 
@@ -79,6 +85,7 @@ class Allocator {
   hipStream_t stream_;
   std::map<size_t, std::vector<void*>> free_;
   size_t cached_ = 0;
+  size_t limit_ = size_t(4) << 30;
 };
 
 struct Context;
@@ -89,6 +96,7 @@ struct Buffer {
   Context* ctx = nullptr;
   void* ptr = nullptr;
   size_t bytes = 0;
+  size_t capacity = 0;  // size class of the block
   size_t num_vars = 0;  // for MLE tables
   ~Buffer();
 };

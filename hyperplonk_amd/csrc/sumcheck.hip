@@ -27,6 +27,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "field29.hpp"
 #include "kernels_common.hpp"
 
 namespace hp {
@@ -108,6 +109,7 @@ struct RoundJob {
   const Fr* const* src;    // tables read (inputs or the previous fold slot)
   Fr* const* dst;          // fold destination tables (folding rounds)
   const Fr* coeffs;        // product coefficients, Montgomery
+  const Fr* coeffs29;      // the same times 2^(5 m_p) for a product of m_p factors
   const uint32_t* offsets; // product_count + 1
   const uint32_t* slots;   // flattened factor slots
   const uint8_t* owner;    // per flattened entry: 1 = writes its factor's fold
@@ -128,32 +130,65 @@ struct JobPack {
 // factors: the first factor's line is scaled by the coefficient (2 products),
 // every further factor costs deg + 1 products and its line values are reached
 // by repeated addition of d_k.
+//
+// Fr29 arithmetic on the stored form (A = a 2^256 read as an integer): each
+// Fr29 product divides by 2^261, so a product of m factors starts from the
+// coefficient times 2^(256 + 5 m) (coeffs29) and ends in the stored form.
+// Bounds (r = the modulus; lo, hi canonical, t <= 8): d < 2r, lines < 17r, the
+// first factor's terms < 9.3r, every later product < 3.3r (R / r = 70.7).
+using f29::Fr29;
+
 template <int D>
 struct ProductTerms {
-  Fr term[D + 1];
-  __device__ __forceinline__ void first(const Fr& coeff, const Fr& lo, const Fr& hi, uint32_t deg) {
-    Fr line = coeff * lo;
-    const Fr d = coeff * (hi - lo);
+  Fr29 term[D + 1];
+  __device__ __forceinline__ void first(const Fr29& coeff, const Fr29& lo, const Fr29& hi,
+                                        uint32_t deg) {
+    Fr29 line = f29::mul(coeff, lo);
+    const Fr29 d = f29::mul(coeff, f29::sub1p(hi, lo));
 #pragma unroll
     for (int t = 0; t <= D; ++t) {
       if (t <= (int)deg) {
         term[t] = line;
-        line = line + d;
+        line = f29::add(line, d);
       }
     }
   }
-  __device__ __forceinline__ void times(const Fr& lo, const Fr& hi, uint32_t deg) {
-    Fr line = lo;
-    const Fr d = hi - lo;
+  __device__ __forceinline__ void times(const Fr29& lo, const Fr29& hi, uint32_t deg) {
+    Fr29 line = lo;
+    const Fr29 d = f29::sub1p(hi, lo);
 #pragma unroll
     for (int t = 0; t <= D; ++t) {
       if (t <= (int)deg) {
-        term[t] = term[t] * line;
-        line = line + d;
+        term[t] = f29::mul(term[t], line);
+        line = f29::add(line, d);
       }
     }
   }
 };
+
+// lo + c (hi - lo) for canonical lo, hi and c29 = c 2^261: canonical result
+__device__ __forceinline__ Fr29 fold29(const Fr29& c29, const Fr29& lo, const Fr29& hi) {
+  const Fr29 f = f29::add(lo, f29::mul(c29, f29::sub1p(hi, lo)));  // < 2.03 r
+  return f29::cond_sub(f29::cond_sub(f, f29::FrP::P), f29::FrP::P);
+}
+
+__device__ __forceinline__ Fr29 load29(const Fr* base, size_t i) {
+  return f29::unpack_fr(load_fr(base, i));
+}
+
+// accumulators stay below 3 r + 4 x 3.3 r < 35 r between reductions
+template <int D>
+__device__ __forceinline__ void reduce_acc(Fr29 (&acc)[D + 1], uint32_t deg) {
+#pragma unroll
+  for (int t = 0; t <= D; ++t)
+    if (t <= (int)deg) acc[t] = f29::reduce_est(acc[t]);
+}
+
+template <int D>
+__device__ __forceinline__ void to_stored(const Fr29 (&acc)[D + 1], Fr (&out)[D + 1]) {
+#pragma unroll
+  for (int t = 0; t <= D; ++t) out[t] = f29::pack_fr(acc[t]);
+}
 
 __device__ __forceinline__ uint32_t job_of_block(const JobPack& pack) {
   uint32_t j = 0;
@@ -191,35 +226,40 @@ __global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
   const uint32_t lblock = blockIdx.x - job.block_begin;
   const uint32_t deg = job.degree;
   const size_t stride = (size_t)job.blocks * kThreads;
-  Fr acc[D + 1];
+  Fr29 acc[D + 1];
 #pragma unroll
-  for (int t = 0; t <= D; ++t) acc[t] = Fr::zero();
+  for (int t = 0; t <= D; ++t) acc[t] = f29::zero_t<f29::FrP>();
+  const Fr29 c29 = f29::prescale_fr(job.challenge);
   const Fr* const* tbl = job.fold ? job.dst : job.src;
   for (size_t b = (size_t)lblock * kThreads + threadIdx.x; b < job.pairs; b += stride) {
     if (job.fold) {
       for (uint32_t f = 0; f < job.factors; ++f) {
         const Fr* o = job.src[f];
-        const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
-        const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
-        store_fr(job.dst[f], 2 * b, a0 + job.challenge * (a1 - a0));
-        store_fr(job.dst[f], 2 * b + 1, a2 + job.challenge * (a3 - a2));
+        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+        store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(fold29(c29, a0, a1)));
+        store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(fold29(c29, a2, a3)));
       }
     }
     for (uint32_t p = 0; p < job.products; ++p) {
       const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
       ProductTerms<D> pt;
       const Fr* t0 = tbl[job.slots[begin]];
-      pt.first(load_fr(job.coeffs, p), load_fr(t0, 2 * b), load_fr(t0, 2 * b + 1), deg);
+      pt.first(load29(job.coeffs29, p), load29(t0, 2 * b), load29(t0, 2 * b + 1), deg);
       for (uint32_t e = begin + 1; e < end; ++e) {
         const Fr* t = tbl[job.slots[e]];
-        pt.times(load_fr(t, 2 * b), load_fr(t, 2 * b + 1), deg);
+        pt.times(load29(t, 2 * b), load29(t, 2 * b + 1), deg);
       }
 #pragma unroll
       for (int t = 0; t <= D; ++t)
-        if (t <= (int)deg) acc[t] = acc[t] + pt.term[t];
+        if (t <= (int)deg) acc[t] = f29::add(acc[t], pt.term[t]);
+      if ((p & 3) == 3) reduce_acc<D>(acc, deg);
     }
+    reduce_acc<D>(acc, deg);
   }
-  finish_job<D>(job, lblock, acc);
+  Fr out[D + 1];
+  to_stored<D>(acc, out);
+  finish_job<D>(job, lblock, out);
 }
 
 // Thread per (product p, pair b), product-major, one item per thread: the
@@ -235,36 +275,39 @@ __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
   const size_t it = (size_t)lblock * kThreads + threadIdx.x;
   ProductTerms<D> pt;
 #pragma unroll
-  for (int t = 0; t <= D; ++t) pt.term[t] = Fr::zero();
+  for (int t = 0; t <= D; ++t) pt.term[t] = f29::zero_t<f29::FrP>();
   if (it < items) {
+    const Fr29 c29 = f29::prescale_fr(job.challenge);
     const uint32_t p = static_cast<uint32_t>(it / job.pairs);
     const size_t b = it - (size_t)p * job.pairs;
     const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
     for (uint32_t e = begin; e < end; ++e) {
       const uint32_t f = job.slots[e];
-      Fr lo, hi;
+      Fr29 lo, hi;
       if (job.fold) {
         const Fr* o = job.src[f];
-        const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
-        const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
-        lo = a0 + job.challenge * (a1 - a0);
-        hi = a2 + job.challenge * (a3 - a2);
+        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+        lo = fold29(c29, a0, a1);
+        hi = fold29(c29, a2, a3);
         if (job.owner[e]) {
-          store_fr(job.dst[f], 2 * b, lo);
-          store_fr(job.dst[f], 2 * b + 1, hi);
+          store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(lo));
+          store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(hi));
         }
       } else {
         const Fr* t = job.src[f];
-        lo = load_fr(t, 2 * b);
-        hi = load_fr(t, 2 * b + 1);
+        lo = load29(t, 2 * b);
+        hi = load29(t, 2 * b + 1);
       }
       if (e == begin)
-        pt.first(load_fr(job.coeffs, p), lo, hi, deg);
+        pt.first(load29(job.coeffs29, p), lo, hi, deg);
       else
         pt.times(lo, hi, deg);
     }
   }
-  finish_job<D>(job, lblock, pt.term);
+  Fr out[D + 1];
+  to_stored<D>(pt.term, out);
+  finish_job<D>(job, lblock, out);
 }
 
 // Any degree: message values for t in [t0, t0 + 8) only (host loops over t0).
@@ -338,6 +381,7 @@ struct State {
   rt::BufferRef slots;                // ping-pong fold slots: 2 per distinct factor
   rt::BufferRef meta;                 // coeffs | table pointers | offsets | slots | owner
   Products prods{};
+  const Fr* coeffs29 = nullptr;  // coefficients times 2^(5 m_p) (Fr29 product chains)
   const uint8_t* owner = nullptr;
   const Fr** table_ptrs[3] = {nullptr, nullptr, nullptr};  // inputs, slot A, slot B
   size_t distinct = 0, product_count = 0;
@@ -412,6 +456,7 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
   sc::RoundJob j{};
   round_tables(s, k, &j.src, &j.dst);
   j.coeffs = s.prods.coeffs;
+  j.coeffs29 = s.coeffs29;
   j.offsets = s.prods.offsets;
   j.slots = s.prods.slots;
   j.owner = s.owner;
@@ -743,20 +788,27 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
       s->slot_base[0] = static_cast<Fr*>(s->slots->ptr);
       s->slot_base[1] = s->slot_base[0] + F * a;
     }
-    // metadata block: coeffs (Fr) | table ptrs x3 (F each) | offsets | slots | owner
+    // metadata block: coeffs (Fr) | coeffs29 (Fr) | table ptrs x3 (F each) | offsets | slots |
+    // owner
     const size_t coeff_bytes = std::max<size_t>(product_count, 1) * rt::kFrBytes;
     const size_t ptr_bytes = 3 * std::max<size_t>(F, 1) * sizeof(void*);
     const size_t off_bytes = (product_count + 1) * sizeof(uint32_t);
     const size_t idx_bytes = std::max<size_t>(index_count, 1) * sizeof(uint32_t);
     const size_t own_bytes = std::max<size_t>(index_count, 1);
-    const size_t meta_bytes = coeff_bytes + ptr_bytes + off_bytes + idx_bytes + own_bytes;
+    const size_t cb = 2 * coeff_bytes;
+    const size_t meta_bytes = cb + ptr_bytes + off_bytes + idx_bytes + own_bytes;
     s->meta = c.allocate(meta_bytes);
     std::vector<uint8_t> host(meta_bytes, 0);
+    Fr k5;
+    std::memcpy(k5.v, f29::FrConsts::K5, sizeof k5.v);
     for (size_t p = 0; p < product_count; ++p) {
       Fr v;
       std::memcpy(v.v, coefficients + rt::kFrBytes * p, rt::kFrBytes);
       v = to_mont(v);
       std::memcpy(host.data() + rt::kFrBytes * p, v.v, rt::kFrBytes);
+      // times 2^(5 m) for the m Fr29 products of the product's chain
+      for (size_t e = product_offsets[p]; e < product_offsets[p + 1]; ++e) v = v * k5;
+      std::memcpy(host.data() + coeff_bytes + rt::kFrBytes * p, v.v, rt::kFrBytes);
     }
     uint8_t* dev = static_cast<uint8_t*>(s->meta->ptr);
     std::vector<const void*> ptrs(3 * std::max<size_t>(F, 1), nullptr);
@@ -768,13 +820,13 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
         ptrs[2 * F + f] = s->slot_base[1] + f * (a_len / 2);
       }
     }
-    std::memcpy(host.data() + coeff_bytes, ptrs.data(), ptr_bytes);
-    uint32_t* offs = reinterpret_cast<uint32_t*>(host.data() + coeff_bytes + ptr_bytes);
+    std::memcpy(host.data() + cb, ptrs.data(), ptr_bytes);
+    uint32_t* offs = reinterpret_cast<uint32_t*>(host.data() + cb + ptr_bytes);
     for (size_t p = 0; p <= product_count; ++p) offs[p] = static_cast<uint32_t>(product_offsets[p]);
     uint32_t* idx = offs + (product_count + 1);
     for (size_t i = 0; i < index_count; ++i) idx[i] = slot_of[factor_indices[i]];
     // the first occurrence of each factor among all product entries writes its fold
-    uint8_t* own = host.data() + coeff_bytes + ptr_bytes + off_bytes + idx_bytes;
+    uint8_t* own = host.data() + cb + ptr_bytes + off_bytes + idx_bytes;
     std::vector<uint8_t> taken(std::max<size_t>(F, 1), 0);
     for (size_t i = 0; i < index_count; ++i) {
       own[i] = taken[idx[i]] ? 0 : 1;
@@ -785,14 +837,15 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     rt::hip_check(hipMemcpyAsync(dev, stage, meta_bytes, hipMemcpyHostToDevice, c.stream));
     c.staging_used();
     s->prods.coeffs = reinterpret_cast<const Fr*>(dev);
-    s->prods.offsets = reinterpret_cast<const uint32_t*>(dev + coeff_bytes + ptr_bytes);
+    s->coeffs29 = reinterpret_cast<const Fr*>(dev + coeff_bytes);
+    s->prods.offsets = reinterpret_cast<const uint32_t*>(dev + cb + ptr_bytes);
     s->prods.slots = s->prods.offsets + (product_count + 1);
     s->prods.count = static_cast<uint32_t>(product_count);
-    const Fr** dptrs = reinterpret_cast<const Fr**>(dev + coeff_bytes);
+    const Fr** dptrs = reinterpret_cast<const Fr**>(dev + cb);
     s->table_ptrs[0] = dptrs;
     s->table_ptrs[1] = dptrs + F;
     s->table_ptrs[2] = dptrs + 2 * F;
-    s->owner = dev + coeff_bytes + ptr_bytes + off_bytes + idx_bytes;
+    s->owner = dev + cb + ptr_bytes + off_bytes + idx_bytes;
     const size_t width = max_degree + 1;
     unsigned max_blocks = message_blocks(size_t(1) << (num_vars - 1));
     for (size_t k = 0; k < num_vars; ++k) {

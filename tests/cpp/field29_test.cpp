@@ -8,7 +8,9 @@
 //   * is_zero_mod on every k p (k < 32) and on k p +- 1, canonical;
 //   * mixed and general XYZZ additions, doublings and every exceptional case
 //     (identity operands, P + P, P + (-P)) against curve.hpp, compared as
-//     affine points.
+//     affine points;
+//   * Fr29 on stored Fr values: prescaled products and product chains of lazy
+//     operands against the Fr product, reduce_est, canonical, is_zero_mod.
 // Prints "FIELD29 PASS" or the failing checks.
 #include <cstdio>
 #include <cstdint>
@@ -48,7 +50,7 @@ static Fq29 lazy(const Fq29& a, uint32_t k) {
   Fq29 kp = zero();
   uint64_t c = 0;
   for (int i = 0; i < L; ++i) {
-    c += (uint64_t)k * Consts::P[i] + a.l[i];
+    c += (uint64_t)k * FqP::P[i] + a.l[i];
     kp.l[i] = i < L - 1 ? ((uint32_t)c & kMask) : (uint32_t)c;
     c >>= kBits;
   }
@@ -93,7 +95,7 @@ int main() {
     CHECK(back(neg_canonical(a29)) == neg(a));
     // product bound: below p (1 + 2^-15) for inputs below 32 p
     const Fq29 m = mul(lazy(a29, 31), lazy(b29, 31));
-    CHECK(m.l[L - 1] <= Consts::P[L - 1]);
+    CHECK(m.l[L - 1] <= FqP::P[L - 1]);
   }
   for (uint32_t k = 0; k < 32; ++k) {
     const Fq29 z = lazy(zero(), k);
@@ -111,7 +113,7 @@ int main() {
       const Fq29 one_raw = lazy(zero(), 0);
       Fq29 o = one_raw;
       o.l[0] = 1;
-      CHECK(!is_zero_mod(sub_kp(z, o, Consts::P2)) || k == 0);  // k p - 1 + 2p
+      CHECK(!is_zero_mod(sub_kp(z, o, FqP::P2)) || k == 0);  // k p - 1 + 2p
     }
     const Fq29 c = canonical(lazy(to29(rand_fq()), k));
     CHECK(!is_zero_mod(c) || all_zero(c));
@@ -178,6 +180,63 @@ int main() {
     Xyzz29 nacc = acc;
     nacc.Y = sub4p(zero(), acc.Y);
     CHECK(f29::add(acc, nacc).is_identity() || ref.is_identity());
+  }
+  // ---- Fr29: prescaled operands keep products in the stored form
+  auto rand_fr = [] {
+    Fr x;
+    for (;;) {
+      for (int i = 0; i < 8; ++i) x.v[i] = (uint32_t)rng();
+      x.v[7] &= 0x7fffffffu;
+      if (!geq_modulus<FrParams>(x.v)) break;
+    }
+    return x;  // any canonical value is a stored (Montgomery) form
+  };
+  auto lazy_r = [](const Fr29& a, uint32_t k) {
+    Fr29 v;
+    uint64_t c = 0;
+    for (int i = 0; i < FrP::L; ++i) {
+      c += (uint64_t)k * FrP::P[i] + a.l[i];
+      v.l[i] = i < FrP::L - 1 ? ((uint32_t)c & kMask) : (uint32_t)c;
+      c >>= kBits;
+    }
+    return v;
+  };
+  auto same_fr = [](const Fr& a, const Fr& b) { return a == b; };
+  Fr k5;
+  for (int i = 0; i < 8; ++i) k5.v[i] = FrConsts::K5[i];
+  for (int it = 0; it < 2000; ++it) {
+    const Fr c = rand_fr(), a = rand_fr(), b = rand_fr(), d = rand_fr();
+    const uint32_t ka = (uint32_t)(rng() % 11), kb = (uint32_t)(rng() % 3);
+    // one product: c 2^261 times a lazy stored value
+    CHECK(same_fr(pack_fr(mul(prescale_fr(c), lazy_r(unpack_fr(a), ka))), c * a));
+    // a chain of three factors: coefficient prescaled by 2^(5 * 3)
+    const Fr c3 = ((c * k5) * k5) * k5;
+    Fr29 t = mul(unpack_fr(c3), lazy_r(unpack_fr(a), ka));
+    t = mul(lazy_r(t, kb), lazy_r(unpack_fr(b), ka));
+    t = mul(t, lazy_r(unpack_fr(d), kb));
+    CHECK(same_fr(pack_fr(t), ((c * a) * b) * d));
+    // the fold f = lo + c (hi - lo)
+    const Fr29 lo = unpack_fr(a), hi = unpack_fr(b);
+    const Fr29 f = add(lo, mul(prescale_fr(c), sub1p(hi, lo)));
+    CHECK(same_fr(pack_fr(f), a + c * (b - a)));
+    // reduce_est: a representative below 3 r of values below 2^260
+    const uint32_t kk = (uint32_t)(rng() % 34);
+    const Fr29 big = lazy_r(unpack_fr(a), kk);
+    const Fr29 red = reduce_est(big);
+    CHECK(same_fr(pack_fr(red), a));
+    CHECK(cond_sub(cond_sub(cond_sub(red, FrP::P2), FrP::P), FrP::P).l[FrP::L - 1] ==
+          canonical(red).l[FrP::L - 1]);
+    CHECK(!is_zero_mod(big) || all_zero(unpack_fr(a)));
+  }
+  for (uint32_t k = 0; k < 32; ++k) {
+    const Fr29 z = lazy_r(zero_t<FrP>(), k);
+    CHECK(is_zero_mod(z));
+    for (int limb = 1; limb < FrP::L; ++limb) {
+      Fr29 t = z;
+      t.l[limb] ^= 1u;
+      CHECK(!is_zero_mod(t));
+    }
+    CHECK(all_zero(canonical(z)));
   }
   std::printf("FIELD29 %s\n", failures ? "FAIL" : "PASS");
   return failures ? 1 : 0;
