@@ -245,15 +245,22 @@ __global__ void k_task_counts(const uint32_t* counts, uint32_t buckets, uint32_t
     ntask[b] = (counts[b] + cap - 1) / cap;
 }
 
+// One thread per task: its bucket is the last b with task_off[b] <= t (a
+// binary search), so a hot bucket's thousands of tasks are written in
+// parallel rather than by one thread.
 __global__ void k_build_tasks(const uint32_t* counts, const uint32_t* starts,
                               const uint32_t* owner, const uint32_t* task_off, uint32_t buckets,
-                              uint32_t cap, Task* tasks) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < buckets;
-       b += gridDim.x * blockDim.x) {
-    const uint32_t cnt = counts[b], start = starts[b], job = owner[b];
-    uint32_t t = task_off[b];
-    for (uint32_t done = 0; done < cnt; done += cap, ++t)
-      tasks[t] = {start + done, min(cap, cnt - done), b, job};
+                              uint32_t cap, const uint32_t* ntotal, Task* tasks) {
+  const uint32_t total = *ntotal;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = buckets;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (task_off[mid] <= t) lo = mid; else hi = mid;
+    }
+    const uint32_t done = (t - task_off[lo]) * cap;
+    tasks[t] = {starts[lo] + done, min(cap, counts[lo] - done), lo, owner[lo]};
   }
 }
 
@@ -984,8 +991,8 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   const size_t max_tasks = m_total_ / cap + G + 1;
   rt::BufferRef tasks_buf = c.allocate(max_tasks * sizeof(Task));
   Task* tasks = static_cast<Task*>(tasks_buf->ptr);
-  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, starts,
-                     owner, task_off, G, cap, tasks);
+  hipLaunchKernelGGL(k_build_tasks, dim3(grid_for(max_tasks)), dim3(256), 0, c.stream, counts,
+                     starts, owner, task_off, G, cap, totals + 1, tasks);
   rt::check_launch();
   rt::BufferRef partials_buf = c.allocate(max_tasks * sizeof(Xyzz29));
   Xyzz29* partials = static_cast<Xyzz29*>(partials_buf->ptr);

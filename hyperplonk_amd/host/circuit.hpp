@@ -4,11 +4,16 @@
 // Generator: SplitMix64, uniform canonical Fr by rejection of 255-bit draws
 // (four words, top word masked to 63 bits), the scheme of
 // backend/src/ascend910b3/native/tests/sumcheck_tensor_edge.cpp:62-80.
-// Circuit (vanilla: hyperplonk/examples/full_prove_bench.rs:470-500 with a
-// random permutation; jellyfish: hyperplonk/src/mock.rs:52-92):
-//   * a uniform permutation sigma of the #wires * 2^nv slots (Fisher-Yates);
-//   * every cycle of sigma holds one random non-zero value, so the copy
-//     constraints w[sigma(i)] = w[i] hold;
+// Circuit (vanilla: hyperplonk/examples/full_prove_bench.rs:470-535 with its
+// row-pair copies generalised to random pairs; jellyfish:
+// hyperplonk/src/mock.rs:52-92):
+//   * sigma pairs the #wires * 2^nv slots by a uniformly random perfect
+//     matching (a Fisher-Yates shuffle, consecutive shuffled slots swapped;
+//     an odd slot out stays fixed), so copies connect random rows of random
+//     columns;
+//   * every 2-cycle of sigma holds one random non-zero value, so the copy
+//     constraints w[sigma(i)] = w[i] hold and the witness has ~half as many
+//     distinct values as slots (as the reference's row pairs have);
 //   * every selector except `solved` is uniform; the solved one makes the
 //     row's gate equation zero (its monomial's witness product is non-zero);
 //   * the permutation oracle holds the labels sigma(i), slot i = column * 2^nv + row
@@ -85,10 +90,14 @@ inline MockCircuit mock_circuit(size_t nv, const CustomizedGates& gate, size_t s
   const size_t n = size_t(1) << nv;
   const size_t nw = gate.num_witness_columns(), ns = gate.num_selector_columns();
   const size_t slots = nw * n;
-  // random permutation and one value per cycle
-  std::vector<uint64_t> sigma(slots);
-  for (size_t i = 0; i < slots; ++i) sigma[i] = i;
-  for (size_t i = slots; i > 1; --i) std::swap(sigma[i - 1], sigma[rng.below(i)]);
+  // random perfect matching of the slots, one value per 2-cycle
+  std::vector<uint64_t> order(slots), sigma(slots);
+  for (size_t i = 0; i < slots; ++i) order[i] = sigma[i] = i;
+  for (size_t i = slots; i > 1; --i) std::swap(order[i - 1], order[rng.below(i)]);
+  for (size_t k = 0; k + 1 < slots; k += 2) {
+    sigma[order[k]] = order[k + 1];
+    sigma[order[k + 1]] = order[k];
+  }
   std::vector<std::vector<Fr>> wit(nw, std::vector<Fr>(n));
   std::vector<uint8_t> seen(slots, 0);
   for (size_t s = 0; s < slots; ++s) {

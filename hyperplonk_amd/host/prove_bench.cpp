@@ -15,8 +15,8 @@
 //             states in shared launches); outputs must be bit-identical
 //   serve     <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>
 //             a request trace (Poisson arrivals at `rate` per second; 0 = all at
-//             once; auto = 0.8 x the batched capacity measured on a closed-loop
-//             prefix) served with continuous batching and again serially
+//             once; auto = 0.8 x the serial capacity; both capacities measured
+//             on a closed-loop probe of the same request mix) served with continuous batching and again serially
 //             (FIFO, one proof at a time); latency = completion - arrival;
 //             every proof of the two runs must be identical
 //   srs       <nv> <path>
@@ -341,22 +341,34 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   cfg.progress_every = count >= 500 ? 100 : 0;
   double rate = 0;
   double capacity = 0;
-  if (rate_arg == "auto") {
-    // closed-loop prefix: everything at t = 0, batched
-    const size_t probe = std::min<size_t>(count, std::max<size_t>(4 * slots, 64));
-    const auto ptrace = serve::make_trace(probe, 0, nv_min, nv_max, jelly, seed ^ 0x70726f6265ULL);
-    const auto prep = serve::run_trace(be, pp, vk, ptrace, cfg);
+  // the request mix (arrival times do not change it) and the probe's mix share one payload pool
+  const size_t probe = std::min<size_t>(count, std::max<size_t>(8 * slots, 128));
+  const auto ptrace = serve::make_trace(probe, 0, nv_min, nv_max, jelly, seed ^ 0x70726f6265ULL);
+  auto trace = serve::make_trace(count, 1.0, nv_min, nv_max, jelly, seed);
+  std::vector<serve::Request> all = ptrace;
+  all.insert(all.end(), trace.begin(), trace.end());
+  t0 = Clock::now();
+  const serve::CircuitPool payloads(all, cfg.variants, cfg.gen_threads);
+  const double pool_ms = ms_since(t0);
+  // closed-loop probes (everything at t = 0): the capacity of each mode
+  double capacity_serial = 0;
+  {
+    const auto prep = serve::run_trace(be, pp, vk, ptrace, payloads, cfg);
     capacity = probe / prep.span_s;
-    rate = 0.8 * capacity;
-  } else {
-    rate = std::atof(rate_arg.c_str());
+    serve::ServeConfig serial = cfg;
+    serial.batched = false;
+    const auto srep = serve::run_trace(be, pp, vk, ptrace, payloads, serial);
+    capacity_serial = probe / srep.span_s;
   }
-  const auto trace = serve::make_trace(count, rate, nv_min, nv_max, jelly, seed);
+  // auto: a fixed rate the serial prover sustains (0.8 x its capacity), so
+  // both modes keep up and the latencies compare like for like
+  rate = rate_arg == "auto" ? 0.8 * capacity_serial : std::atof(rate_arg.c_str());
+  trace = serve::make_trace(count, rate, nv_min, nv_max, jelly, seed);
   serve::ServeConfig batched = cfg;
-  const auto brep = serve::run_trace(be, pp, vk, trace, batched);
+  const auto brep = serve::run_trace(be, pp, vk, trace, payloads, batched);
   serve::ServeConfig serial = cfg;
   serial.batched = false;
-  const auto srep = serve::run_trace(be, pp, vk, trace, serial);
+  const auto srep = serve::run_trace(be, pp, vk, trace, payloads, serial);
   size_t identical = 0;
   for (size_t i = 0; i < trace.size(); ++i)
     identical += brep.outcomes[i].error.empty() && brep.outcomes[i].proof == srep.outcomes[i].proof;
@@ -364,7 +376,7 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   size_t verified = 0, checked = 0;
   for (size_t i = 0; i < trace.size() && checked < 3; ++i) {
     if (trace[i].nv > 14) continue;
-    const auto c = serve::circuit_of(trace[i]);
+    const auto c = serve::circuit_of(trace[i], cfg.variants);
     auto [pk, vkey] = preprocess(be, c.index, pp, vk);
     verified += verify(vkey, c.public_inputs, prove(be, pk, c.public_inputs, c.witnesses));
     ++checked;
@@ -374,14 +386,18 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   for (const auto& r : trace) jelly_n += r.jellyfish;
   std::printf(
       "{\"mode\": \"serve\", \"requests\": %zu, \"jellyfish\": %zu, \"nv\": [%zu, %zu], \"rate_per_s\": %.3f, "
-      "\"capacity_probe_per_s\": %.3f, \"slots\": %zu, \"srs_num_vars\": %zu, \"srs_ms\": %.1f, "
+      "\"capacity_probe_per_s\": %.3f, \"capacity_serial_probe_per_s\": %.3f, \"probe_requests\": %zu, "
+      "\"payload_circuits\": %zu, "
+      "\"payload_pool_ms\": %.0f, \"slots\": %zu, \"srs_num_vars\": %zu, \"srs_ms\": %.1f, "
       "\"batched\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
       "\"errors\": %zu, \"max_concurrent\": %zu, \"ticks\": %zu, \"ops\": %zu, \"kzg_calls\": %zu, "
       "\"round_calls\": %zu, \"eval_calls\": %zu, \"kzg_held\": %zu, \"kzg_hold_us\": %ld, \"device_ms\": {\"gather\": %.1f, \"generic\": %.1f, "
       "\"kzg\": %.1f, \"rounds\": %.1f, \"evals\": %.1f}, \"request_wait_ms\": %.1f, \"request_total_ms\": %.1f}, "
       "\"serial\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
       "\"errors\": %zu}, \"identical_proofs\": %zu, \"verified\": %zu, \"verify_checked\": %zu}\n",
-      trace.size(), jelly_n, nv_min, nv_max, rate, capacity, slots, N, srs_ms, b.proofs_per_s, b.p50, b.p99,
+      trace.size(), jelly_n, nv_min, nv_max, rate, capacity, capacity_serial, probe, payloads.size(),
+      pool_ms, slots, N,
+      srs_ms, b.proofs_per_s, b.p50, b.p99,
       b.mean, b.errors, brep.max_concurrent, brep.stats.ticks, brep.stats.ops, brep.stats.kzg_calls,
       brep.stats.round_calls, brep.stats.eval_calls, brep.stats.kzg_held, (long)cfg.kzg_hold.count(),
       brep.stats.gather_ms, brep.stats.generic_ms,

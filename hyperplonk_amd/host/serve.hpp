@@ -30,6 +30,7 @@
 #include <optional>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "circuit.hpp"
@@ -375,18 +376,28 @@ inline std::vector<Request> make_trace(size_t count, double rate, size_t nv_min,
     r.jellyfish = u < jelly_frac;
     r.nv = nv_min + rng.below(nv_max - nv_min + 1);
     r.seed = rng.next();
-    if (rate > 0) {
-      const double e = ((rng.next() >> 11) + 1) * (1.0 / 9007199254740993.0);
-      t += -std::log(e) / rate;
-    }
+    // drawn for every rate, so the request mix does not depend on the rate
+    const double e = ((rng.next() >> 11) + 1) * (1.0 / 9007199254740993.0);
+    if (rate > 0) t += -std::log(e) / rate;
     r.arrival_s = t;
     out.push_back(r);
   }
   return out;
 }
 
-inline host::MockCircuit circuit_of(const Request& r) {
-  return r.jellyfish ? host::jellyfish_circuit(r.nv, r.seed) : host::vanilla_circuit(r.nv, r.seed);
+// Request payloads (witness + index) come from a pool of `variants` seeded
+// circuits per (gate, nv): request r carries variant r.seed % variants.  The
+// pool is built before any clock starts, so serving figures measure the
+// prover and not the synthesis of test inputs (an nv-20 mock circuit takes
+// seconds of host time to solve).
+inline uint64_t circuit_seed(const Request& r, size_t variants) {
+  const uint64_t v = r.seed % std::max<size_t>(variants, 1);
+  return ((uint64_t)r.nv << 40) ^ ((uint64_t)r.jellyfish << 32) ^ (v * 0x9E3779B97F4A7C15ULL) ^ 0x7061796cULL;
+}
+
+inline host::MockCircuit circuit_of(const Request& r, size_t variants) {
+  const uint64_t seed = circuit_seed(r, variants);
+  return r.jellyfish ? host::jellyfish_circuit(r.nv, seed) : host::vanilla_circuit(r.nv, seed);
 }
 
 // Device bytes a request may hold at its peak: its tables (witnesses,
@@ -394,67 +405,38 @@ inline host::MockCircuit circuit_of(const Request& r) {
 // the sumcheck fold slots and MSM working memory, ~64 tables of 2^nv Fr.
 inline size_t footprint_bytes(const Request& r) { return size_t(64) * 32 * (size_t(1) << r.nv); }
 
-// Circuits built ahead of admission by a pool of threads, at most `ahead`
-// requests past the one admitted last.
-class CircuitPipeline {
+class CircuitPool {
  public:
-  CircuitPipeline(const std::vector<Request>& trace, size_t threads, size_t ahead)
-      : trace_(trace), ahead_(ahead) {
-    for (size_t i = 0; i < threads; ++i) pool_.emplace_back([this] { work(); });
+  CircuitPool(const std::vector<Request>& trace, size_t variants, size_t threads)
+      : variants_(variants) {
+    std::vector<const Request*> todo;
+    for (const Request& r : trace)
+      if (map_.emplace(key(r), nullptr).second) todo.push_back(&r);
+    // largest first, so the slow nv-20 solves overlap
+    std::sort(todo.begin(), todo.end(), [](const Request* a, const Request* b) { return a->nv > b->nv; });
+    std::atomic<size_t> next{0};
+    std::mutex mu;
+    auto work = [&] {
+      for (size_t i; (i = next.fetch_add(1)) < todo.size();) {
+        auto c = std::make_shared<const host::MockCircuit>(circuit_of(*todo[i], variants_));
+        std::lock_guard<std::mutex> g(mu);
+        map_[key(*todo[i])] = std::move(c);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < std::max<size_t>(threads, 1); ++t) pool.emplace_back(work);
+    for (auto& t : pool) t.join();
   }
-  ~CircuitPipeline() {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto& t : pool_) t.join();
-  }
-  // block until the first k requests of the trace are built
-  void prefill(size_t k) {
-    std::unique_lock<std::mutex> lk(mu_);
-    cv_.wait(lk, [&] {
-      for (size_t i = 0; i < k; ++i)
-        if (!ready_.count(i)) return false;
-      return true;
-    });
-  }
-  std::shared_ptr<host::MockCircuit> take(size_t id) {
-    std::unique_lock<std::mutex> lk(mu_);
-    cv_.wait(lk, [&] { return ready_.count(id) > 0; });
-    auto c = ready_[id];
-    ready_.erase(id);
-    taken_ = std::max(taken_, id + 1);
-    cv_.notify_all();
-    return c;
-  }
+  std::shared_ptr<const host::MockCircuit> get(const Request& r) const { return map_.at(key(r)); }
+  size_t size() const { return map_.size(); }
+  size_t variants() const { return variants_; }
 
  private:
-  void work() {
-    for (;;) {
-      size_t id;
-      {
-        std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return stop_ || (next_ < trace_.size() && next_ < taken_ + ahead_); });
-        if (stop_) return;
-        id = next_++;
-      }
-      auto c = std::make_shared<host::MockCircuit>(circuit_of(trace_[id]));
-      {
-        std::lock_guard<std::mutex> g(mu_);
-        ready_[id] = std::move(c);
-      }
-      cv_.notify_all();
-    }
+  std::tuple<bool, size_t, uint64_t> key(const Request& r) const {
+    return {r.jellyfish, r.nv, r.seed % std::max<size_t>(variants_, 1)};
   }
-  const std::vector<Request>& trace_;
-  size_t ahead_;
-  std::mutex mu_;
-  std::condition_variable cv_;
-  std::map<size_t, std::shared_ptr<host::MockCircuit>> ready_;
-  size_t next_ = 0, taken_ = 0;
-  bool stop_ = false;
-  std::vector<std::thread> pool_;
+  size_t variants_;
+  std::map<std::tuple<bool, size_t, uint64_t>, std::shared_ptr<const host::MockCircuit>> map_;
 };
 
 struct Outcome {
@@ -466,8 +448,8 @@ struct Outcome {
 struct ServeConfig {
   size_t slots = 16;
   size_t hbm_budget = size_t(64) << 30;
-  size_t gen_threads = 8;
-  size_t ahead = 64;
+  size_t gen_threads = 8;  // threads that build the payload pool
+  size_t variants = 2;     // pooled circuits per (gate, nv)
   std::chrono::microseconds window{300};
   std::chrono::microseconds kzg_hold{3000};
   bool batched = true;  // false: serial FIFO, one request at a time, plain backend
@@ -486,12 +468,10 @@ struct ServeReport {
 // time on the plain backend.
 inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProverParam& pp,
                              const host::MultilinearVerifierParam& vk, const std::vector<Request>& trace,
-                             const ServeConfig& cfg) {
+                             const CircuitPool& payloads, const ServeConfig& cfg) {
   using Clock = std::chrono::steady_clock;
   ServeReport rep;
   rep.outcomes.resize(trace.size());
-  CircuitPipeline pipe(trace, cfg.gen_threads, cfg.ahead);
-  pipe.prefill(std::min(cfg.ahead, trace.size()));  // inputs of the first requests exist before the clock starts
   const auto t0 = Clock::now();
   auto since = [&](Clock::time_point t) { return std::chrono::duration<double>(t - t0).count(); };
   std::atomic<size_t> completed{0};
@@ -509,7 +489,7 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
   };
   if (!cfg.batched) {
     for (const Request& r : trace) {
-      auto c = pipe.take(r.id);
+      auto c = payloads.get(r);
       wait_arrival(r);
       Outcome& o = rep.outcomes[r.id];
       o.start_s = since(Clock::now());
@@ -529,10 +509,10 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
     std::condition_variable cv;
     size_t running = 0, used = 0;
     bool closed = false;
-    std::deque<std::pair<const Request*, std::shared_ptr<host::MockCircuit>>> admitted;
+    std::deque<std::pair<const Request*, std::shared_ptr<const host::MockCircuit>>> admitted;
     auto worker = [&] {
       for (;;) {
-        std::pair<const Request*, std::shared_ptr<host::MockCircuit>> job;
+        std::pair<const Request*, std::shared_ptr<const host::MockCircuit>> job;
         {
           std::unique_lock<std::mutex> lk(mu);
           cv.wait(lk, [&] { return closed || !admitted.empty(); });
@@ -563,7 +543,7 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
     std::vector<std::thread> pool;
     for (size_t i = 0; i < cfg.slots; ++i) pool.emplace_back(worker);
     for (const Request& r : trace) {
-      auto c = pipe.take(r.id);
+      auto c = payloads.get(r);
       wait_arrival(r);
       const size_t need = std::min(footprint_bytes(r), cfg.hbm_budget);
       std::unique_lock<std::mutex> lk(mu);
