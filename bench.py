@@ -48,7 +48,7 @@ FQM_PER_MIXED_ADD = 10
 # WRITE_SIZE passes calibrated on a 96 MiB device copy
 # (profiles/k_accumulate_msm_2_20_pmc.md); rocprofv3 counters cannot be read
 # from inside this process.
-ACCUMULATE_TRAFFIC_2_20 = 4.542e9
+ACCUMULATE_TRAFFIC_2_20 = 3.852e9
 
 
 def parse():

@@ -100,6 +100,26 @@ HP_HD Xyzz29 add(const Xyzz29& p, const Xyzz29& q) {
   return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
 }
 
+// a^(p - 2) (zero maps to zero); the one inversion per thread of the batch
+// normalisations
+HP_HD Fq29 inverse(const Fq29& a) {
+  uint32_t exp[12];  // p - 2
+  uint32_t borrow = 2;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    const uint64_t d = (uint64_t)FqParams::MOD[i] - borrow;
+    exp[i] = (uint32_t)d;
+    borrow = (uint32_t)(d >> 63);
+  }
+  Fq29 acc = one();
+  for (int i = 11; i >= 0; --i)
+    for (int b = 31; b >= 0; --b) {
+      acc = sqr(acc);
+      if ((exp[i] >> b) & 1u) acc = mul(acc, a);
+    }
+  return acc;
+}
+
 // Conversions to the 32-bit-limb XYZZ form (identity: ZZ = ZZZ = 0).
 HP_HD G1Xyzz to_xyzz(const Xyzz29& p) {
   if (p.is_identity()) return G1Xyzz::identity();

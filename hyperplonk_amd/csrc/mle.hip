@@ -10,6 +10,7 @@
 #include <cstring>
 #include <vector>
 
+#include "field29.hpp"
 #include "kernels_common.hpp"
 #include "mle.hpp"
 
@@ -208,7 +209,7 @@ __global__ void k_frac(const Fr* const* nums, const Fr* const* dens, size_t coun
   }
   if (saw_zero) atomicOr(zero, 1u);
   if (tid >= n) return;
-  Fr inv = inverse(acc);
+  Fr inv = f29::inverse_fr(acc);
   // pass 2 (backwards): inv_j = inv * prefix_j; inv *= d_j
   size_t last = tid + ((n - 1 - tid) / threads) * threads;
   for (size_t j = last;; j -= threads) {

@@ -456,27 +456,31 @@ __global__ void __launch_bounds__(kHeavyThreads)
   }
 }
 
-// Level 1: blocks of kRed1 = 256 buckets.  U_t = sum_{t' >= t} S_t' (suffix scan),
+// Level 1: blocks of G buckets.  U_t = sum_{t' >= t} S_t' (suffix scan),
 // A = sum_t U_t = sum_t (t + 1) S_t, R = U_0; written in the 32-bit-limb form
 // for the host.
+// Bucket ranges are padded to kRed1 = 256; small batches reduce in groups of
+// 64 (12 dependent additions instead of 16, more groups for the host).
 constexpr unsigned kRed1 = 256;
-__global__ void __launch_bounds__(kRed1) k_reduce_buckets(const Xyzz29* sums, uint32_t buckets,
-                                                          G1Xyzz* pair_a, G1Xyzz* pair_r) {
-  __shared__ Xyzz29 u[kRed1];
+constexpr unsigned kRedSmall = 64;
+template <unsigned G>
+__global__ void __launch_bounds__(G) k_reduce_buckets(const Xyzz29* sums, uint32_t buckets,
+                                                      G1Xyzz* pair_a, G1Xyzz* pair_r) {
+  __shared__ Xyzz29 u[G];
   const uint32_t t = threadIdx.x;
-  const uint32_t b = blockIdx.x * kRed1 + t;
+  const uint32_t b = blockIdx.x * G + t;
   Xyzz29 v = b < buckets ? load_xyzz29(sums, b) : Xyzz29::identity();
   u[t] = v;
   __syncthreads();
-  for (unsigned d = 1; d < kRed1; d <<= 1) {
-    if (t + d < kRed1) v = f29::add(v, u[t + d]);
+  for (unsigned d = 1; d < G; d <<= 1) {
+    if (t + d < G) v = f29::add(v, u[t + d]);
     __syncthreads();
     u[t] = v;
     __syncthreads();
   }
   if (t == 0) store_xyzz(pair_r, blockIdx.x, f29::to_xyzz(v));
   // tree sum of U
-  for (unsigned d = kRed1 / 2; d > 0; d >>= 1) {
+  for (unsigned d = G / 2; d > 0; d >>= 1) {
     if (t < d) {
       v = f29::add(v, u[t + d]);
       u[t] = v;
@@ -511,32 +515,39 @@ __global__ void k_bases_from29(const G1Affine* pts, G1Affine* out, size_t n) {
 }
 
 // ---- SRS preparation --------------------------------------------------------
-// next[i] = 2^c prev[i], normalised with one inversion per thread
-// (Montgomery batch inversion over the thread's strided points).
-__global__ void k_shift_window(const G1Affine* prev, G1Affine* next, size_t n, uint32_t c,
-                               G1Xyzz* scratch, Fq* prefix) {
+// next[i] = 2^c prev[i] on packed Fq29 coordinates: c XYZZ doublings on
+// 14 x 29-bit limbs, normalised with one inversion per thread (Montgomery
+// batch inversion over the thread's strided points).
+__global__ void k_shift_window29(const G1Affine* prev, G1Affine* next, size_t n, uint32_t c,
+                                 Xyzz29* scratch, Fq29* prefix) {
   const size_t threads = (size_t)gridDim.x * blockDim.x;
   const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (tid >= n) return;
-  Fq acc = Fq::one();
+  Fq29 acc = f29::one();
   for (size_t i = tid; i < n; i += threads) {
-    const G1Affine p = load_affine(prev, i);
-    G1Xyzz q = p.is_identity() ? G1Xyzz::identity() : dbl_affine(p);
-    for (uint32_t s = 1; s < c; ++s) q = dbl(q);
-    store_xyzz(scratch, i, q);
-    store_fq(&prefix[i], acc);
-    if (!q.is_identity()) acc = acc * (q.ZZ * q.ZZZ);
+    const G1Affine raw = load_affine(prev, i);
+    Xyzz29 q = Xyzz29::identity();
+    if (!raw.is_identity()) {
+      q = f29::dbl_affine(f29::Aff29{f29::unpack(raw.x.v), f29::unpack(raw.y.v)});
+      for (uint32_t s = 1; s < c; ++s) q = f29::dbl(q);
+    }
+    store_xyzz29(scratch, i, q);
+    prefix[i] = acc;
+    if (!q.is_identity()) acc = f29::mul(acc, f29::mul(q.ZZ, q.ZZZ));
   }
-  Fq inv = inverse(acc);
+  Fq29 inv = f29::inverse(acc);
   const size_t last = tid + ((n - 1 - tid) / threads) * threads;
   for (size_t i = last;; i -= threads) {
-    const G1Xyzz q = load_xyzz(scratch, i);
+    const Xyzz29 q = load_xyzz29(scratch, i);
     if (q.is_identity()) {
       store_affine(next, i, G1Affine::identity());
     } else {
-      const Fq d_inv = inv * load_fq(&prefix[i]);  // 1 / (ZZ ZZZ)
-      inv = inv * (q.ZZ * q.ZZZ);
-      store_affine(next, i, {q.X * (d_inv * q.ZZZ), q.Y * (d_inv * q.ZZ)});
+      const Fq29 d_inv = f29::mul(inv, prefix[i]);  // 1 / (ZZ ZZZ)
+      inv = f29::mul(inv, f29::mul(q.ZZ, q.ZZZ));
+      G1Affine out;
+      f29::pack(f29::canonical(f29::mul(q.X, f29::mul(d_inv, q.ZZZ))), out.x.v);
+      f29::pack(f29::canonical(f29::mul(q.Y, f29::mul(d_inv, q.ZZ))), out.y.v);
+      store_affine(next, i, out);
     }
     if (i < threads) break;
   }
@@ -811,20 +822,21 @@ void prepare_layer_from_device(rt::Context& c, Layer& layer, const G1Affine* poi
   G1Affine* bases = static_cast<G1Affine*>(layer.bases->ptr);
   rt::hip_check(hipMemcpyAsync(bases, points, n * sizeof(G1Affine), hipMemcpyDeviceToDevice,
                                c.stream));
+  // the bucket stages read packed Fq29 coordinates: window 0 is converted,
+  // the shifted copies are built from it in that form
+  hipLaunchKernelGGL(k_bases_to29, dim3(grid_for(n)), dim3(256), 0, c.stream, bases, n);
+  rt::check_launch();
   if (W > 1) {
     const size_t threads = std::max<size_t>(1, std::min<size_t>(n, 65536));
-    rt::BufferRef scratch = c.allocate(n * sizeof(G1Xyzz));
-    rt::BufferRef prefix = c.allocate(n * sizeof(Fq));
+    rt::BufferRef scratch = c.allocate(n * sizeof(Xyzz29));
+    rt::BufferRef prefix = c.allocate(n * sizeof(Fq29));
     for (uint32_t w = 1; w < W; ++w) {
-      hipLaunchKernelGGL(k_shift_window, dim3(grid_for(threads, 64)), dim3(64), 0, c.stream,
+      hipLaunchKernelGGL(k_shift_window29, dim3(grid_for(threads, 64)), dim3(64), 0, c.stream,
                          bases + (w - 1) * n, bases + w * n, n, layer.plan.c,
-                         static_cast<G1Xyzz*>(scratch->ptr), static_cast<Fq*>(prefix->ptr));
+                         static_cast<Xyzz29*>(scratch->ptr), static_cast<Fq29*>(prefix->ptr));
       rt::check_launch();
     }
   }
-  // the bucket stages read packed Fq29 coordinates
-  hipLaunchKernelGGL(k_bases_to29, dim3(grid_for(W * n)), dim3(256), 0, c.stream, bases, W * n);
-  rt::check_launch();
 }
 
 namespace {
@@ -903,7 +915,9 @@ void Batch::add(const Layer& layer, const Fr* scalars) {
   entries_.push_back(e);
 }
 
-size_t Batch::result_slots() const { return 2 * (size_t)(gbuckets_ / kRed1); }
+// level-1 group size: 64 buckets for batches of up to 16384 buckets
+unsigned Batch::red_group() const { return gbuckets_ <= 16384 ? kRedSmall : kRed1; }
+size_t Batch::result_slots() const { return 2 * (size_t)(gbuckets_ / red_group()); }
 
 void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   auto mark = [&](int i) {
@@ -1026,11 +1040,16 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
                      partials, heavy, heavy_count, sums);
   rt::check_launch();
   keep.push_back(std::move(heavy_parts));
-  // level 1 of the weighted bucket reduction: (A_k, R_k) per kRed1 buckets
-  const size_t groups = G / kRed1;
+  // level 1 of the weighted bucket reduction: (A_k, R_k) per red_group() buckets
+  const unsigned rg = red_group();
+  const size_t groups = G / rg;
   mark(5);
-  hipLaunchKernelGGL(k_reduce_buckets, dim3((unsigned)groups), dim3(kRed1), 0, c.stream, sums, G,
-                     result, result + groups);
+  if (rg == kRedSmall)
+    hipLaunchKernelGGL(k_reduce_buckets<kRedSmall>, dim3((unsigned)groups), dim3(kRedSmall), 0,
+                       c.stream, sums, G, result, result + groups);
+  else
+    hipLaunchKernelGGL(k_reduce_buckets<kRed1>, dim3((unsigned)groups), dim3(kRed1), 0, c.stream,
+                       sums, G, result, result + groups);
   rt::check_launch();
   if (ev) {
     mark(6);
@@ -1040,7 +1059,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   }
 }
 
-// Host end of every MSM of the batch: MSM j is sum_k A_k + kRed1 sum_k k R_k
+// Host end of every MSM of the batch: MSM j is sum_k A_k + g sum_k k R_k (g the group size)
 // over its level-1 groups k.  The groups are cut into chunks of kFinishChunk
 // (c covers [c L, c L + L)), summed on the worker pool, then combined per MSM:
 //   sum_k k R_k = sum_c (W_c - S_c) + L sum_c c S_c,
@@ -1062,7 +1081,7 @@ ChunkSums chunk_sums(const G1Xyzz* A, const G1Xyzz* R, size_t lo, size_t hi) {
   return c;
 }
 
-host::Xyzz64 combine(const ChunkSums* cs, size_t n) {
+host::Xyzz64 combine(const ChunkSums* cs, size_t n, unsigned group) {
   host::Xyzz64 a{}, t1{}, run{}, t2{};
   for (size_t c = 0; c < n; ++c) {
     a = host::add(a, cs[c].a);
@@ -1074,13 +1093,14 @@ host::Xyzz64 combine(const ChunkSums* cs, size_t n) {
   }
   for (size_t m = 1; m < kFinishChunk; m <<= 1) t2 = host::dbl(t2);
   host::Xyzz64 acc = host::add(t1, t2);
-  for (unsigned m = 1; m < kRed1; m <<= 1) acc = host::dbl(acc);
+  for (unsigned m = 1; m < group; m <<= 1) acc = host::dbl(acc);
   return host::add(a, acc);
 }
 }  // namespace
 
 void Batch::finish_packets(const G1Xyzz* slots, uint8_t* out97) const {
-  const size_t all = gbuckets_ / kRed1, njobs = entries_.size();
+  const unsigned rg = red_group();
+  const size_t all = gbuckets_ / rg, njobs = entries_.size();
   struct Item {
     size_t job, lo, hi;
   };
@@ -1088,18 +1108,18 @@ void Batch::finish_packets(const G1Xyzz* slots, uint8_t* out97) const {
   std::vector<size_t> first(njobs + 1);
   for (size_t j = 0; j < njobs; ++j) {
     first[j] = items.size();
-    const size_t groups = (entries_[j].layer->plan.buckets + kRed1 - 1) / kRed1;
+    const size_t groups = (entries_[j].layer->plan.buckets + kRed1 - 1) / kRed1 * (kRed1 / rg);
     for (size_t lo = 0; lo < groups; lo += kFinishChunk)
       items.push_back({j, lo, std::min(groups, lo + kFinishChunk)});
   }
   first[njobs] = items.size();
   std::vector<ChunkSums> sums(items.size());
   rt::parallel_for(items.size(), [&](size_t i) {
-    const size_t g0 = entries_[items[i].job].bucket_base / kRed1;
+    const size_t g0 = entries_[items[i].job].bucket_base / rg;
     sums[i] = chunk_sums(slots + g0, slots + all + g0, items[i].lo, items[i].hi);
   });
   rt::parallel_for(njobs, [&](size_t j) {
-    host::to_packet(combine(sums.data() + first[j], first[j + 1] - first[j]),
+    host::to_packet(combine(sums.data() + first[j], first[j + 1] - first[j], rg),
                     out97 + rt::kG1Bytes * j);
   });
 }

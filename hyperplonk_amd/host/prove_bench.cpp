@@ -16,7 +16,8 @@
 //   serve     <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>
 //             a request trace (Poisson arrivals at `rate` per second; 0 = all at
 //             once; auto = 0.8 x the serial capacity; both capacities measured
-//             on a closed-loop probe of the same request mix) served with continuous batching and again serially
+//             on a closed-loop probe of the same request mix), then again at a
+//             high load (0.9 x the batched capacity, above the serial one) served with continuous batching and again serially
 //             (FIFO, one proof at a time); latency = completion - arrival;
 //             every proof of the two runs must be identical
 //   srs       <nv> <path>
@@ -382,6 +383,17 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
     ++checked;
   }
   const Summary b = summarize(trace, brep), s = summarize(trace, srep);
+  // high load: 0.9 x the batched capacity, above what the serial prover sustains
+  const double rate_hi = 0.9 * capacity;
+  const auto trace_hi = serve::make_trace(count, rate_hi, nv_min, nv_max, jelly, seed);
+  const auto brep_hi = serve::run_trace(be, pp, vk, trace_hi, payloads, batched);
+  const auto srep_hi = serve::run_trace(be, pp, vk, trace_hi, payloads, serial);
+  size_t identical_hi = 0;
+  for (size_t i = 0; i < trace_hi.size(); ++i)
+    identical_hi += brep_hi.outcomes[i].error.empty() &&
+                    brep_hi.outcomes[i].proof == srep_hi.outcomes[i].proof &&
+                    brep_hi.outcomes[i].proof == brep.outcomes[i].proof;
+  const Summary bh = summarize(trace_hi, brep_hi), sh = summarize(trace_hi, srep_hi);
   size_t jelly_n = 0;
   for (const auto& r : trace) jelly_n += r.jellyfish;
   std::printf(
@@ -394,7 +406,11 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
       "\"round_calls\": %zu, \"eval_calls\": %zu, \"kzg_held\": %zu, \"kzg_hold_us\": %ld, \"device_ms\": {\"gather\": %.1f, \"generic\": %.1f, "
       "\"kzg\": %.1f, \"rounds\": %.1f, \"evals\": %.1f}, \"request_wait_ms\": %.1f, \"request_total_ms\": %.1f}, "
       "\"serial\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, "
-      "\"errors\": %zu}, \"identical_proofs\": %zu, \"verified\": %zu, \"verify_checked\": %zu}\n",
+      "\"errors\": %zu}, \"identical_proofs\": %zu, "
+      "\"high_load\": {\"rate_per_s\": %.3f, \"batched\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, "
+      "\"p99_ms\": %.2f, \"mean_ms\": %.2f, \"errors\": %zu}, \"serial\": {\"proofs_per_s\": %.3f, "
+      "\"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, \"errors\": %zu}, \"identical_proofs\": %zu}, "
+      "\"verified\": %zu, \"verify_checked\": %zu}\n",
       trace.size(), jelly_n, nv_min, nv_max, rate, capacity, capacity_serial, probe, payloads.size(),
       pool_ms, slots, N,
       srs_ms, b.proofs_per_s, b.p50, b.p99,
@@ -402,8 +418,9 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
       brep.stats.round_calls, brep.stats.eval_calls, brep.stats.kzg_held, (long)cfg.kzg_hold.count(),
       brep.stats.gather_ms, brep.stats.generic_ms,
       brep.stats.kzg_ms, brep.stats.round_ms, brep.stats.eval_ms, brep.stats.waiting_ms, busy_ms(brep), s.proofs_per_s, s.p50, s.p99, s.mean, s.errors, identical,
-      verified, checked);
-  return identical == trace.size() && verified == checked ? 0 : 1;
+      rate_hi, bh.proofs_per_s, bh.p50, bh.p99, bh.mean, bh.errors, sh.proofs_per_s, sh.p50, sh.p99,
+      sh.mean, sh.errors, identical_hi, verified, checked);
+  return identical == trace.size() && identical_hi == trace_hi.size() && verified == checked ? 0 : 1;
 }
 
 static int run_srs(size_t nv, const std::string& path) {

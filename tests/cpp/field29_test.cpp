@@ -10,7 +10,8 @@
 //     (identity operands, P + P, P + (-P)) against curve.hpp, compared as
 //     affine points;
 //   * Fr29 on stored Fr values: prescaled products and product chains of lazy
-//     operands against the Fr product, reduce_est, canonical, is_zero_mod.
+//     operands against the Fr product, reduce_est, canonical, is_zero_mod,
+//     inverse_fr against the 32-bit-limb inverse.
 // Prints "FIELD29 PASS" or the failing checks.
 #include <cstdio>
 #include <cstdint>
@@ -96,6 +97,10 @@ int main() {
     // product bound: below p (1 + 2^-15) for inputs below 32 p
     const Fq29 m = mul(lazy(a29, 31), lazy(b29, 31));
     CHECK(m.l[L - 1] <= FqP::P[L - 1]);
+  }
+  for (int it = 0; it < 100; ++it) {
+    const Fq a = rand_fq();
+    CHECK(back(f29::inverse(lazy(to29(a), (uint32_t)(rng() % 8)))) == inverse(a));
   }
   for (uint32_t k = 0; k < 32; ++k) {
     const Fq29 z = lazy(zero(), k);
@@ -228,6 +233,13 @@ int main() {
           canonical(red).l[FrP::L - 1]);
     CHECK(!is_zero_mod(big) || all_zero(unpack_fr(a)));
   }
+  for (int it = 0; it < 200; ++it) {
+    const Fr a = rand_fr();
+    const Fr inv = inverse_fr(a);
+    CHECK(inv == inverse(a));
+    CHECK(a.is_zero() || a * inv == Fr::one());
+  }
+  CHECK(inverse_fr(Fr::zero()).is_zero());
   for (uint32_t k = 0; k < 32; ++k) {
     const Fr29 z = lazy_r(zero_t<FrP>(), k);
     CHECK(is_zero_mod(z));
