@@ -209,11 +209,22 @@ __device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x <= deg) {
-    Fr s = Fr::zero();
-    for (uint32_t blk = 0; blk < job.blocks; ++blk)
-      s = s + load_fr(job.partial, (size_t)blk * (deg + 1) + threadIdx.x);
-    store_fr(job.out, threadIdx.x, from_mont(s));
+  // every thread of the last block sums a strided share of the block partials
+  // of one point t (groups of deg + 1 threads), then one thread per t adds the
+  // groups: independent loads instead of one chain over all blocks
+  const uint32_t width = deg + 1, groups = kThreads / width;
+  const uint32_t t = threadIdx.x % width, g = threadIdx.x / width;
+  Fr s = Fr::zero();
+  if (g < groups)
+    for (uint32_t blk = g; blk < job.blocks; blk += groups)
+      s = s + load_fr(job.partial, (size_t)blk * width + t);
+  __shared__ Fr red[kThreads];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < width) {
+    Fr total = Fr::zero();
+    for (uint32_t k = 0; k < groups; ++k) total = total + red[k * width + threadIdx.x];
+    store_fr(job.out, threadIdx.x, from_mont(total));
   }
   if (threadIdx.x == 0) *job.ticket = 0;
 }
@@ -307,6 +318,60 @@ __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
   }
   Fr out[D + 1];
   to_stored<D>(pt.term, out);
+  finish_job<D>(job, lblock, out);
+}
+
+// Thread per (product p, pair b, point t), t fastest, for the smallest
+// rounds: each thread folds p's factors on the pair (the t = 0 thread of a
+// factor's first occurrence stores the fold) and evaluates p at its one point,
+// a chain of 3 m products for m factors instead of (deg + 1)(m - 1) + 2 + 2 m.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
+  const RoundJob& job = pack.job[job_of_block(pack)];
+  const uint32_t lblock = blockIdx.x - job.block_begin;
+  const uint32_t deg = job.degree, width = deg + 1;
+  const size_t items = (size_t)job.pairs * job.products * width;
+  const size_t it = (size_t)lblock * kThreads + threadIdx.x;
+  Fr out[D + 1];
+#pragma unroll
+  for (int t = 0; t <= D; ++t) out[t] = Fr::zero();
+  if (it < items) {
+    const uint32_t t = static_cast<uint32_t>(it % width);
+    const size_t pb = it / width;
+    const uint32_t p = static_cast<uint32_t>(pb / job.pairs);
+    const size_t b = pb - (size_t)p * job.pairs;
+    const Fr29 c29 = f29::prescale_fr(job.challenge);
+    const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
+    Fr29 term = load29(job.coeffs29, p);
+    for (uint32_t e = begin; e < end; ++e) {
+      const uint32_t f = job.slots[e];
+      Fr29 lo, hi;
+      if (job.fold) {
+        const Fr* o = job.src[f];
+        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+        lo = fold29(c29, a0, a1);
+        hi = fold29(c29, a2, a3);
+        if (t == 0 && job.owner[e]) {
+          store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(lo));
+          store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(hi));
+        }
+      } else {
+        const Fr* tb = job.src[f];
+        lo = load29(tb, 2 * b);
+        hi = load29(tb, 2 * b + 1);
+      }
+      // lo + t (hi - lo) < 17 r
+      const Fr29 d = f29::sub1p(hi, lo);
+      Fr29 line = lo;
+      for (uint32_t k = 0; k < t; ++k) line = f29::add(line, d);
+      term = f29::mul(term, line);
+    }
+    const Fr v = f29::pack_fr(term);
+#pragma unroll
+    for (int k = 0; k <= D; ++k)
+      if ((uint32_t)k == t) out[k] = v;
+  }
   finish_job<D>(job, lblock, out);
 }
 
@@ -408,7 +473,17 @@ unsigned message_blocks(size_t pairs) {
   return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, sc::kMaxBlocks)));
 }
 
-// tables at or below this many pairs per round use the per-(product, pair) decomposition
+// tables at or below this many pairs per round use the per-(product, pair)
+// decomposition; at or below points_below() (product, pair) items, the
+// per-(product, pair, point) one
+size_t points_below() {
+  static const size_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_POINTS_BELOW");
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(1) << 14;
+  }();
+  return v;
+}
+
 size_t items_below() {
   static const size_t v = [] {
     const char* e = std::getenv("HP_MI355X_SC_ITEMS_BELOW");
@@ -417,14 +492,19 @@ size_t items_below() {
   return v;
 }
 
-// grid blocks of round k of a state (same rule at create time and at run time)
-unsigned job_blocks(const sc::State& s, size_t k, bool* items) {
+// grid blocks and decomposition (0 pairs, 1 items, 2 points) of round k of a
+// state (same rule at create time and at run time)
+unsigned job_blocks(const sc::State& s, size_t k, uint32_t* mode) {
   const size_t pairs = size_t(1) << (s.num_vars - k - 1);
-  *items = s.product_count == 0 || pairs < items_below();
-  if (!*items) return message_blocks(pairs);
-  // one item per thread
-  return static_cast<unsigned>(std::max<size_t>(
-      1, (pairs * s.product_count + sc::kThreads - 1) / sc::kThreads));
+  if (s.product_count != 0 && pairs >= items_below()) {
+    *mode = 0;
+    return message_blocks(pairs);
+  }
+  const size_t items = pairs * s.product_count;
+  const size_t width = s.max_degree + 1;
+  *mode = s.product_count != 0 && items <= points_below() && s.max_degree <= kMaxJobDegree ? 2 : 1;
+  const size_t threads = *mode == 2 ? items * width : items;
+  return static_cast<unsigned>(std::max<size_t>(1, (threads + sc::kThreads - 1) / sc::kThreads));
 }
 
 // Generic path (degree > kMaxJobDegree): message for t in chunks of 8 values.
@@ -469,15 +549,15 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
   j.factors = static_cast<uint32_t>(s.distinct);
   j.degree = static_cast<uint32_t>(s.max_degree);
   j.fold = (k > 0 && s.product_count > 0) ? 1 : 0;
-  bool items = false;
-  j.blocks = job_blocks(s, k, &items);
-  j.items_mode = items ? 1 : 0;
+  uint32_t mode = 0;
+  j.blocks = job_blocks(s, k, &mode);
+  j.items_mode = mode;
   return j;
 }
 
 struct PackLaunch {
   uint32_t degree;
-  bool items;
+  uint32_t mode;  // 0 pairs, 1 items, 2 points
   unsigned grid;
   sc::JobPack pack;
 };
@@ -497,7 +577,7 @@ std::vector<PackLaunch> plan_packs(const std::vector<sc::RoundJob>& jobs) {
     const auto k = key(order[i]);
     PackLaunch l{};
     l.degree = k.first;
-    l.items = k.second != 0;
+    l.mode = k.second;
     while (i < order.size() && key(order[i]) == k && l.pack.count < (uint32_t)sc::kJobsPerLaunch) {
       sc::RoundJob j = jobs[order[i]];
       j.block_begin = l.grid;
@@ -511,21 +591,22 @@ std::vector<PackLaunch> plan_packs(const std::vector<sc::RoundJob>& jobs) {
 }
 
 template <int D>
-const void* kernel_for(bool items) {
-  return items ? reinterpret_cast<const void*>(&sc::k_round_items<D>)
-               : reinterpret_cast<const void*>(&sc::k_round_pairs<D>);
+const void* kernel_for(uint32_t mode) {
+  return mode == 2   ? reinterpret_cast<const void*>(&sc::k_round_points<D>)
+         : mode == 1 ? reinterpret_cast<const void*>(&sc::k_round_items<D>)
+                     : reinterpret_cast<const void*>(&sc::k_round_pairs<D>);
 }
 
-const void* kernel_of(uint32_t d, bool items) {
+const void* kernel_of(uint32_t d, uint32_t mode) {
   switch (d) {
-    case 1: return kernel_for<1>(items);
-    case 2: return kernel_for<2>(items);
-    case 3: return kernel_for<3>(items);
-    case 4: return kernel_for<4>(items);
-    case 5: return kernel_for<5>(items);
-    case 6: return kernel_for<6>(items);
-    case 7: return kernel_for<7>(items);
-    case 8: return kernel_for<8>(items);
+    case 1: return kernel_for<1>(mode);
+    case 2: return kernel_for<2>(mode);
+    case 3: return kernel_for<3>(mode);
+    case 4: return kernel_for<4>(mode);
+    case 5: return kernel_for<5>(mode);
+    case 6: return kernel_for<6>(mode);
+    case 7: return kernel_for<7>(mode);
+    case 8: return kernel_for<8>(mode);
     default: rt::fail_state(-3);
   }
 }
@@ -533,7 +614,7 @@ const void* kernel_of(uint32_t d, bool items) {
 void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
   for (PackLaunch& l : plan_packs(jobs)) {
     void* args[] = {&l.pack};
-    rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.items), dim3(l.grid), dim3(sc::kThreads), args,
+    rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.mode), dim3(l.grid), dim3(sc::kThreads), args,
                                   0, c.stream));
   }
 }
@@ -588,7 +669,7 @@ RoundGraphCache& graph_cache(rt::Context& c) {
 
 hipKernelNodeParams node_params(PackLaunch& l, void** args) {
   hipKernelNodeParams p{};
-  p.func = const_cast<void*>(kernel_of(l.degree, l.items));
+  p.func = const_cast<void*>(kernel_of(l.degree, l.mode));
   p.gridDim = dim3(l.grid);
   p.blockDim = dim3(sc::kThreads);
   p.sharedMemBytes = 0;
@@ -607,7 +688,7 @@ void run_jobs_graph(rt::Context& c, std::vector<sc::RoundJob>& jobs,
     std::vector<PackLaunch> shape = plan_packs(jobs);
     for (const PackLaunch& l : shape) {
       key.push_back(l.degree);
-      key.push_back(l.items ? 1u : 0u);
+      key.push_back(l.mode);
       key.push_back(l.grid);
       key.push_back(l.pack.count);
       for (uint32_t j = 0; j < l.pack.count; ++j) key.push_back(l.pack.job[j].degree);
@@ -849,8 +930,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     const size_t width = max_degree + 1;
     unsigned max_blocks = message_blocks(size_t(1) << (num_vars - 1));
     for (size_t k = 0; k < num_vars; ++k) {
-      bool items = false;
-      max_blocks = std::max(max_blocks, job_blocks(*s, k, &items));
+      uint32_t mode = 0;
+      max_blocks = std::max(max_blocks, job_blocks(*s, k, &mode));
     }
     s->partial = c.allocate(size_t(max_blocks) * width * rt::kFrBytes);
     s->result = c.allocate(width * rt::kFrBytes);
