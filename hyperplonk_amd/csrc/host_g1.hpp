@@ -72,28 +72,59 @@ inline Fq64 sub(const Fq64& a, const Fq64& b) {
 
 // CIOS Montgomery product in the "no final carry" form: q's top limb is below
 // 2^63 - 1, so every intermediate fits in six words (the carries A and C of
-// the product and reduction rows never overflow a word).
+// the product and reduction rows never overflow a word).  Rows written out so
+// the compiler keeps t in registers (about half the time of the looped form).
 inline Fq64 mul(const Fq64& a, const Fq64& b) {
-  uint64_t t[6] = {0};
-  for (int i = 0; i < 6; ++i) {
-    unsigned __int128 x = (unsigned __int128)a.v[0] * b.v[i] + t[0];
-    uint64_t A = (uint64_t)(x >> 64);
-    t[0] = (uint64_t)x;
-    const uint64_t m = t[0] * kQInv;
-    unsigned __int128 y = (unsigned __int128)m * kQ[0] + t[0];
-    uint64_t C = (uint64_t)(y >> 64);
-    for (int j = 1; j < 6; ++j) {
-      x = (unsigned __int128)a.v[j] * b.v[i] + t[j] + A;
-      A = (uint64_t)(x >> 64);
-      y = (unsigned __int128)m * kQ[j] + (uint64_t)x + C;
-      C = (uint64_t)(y >> 64);
-      t[j - 1] = (uint64_t)y;
-    }
-    t[5] = C + A;
+  using u128 = unsigned __int128;
+  uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0;
+#define HP_CIOS_ROW(i)                                                               \
+  {                                                                                  \
+    const uint64_t bi = b.v[i];                                                      \
+    u128 x = (u128)a.v[0] * bi + t0;                                                 \
+    uint64_t A = (uint64_t)(x >> 64);                                                \
+    const uint64_t m = (uint64_t)x * kQInv;                                          \
+    u128 y = (u128)m * kQ[0] + (uint64_t)x;                                          \
+    uint64_t C = (uint64_t)(y >> 64);                                                \
+    x = (u128)a.v[1] * bi + t1 + A;                                                  \
+    A = (uint64_t)(x >> 64);                                                         \
+    y = (u128)m * kQ[1] + (uint64_t)x + C;                                           \
+    C = (uint64_t)(y >> 64);                                                         \
+    t0 = (uint64_t)y;                                                                \
+    x = (u128)a.v[2] * bi + t2 + A;                                                  \
+    A = (uint64_t)(x >> 64);                                                         \
+    y = (u128)m * kQ[2] + (uint64_t)x + C;                                           \
+    C = (uint64_t)(y >> 64);                                                         \
+    t1 = (uint64_t)y;                                                                \
+    x = (u128)a.v[3] * bi + t3 + A;                                                  \
+    A = (uint64_t)(x >> 64);                                                         \
+    y = (u128)m * kQ[3] + (uint64_t)x + C;                                           \
+    C = (uint64_t)(y >> 64);                                                         \
+    t2 = (uint64_t)y;                                                                \
+    x = (u128)a.v[4] * bi + t4 + A;                                                  \
+    A = (uint64_t)(x >> 64);                                                         \
+    y = (u128)m * kQ[4] + (uint64_t)x + C;                                           \
+    C = (uint64_t)(y >> 64);                                                         \
+    t3 = (uint64_t)y;                                                                \
+    x = (u128)a.v[5] * bi + t5 + A;                                                  \
+    A = (uint64_t)(x >> 64);                                                         \
+    y = (u128)m * kQ[5] + (uint64_t)x + C;                                           \
+    C = (uint64_t)(y >> 64);                                                         \
+    t4 = (uint64_t)y;                                                                \
+    t5 = C + A;                                                                      \
   }
+  HP_CIOS_ROW(0) HP_CIOS_ROW(1) HP_CIOS_ROW(2) HP_CIOS_ROW(3) HP_CIOS_ROW(4) HP_CIOS_ROW(5)
+#undef HP_CIOS_ROW
+  // t < 2q: one conditional subtraction, branch-free
+  const uint64_t t[6] = {t0, t1, t2, t3, t4, t5};
+  uint64_t d[6], borrow = 0;
+  for (int i = 0; i < 6; ++i) {
+    const u128 z = (u128)t[i] - kQ[i] - borrow;
+    d[i] = (uint64_t)z;
+    borrow = (uint64_t)(z >> 64) & 1;
+  }
+  const uint64_t keep = 0 - borrow;  // borrow: t < q, keep t
   Fq64 r;
-  std::memcpy(r.v, t, sizeof r.v);
-  cond_sub_q(r.v);
+  for (int i = 0; i < 6; ++i) r.v[i] = (t[i] & keep) | (d[i] & ~keep);
   return r;
 }
 
