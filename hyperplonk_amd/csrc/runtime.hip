@@ -7,6 +7,7 @@
 // freed while work may still read them, and context_destroy drains the stream
 // before returning memory to the driver.
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 #include <condition_variable>
 #include <deque>
