@@ -66,11 +66,11 @@ kzg::Params* bind(HpMi355xContext* ctx, rt::Context& c, const HpMi355xKzgParams*
 // Run a batch of MSMs and write one 97-byte packet per MSM.
 void run_batch(rt::Context& c, const msm::Batch& batch, uint8_t* out97) {
   const size_t slots = batch.result_slots();
-  rt::BufferRef res = c.allocate(std::max<size_t>(slots, 1) * sizeof(G1Xyzz));
-  batch.enqueue(c, static_cast<G1Xyzz*>(res->ptr));
-  std::vector<G1Xyzz> host(slots);
-  rt::download_bytes(c, res->ptr, slots * sizeof(G1Xyzz), reinterpret_cast<uint8_t*>(host.data()));
-  batch.finish_packets(host.data(), out97);
+  // the level-1 reduction writes its points straight into pinned host memory
+  G1Xyzz* host = reinterpret_cast<G1Xyzz*>(c.results(std::max<size_t>(slots, 1) * sizeof(G1Xyzz)));
+  batch.enqueue(c, host);
+  c.sync();
+  batch.finish_packets(host, out97);
 }
 
 }  // namespace

@@ -487,7 +487,10 @@ __global__ void __launch_bounds__(G) k_reduce_buckets(const Xyzz29* sums, uint32
     }
     __syncthreads();
   }
-  if (t == 0) store_xyzz(pair_a, blockIdx.x, f29::to_xyzz(v));
+  if (t == 0) {
+    store_xyzz(pair_a, blockIdx.x, f29::to_xyzz(v));
+    __threadfence_system();  // the results may be pinned host memory read after the sync
+  }
 }
 
 // Prepared bases: 32-bit-limb Montgomery -> packed Fq29 (in place) and back.

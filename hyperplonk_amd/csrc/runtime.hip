@@ -152,6 +152,19 @@ uint8_t* Context::staging(size_t bytes) {
   return pinned;
 }
 
+uint8_t* Context::results(size_t bytes) {
+  if (bytes > results_bytes) {
+    hip_check(hipStreamSynchronize(stream));  // nothing still writes the old block
+    if (results_host) hip_check(hipHostFree(results_host));
+    results_host = nullptr;
+    results_bytes = 0;
+    const size_t want = std::max(bytes, size_t(64) << 10);
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&results_host), want, hipHostMallocDefault));
+    results_bytes = want;
+  }
+  return results_host;
+}
+
 void Context::staging_used() {
   if (!pinned_free) hip_check(hipEventCreateWithFlags(&pinned_free, hipEventDisableTiming));
   hip_check(hipEventRecord(pinned_free, stream));
@@ -361,6 +374,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext* ctx) {
   c.registry.clear();
   c.alloc.reset();  // frees every cached block (after the drain)
   if (c.pinned) (void)hipHostFree(c.pinned);
+  if (c.results_host) (void)hipHostFree(c.results_host);
   if (c.pinned_free) (void)hipEventDestroy(c.pinned_free);
   (void)hipStreamDestroy(c.stream);
   delete ctx;

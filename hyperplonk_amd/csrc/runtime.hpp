@@ -129,6 +129,12 @@ struct Context {
   // Pinned host bytes of at least `bytes`, after previous users completed.
   uint8_t* staging(size_t bytes);
   void staging_used();  // record that queued work reads/writes staging
+  // Pinned host memory the device writes small results into directly (no
+  // copy back): valid after the stream is synchronised; reused by the next
+  // call, so callers copy out before returning.
+  uint8_t* results(size_t bytes);
+  uint8_t* results_host = nullptr;
+  size_t results_bytes = 0;
   void sync();
   void poison(int32_t code) {
     if (!poisoned) first_error = code ? code : -1;

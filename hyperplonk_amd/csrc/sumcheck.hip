@@ -225,6 +225,7 @@ __device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1
     Fr total = Fr::zero();
     for (uint32_t k = 0; k < groups; ++k) total = total + red[k * width + threadIdx.x];
     store_fr(job.out, threadIdx.x, from_mont(total));
+    __threadfence_system();  // `out` may be pinned host memory read after the sync
   }
   if (threadIdx.x == 0) *job.ticket = 0;
 }
@@ -968,14 +969,18 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
         return rt::ok();
       }
       if (s.max_degree <= kMaxJobDegree) {
-        std::vector<sc::RoundJob> jobs{make_job(s, ch, static_cast<Fr*>(s.result->ptr))};
+        // the last block writes the message straight into pinned host memory
+        Fr* host = reinterpret_cast<Fr*>(c.results(width * rt::kFrBytes));
+        std::vector<sc::RoundJob> jobs{make_job(s, ch, host)};
         launch_jobs(c, jobs);
+        c.sync();
+        std::memcpy(out_evaluations, host, width * rt::kFrBytes);
       } else {
         generic_round(c, s, ch);
+        std::vector<uint8_t> out(width * rt::kFrBytes);
+        rt::download_bytes(c, s.result->ptr, out.size(), out.data());
+        std::memcpy(out_evaluations, out.data(), out.size());
       }
-      std::vector<uint8_t> out(width * rt::kFrBytes);
-      rt::download_bytes(c, s.result->ptr, out.size(), out.data());
-      std::memcpy(out_evaluations, out.data(), out.size());
       s.round += 1;
       return rt::ok();
     } catch (const rt::Failure& f) {
@@ -1032,8 +1037,9 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
         }
         return rt::ok();
       }
-      rt::BufferRef outs = c.allocate(total * rt::kFrBytes);
-      Fr* out_base = static_cast<Fr*>(outs->ptr);
+      // messages land in pinned host memory: the job kernels' last blocks
+      // write them there, the generic rounds copy them there
+      Fr* out_base = reinterpret_cast<Fr*>(c.results(total * rt::kFrBytes));
       std::vector<sc::RoundJob> jobs;
       for (size_t i = 0; i < count; ++i) {
         sc::State& s = *ss[i];
@@ -1043,16 +1049,14 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
         } else {
           generic_round(c, s, ch);
           rt::hip_check(hipMemcpyAsync(out_base + offset[i], s.result->ptr,
-                                       (s.max_degree + 1) * rt::kFrBytes,
-                                       hipMemcpyDeviceToDevice, c.stream));
+                                       (s.max_degree + 1) * rt::kFrBytes, hipMemcpyDefault,
+                                       c.stream));
         }
       }
       if (!jobs.empty()) launch_jobs(c, jobs);
-      std::vector<uint8_t> host(total * rt::kFrBytes);
-      rt::download_bytes(c, out_base, host.size(), host.data());
+      c.sync();
       for (size_t i = 0; i < count; ++i) {
-        std::memcpy(out_evaluations[i], host.data() + offset[i] * rt::kFrBytes,
-                    (ss[i]->max_degree + 1) * rt::kFrBytes);
+        std::memcpy(out_evaluations[i], out_base + offset[i], (ss[i]->max_degree + 1) * rt::kFrBytes);
         ss[i]->round += 1;
       }
       return rt::ok();
