@@ -176,11 +176,21 @@ __global__ void __launch_bounds__(kThreads) k_eval_dots(const DotJob* jobs, uint
     last = atomicAdd(job.ticket, 1u) == job.blocks - 1;
   }
   __syncthreads();
-  if (!last || threadIdx.x != 0) return;
+  if (!last) return;
   __threadfence();
+  // the last block sums the block partials with all its threads
   Fr s = Fr::zero();
-  for (uint32_t b = 0; b < job.blocks; ++b) s = s + load_fr(job.partial, b);
-  store_fr(job.out, 0, from_mont(s));
+  for (uint32_t b = threadIdx.x; b < job.blocks; b += kThreads) s = s + load_fr(job.partial, b);
+  s = wave_sum_fr(s);
+  __syncthreads();  // lds reused
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Fr total = lds[0];
+    for (unsigned w = 1; w < kThreads / 64; ++w) total = total + lds[w];
+    store_fr(job.out, 0, from_mont(total));
+    __threadfence_system();  // `out` is pinned host memory read after the sync
+  }
 }
 
 __global__ void k_zero_words(unsigned* p, size_t n) {
@@ -506,13 +516,14 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(
     }
     rt::BufferRef partial = c.allocate(size_t(grid_blocks) * rt::kFrBytes);
     rt::BufferRef tickets = c.allocate(count * sizeof(unsigned));
-    rt::BufferRef results = c.allocate(count * rt::kFrBytes);
+    // the values land in pinned host memory
+    Fr* results = reinterpret_cast<Fr*>(c.results(count * rt::kFrBytes));
     for (size_t i = 0; i < count; ++i) {
       jobs[i].table = static_cast<const Fr*>(tables[i]->ptr);
       jobs[i].eq = eqs[point_of[i]] ? static_cast<const Fr*>(eqs[point_of[i]]->ptr) : nullptr;
       jobs[i].partial = static_cast<Fr*>(partial->ptr) + jobs[i].block_begin;
       jobs[i].ticket = static_cast<unsigned*>(tickets->ptr) + i;
-      jobs[i].out = static_cast<Fr*>(results->ptr) + i;
+      jobs[i].out = results + i;
     }
     rt::BufferRef djobs = c.allocate(count * sizeof(DotJob));
     uint8_t* stage = c.staging(count * sizeof(DotJob));
@@ -526,7 +537,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(
     hipLaunchKernelGGL(k_eval_dots, dim3(grid_blocks), dim3(kThreads), 0, c.stream,
                        static_cast<const DotJob*>(djobs->ptr), static_cast<uint32_t>(count));
     rt::check_launch();
-    rt::download_bytes(c, results->ptr, count * rt::kFrBytes, out_evaluations);
+    c.sync();
+    std::memcpy(out_evaluations, results, count * rt::kFrBytes);
     return rt::ok();
   });
 }

@@ -184,6 +184,7 @@ __global__ void k_fr_from_mont(const Fr* src, Fr* dst, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     store_fr(dst, i, from_mont(load_fr(src, i)));
+  __threadfence_system();  // dst may be pinned host memory read after the sync
 }
 
 unsigned grid_for(size_t n) {
@@ -234,6 +235,16 @@ void download_bytes(Context& c, const void* src, size_t bytes, uint8_t* host) {
 
 void download_fr(Context& c, const Fr* src, size_t count, uint8_t* host) {
   if (count == 0) return;
+  if (count * kFrBytes <= kStagingChunk) {
+    // canonical values written straight into pinned host memory
+    uint8_t* out = c.results(count * kFrBytes);
+    hipLaunchKernelGGL(k_fr_from_mont, dim3(grid_for(count)), dim3(256), 0, c.stream, src,
+                       reinterpret_cast<Fr*>(out), count);
+    check_launch();
+    c.sync();
+    std::memcpy(host, out, count * kFrBytes);
+    return;
+  }
   BufferRef tmp = c.allocate(count * kFrBytes);
   hipLaunchKernelGGL(k_fr_from_mont, dim3(grid_for(count)), dim3(256), 0, c.stream, src,
                      static_cast<Fr*>(tmp->ptr), count);
