@@ -6,13 +6,15 @@
 // which fixes the multiple of p each subtraction adds:
 //   madd-2008-s:  P = U2 + 16p - X1 (< 18p), R = S2 + 4p - Y1 (< 6p),
 //                 X3 = R^2 + 8p - (PPP + 2Q) (< 10p),
-//                 Y3 = R (Q + 16p - X3) + 2p - Y1 PPP (< 4p);
-//   add-2008-s:   P = U2 + 2p - U1, R = S2 + 2p - S1 (< 4p), X3, Y3 as above;
+//                 Y3 = R (Q + 16p - X3) + (4p - Y1) PPP;
+//   add-2008-s:   P = U2 + 2p - U1, R = S2 + 2p - S1 (< 4p), X3 as above,
+//                 Y3 = R (Q + 16p - X3) + (2p - S1) PPP;
 //   dbl-2008-s-1: M = 3 X^2 (< 6p), X3 = M^2 + 4p - 2S (< 6p),
-//                 Y3 = M (S + 16p - X3) + 2p - W Y (< 4p).
-// Products stay below p (1 + 2^-15) for inputs below 32 p.  Exceptional
-// cases (identity operands, P == Q, P == -Q) are exact, tested with
-// is_zero_mod, so the group element does not depend on the order of
+//                 Y3 = M (S + 16p - X3) + (4p - Y) W.
+// Products stay below p (1 + 2^-15) for inputs below 32 p; each Y3 is one
+// sum of two products with a single reduction (mul_sum), below p (1 + 2^-18).
+// Exceptional cases (identity operands, P == Q, P == -Q) are exact, tested
+// with is_zero_mod, so the group element does not depend on the order of
 // accumulation.
 #pragma once
 #include "curve.hpp"
@@ -42,7 +44,7 @@ HP_HD Xyzz29 dbl_affine(const Aff29& a) {
   const Fq29 xx = sqr(a.x);
   const Fq29 M = add_twice(xx, xx);
   const Fq29 X3 = sub4p(sqr(M), add(S, S));
-  const Fq29 Y3 = sub2p(mul(M, sub16p(S, X3)), mul(W, a.y));
+  const Fq29 Y3 = mul_sum(M, sub16p(S, X3), sub2p(zero(), a.y), W);
   return {X3, Y3, V, W};
 }
 
@@ -55,7 +57,7 @@ HP_HD Xyzz29 dbl(const Xyzz29& p) {
   const Fq29 xx = sqr(p.X);
   const Fq29 M = add_twice(xx, xx);
   const Fq29 X3 = sub4p(sqr(M), add(S, S));
-  const Fq29 Y3 = sub2p(mul(M, sub16p(S, X3)), mul(W, p.Y));
+  const Fq29 Y3 = mul_sum(M, sub16p(S, X3), sub4p(zero(), p.Y), W);
   return {X3, Y3, mul(V, p.ZZ), mul(W, p.ZZZ)};
 }
 
@@ -74,7 +76,7 @@ HP_HD Xyzz29 add_mixed(const Xyzz29& p, const Aff29& q) {
   const Fq29 PPP = mul(P, PP);
   const Fq29 Qv = mul(p.X, PP);
   const Fq29 X3 = sub8p(sqr(R), add_twice(PPP, Qv));
-  const Fq29 Y3 = sub2p(mul(R, sub16p(Qv, X3)), mul(p.Y, PPP));
+  const Fq29 Y3 = mul_sum(R, sub16p(Qv, X3), sub4p(zero(), p.Y), PPP);
   return {X3, Y3, mul(p.ZZ, PP), mul(p.ZZZ, PPP)};
 }
 
@@ -96,7 +98,7 @@ HP_HD Xyzz29 add(const Xyzz29& p, const Xyzz29& q) {
   const Fq29 PPP = mul(P, PP);
   const Fq29 Qv = mul(U1, PP);
   const Fq29 X3 = sub8p(sqr(R), add_twice(PPP, Qv));
-  const Fq29 Y3 = sub2p(mul(R, sub16p(Qv, X3)), mul(S1, PPP));
+  const Fq29 Y3 = mul_sum(R, sub16p(Qv, X3), sub2p(zero(), S1), PPP);
   return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
 }
 

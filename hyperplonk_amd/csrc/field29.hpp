@@ -125,9 +125,79 @@ HP_HD F29<P> mul(const F29<P>& a, const F29<P>& b) {
   return t;
 }
 
+// a^2 / R: every cross product once, against the doubled operand (2 a_j is
+// below 2^30); the columns are those of mul(a, a), so the result is the same
+// integer.  Column bound: 7 cross products below 2^59, one square and 14
+// m p products below 2^58: 29 * 2^58 < 2^63.
 template <class P>
 HP_HD F29<P> sqr(const F29<P>& a) {
-  return mul(a, a);
+  constexpr int L = P::L;
+  uint32_t d[L], m[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) d[i] = a.l[i] << 1;
+  F29<P> t;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+#pragma unroll
+    for (int i = 0; i < k - i; ++i) acc += (uint64_t)a.l[i] * d[k - i];
+    if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+    for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * P::P[k - i];
+    m[k] = ((uint32_t)acc * P::INV) & kMask;
+    acc += (uint64_t)m[k] * P::P[0];
+    acc >>= kBits;
+  }
+#pragma unroll
+  for (int k = L; k < 2 * L - 1; ++k) {
+#pragma unroll
+    for (int i = k - L + 1; i < k - i; ++i) acc += (uint64_t)a.l[i] * d[k - i];
+    if ((k & 1) == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+    for (int i = k - L + 1; i < L; ++i) acc += (uint64_t)m[i] * P::P[k - i];
+    t.l[k - L] = (uint32_t)acc & kMask;
+    acc >>= kBits;
+  }
+  t.l[L - 1] = (uint32_t)acc;
+  return t;
+}
+
+// (a b + c d) / R with one Montgomery reduction.  A column holds at most
+// 3 L = 42 products below 2^58 (< 2^63.4) plus the carry; the result is below
+// (a b + c d) / R + p.
+template <class P>
+HP_HD F29<P> mul_sum(const F29<P>& a, const F29<P>& b, const F29<P>& c, const F29<P>& d) {
+  constexpr int L = P::L;
+  uint32_t m[L];
+  F29<P> t;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+#pragma unroll
+    for (int i = 0; i < k; ++i) {
+      acc += (uint64_t)a.l[i] * b.l[k - i];
+      acc += (uint64_t)c.l[i] * d.l[k - i];
+      acc += (uint64_t)m[i] * P::P[k - i];
+    }
+    acc += (uint64_t)a.l[k] * b.l[0];
+    acc += (uint64_t)c.l[k] * d.l[0];
+    m[k] = ((uint32_t)acc * P::INV) & kMask;
+    acc += (uint64_t)m[k] * P::P[0];
+    acc >>= kBits;
+  }
+#pragma unroll
+  for (int k = L; k < 2 * L - 1; ++k) {
+#pragma unroll
+    for (int i = k - L + 1; i < L; ++i) {
+      acc += (uint64_t)a.l[i] * b.l[k - i];
+      acc += (uint64_t)c.l[i] * d.l[k - i];
+      acc += (uint64_t)m[i] * P::P[k - i];
+    }
+    t.l[k - L] = (uint32_t)acc & kMask;
+    acc >>= kBits;
+  }
+  t.l[L - 1] = (uint32_t)acc;
+  return t;
 }
 
 template <class P>

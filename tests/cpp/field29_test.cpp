@@ -97,6 +97,30 @@ int main() {
     // product bound: below p (1 + 2^-15) for inputs below 32 p
     const Fq29 m = mul(lazy(a29, 31), lazy(b29, 31));
     CHECK(m.l[L - 1] <= FqP::P[L - 1]);
+    // the squaring takes each cross product once: same integer as mul(x, x)
+    const Fq29 x = lazy(a29, (uint32_t)(rng() % 32));
+    const Fq29 s = sqr(x), sm = mul(x, x);
+    bool same_sq = true;
+    for (int i = 0; i < L; ++i) same_sq = same_sq && s.l[i] == sm.l[i];
+    CHECK(same_sq);
+    // one reduction for a sum of two products, and the same bound
+    const Fq c = rand_fq(), d = rand_fq();
+    const uint32_t kc = (uint32_t)(rng() % 32), kd = (uint32_t)(rng() % 32);
+    const Fq29 ms = mul_sum(lazy(a29, ka + 16), lazy(b29, kb + 15), lazy(to29(c), kc),
+                            lazy(to29(d), kd));
+    CHECK(back(ms) == a * b + c * d);
+    CHECK(ms.l[L - 1] <= FqP::P[L - 1]);
+  }
+  {
+    // largest limbs: 2^29 - 1 below the top limb, top limb of 31 p
+    Fq29 big = lazy(zero(), 31);
+    for (int i = 0; i < L - 1; ++i) big.l[i] = kMask;
+    const Fq29 s = sqr(big), sm = mul(big, big);
+    bool same_sq = true;
+    for (int i = 0; i < L; ++i) same_sq = same_sq && s.l[i] == sm.l[i];
+    CHECK(same_sq);
+    const Fq29 ms = mul_sum(big, big, big, big), twice = add(sm, sm);
+    CHECK(back(ms) == back(twice));
   }
   for (int it = 0; it < 100; ++it) {
     const Fq a = rand_fq();
