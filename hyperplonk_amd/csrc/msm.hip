@@ -2,10 +2,11 @@
 //
 // Stages (all on the context stream; the host finishes the last level):
 //   1. sort      counting sort of the signed c-bit digits by bucket, without
-//                materialising digits: blocks (bucket range r, scalar chunk e)
-//                recompute every digit of their chunk (Montgomery -> canonical,
-//                W windows), count those in their range with LDS atomics
-//                (8192 buckets per range), an exclusive scan turns the
+//                materialising digits: the scalars are made canonical and
+//                folded below (r - 1) / 2 once; blocks (bucket range r,
+//                scalar chunk e) recompute every digit of their chunk (W
+//                windows), count those in their range with LDS atomics
+//                (16384 buckets per range), an exclusive scan turns the
 //                (bucket, chunk) counts into offsets, and a second pass
 //                scatters base references (w n + i | sign) into
 //                bucket-contiguous order.
@@ -18,8 +19,8 @@
 //                (A, R) per block; the host adds the blocks with their
 //                weights (sum A_k + 256 sum k R_k) on 64-bit limbs.
 // The group law is exact in every case, so the result does not depend on the
-// order of accumulation.  Work per 2^20 scalars at c = 16: W = 16 windows,
-// ~16.8M mixed additions, 2^15 buckets.
+// order of accumulation.  Work per 2^20 scalars at c = 17: W = 15 windows,
+// ~15.7M mixed additions, 2^16 buckets.
 //
 // The reference computes the same value with ark-ec's msm_unchecked
 // (backend/src/cpu/multilinear_kzg.rs:55,131); the reference accelerator
@@ -53,6 +54,23 @@ __device__ __forceinline__ uint32_t scalar_bits(const uint32_t* s, uint32_t pos,
   return (uint32_t)(w >> sh) & ((1u << c) - 1u);
 }
 
+// Canonical scalar i, folded into the lower half: s > (r - 1) / 2 becomes
+// r - s with *neg set (the point is negated instead), so every scalar is
+// below 2^254 and W = ceil(255 / c) windows of signed digits end without a
+// carry.  Same value: (r - s) (-P) = s P.
+__device__ __forceinline__ Fr half_scalar(const Fr* scalars, size_t i, bool& neg) {
+  // (r - 1) / 2, little-endian words
+  constexpr uint32_t kHalf[8] = {0x80000000u, 0x7fffffffu, 0x7fff2dffu, 0xa9ded201u,
+                                 0x04d0ec02u, 0x199cec04u, 0x94cebea4u, 0x39f6d3a9u};
+  const Fr s = from_mont(load_fr(scalars, i));
+  int cmp = 0;  // sign of s - half, from the top word down
+#pragma unroll
+  for (int k = 7; k >= 0; --k)
+    if (cmp == 0 && s.v[k] != kHalf[k]) cmp = s.v[k] > kHalf[k] ? 1 : -1;
+  neg = cmp > 0;
+  return neg ? hp::neg(s) : s;
+}
+
 // Signed c-bit digit of window w given the running carry; returns the bucket
 // key (|d| - 1, or kSkip for a zero digit) and sets *neg for a negative digit.
 __device__ __forceinline__ uint32_t window_digit(const uint32_t* s, uint32_t w, uint32_t c,
@@ -81,7 +99,7 @@ struct Job {
   const Fr* scalars;
   const G1Affine* bases;  // W window-shifted copies of the layer, window-major
   uint32_t n, c, windows, buckets;
-  uint32_t bucket_base, chunks, cc_base, pad;
+  uint32_t bucket_base, chunks, cc_base, fold_base;  // fold_base: first folded scalar
   // bucket sums: 2^lanes_log2 lanes per bucket, threads [sum_base, sum_base +
   // buckets << lanes_log2); buckets with more than seq_max partials per lane
   // take the heavy path
@@ -91,12 +109,37 @@ struct SortItem {
   uint32_t job, b0, chunk, pad;
 };
 
-constexpr uint32_t kRangeBuckets = 8192;
+constexpr uint32_t kRangeBuckets = 16384;  // 64 KB of LDS counters per block
 constexpr unsigned kSortThreads = 1024;
+
+// Folded canonical scalars of every job of a batch, computed once for both
+// sort passes (which read them once per bucket range): eight words, the sign
+// of the fold in bit 31 of the top word (folded values are below 2^254).
+__global__ void k_fold_scalars(const Job* jobs, uint32_t njobs, uint32_t total, uint4* folded) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = njobs;  // the job with fold_base <= i < fold_base + n
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (jobs[mid].fold_base <= i) lo = mid; else hi = mid;
+    }
+    bool neg;
+    Fr s = half_scalar(jobs[lo].scalars, i - jobs[lo].fold_base, neg);
+    if (neg) s.v[7] |= 0x80000000u;
+    folded[2 * (size_t)i] = make_uint4(s.v[0], s.v[1], s.v[2], s.v[3]);
+    folded[2 * (size_t)i + 1] = make_uint4(s.v[4], s.v[5], s.v[6], s.v[7]);
+  }
+}
+
+__device__ __forceinline__ bool load_folded(const uint4* folded, size_t i, uint32_t* s) {
+  const uint4 a = folded[2 * i], b = folded[2 * i + 1];
+  s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+  s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w & 0x7fffffffu;
+  return (b.w >> 31) != 0;
+}
 
 // counts[cc_base + b * chunks + e] = digits of chunk e of the job in bucket b.
 __global__ void __launch_bounds__(kSortThreads)
-    k_count_items(const SortItem* items, const Job* jobs, uint32_t* counts) {
+    k_count_items(const SortItem* items, const Job* jobs, const uint4* folded, uint32_t* counts) {
   __shared__ uint32_t cnt[kRangeBuckets];
   const SortItem it = items[blockIdx.x];
   const Job j = jobs[it.job];
@@ -105,11 +148,12 @@ __global__ void __launch_bounds__(kSortThreads)
   __syncthreads();
   const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const Fr s = from_mont(load_fr(j.scalars, i));
+    uint32_t s[8];
+    load_folded(folded, j.fold_base + i, s);
     uint32_t carry = 0;
     for (uint32_t w = 0; w < j.windows; ++w) {
       bool neg;
-      const uint32_t key = window_digit(s.v, w, j.c, carry, neg);
+      const uint32_t key = window_digit(s, w, j.c, carry, neg);
       if (key - b0 < span) atomicAdd(&cnt[key - b0], 1u);  // kSkip never lands in range
     }
   }
@@ -120,8 +164,8 @@ __global__ void __launch_bounds__(kSortThreads)
 
 // sorted[offsets[...] + rank] = (w * n + i) | sign << 31  (index into job bases)
 __global__ void __launch_bounds__(kSortThreads)
-    k_scatter_items(const SortItem* items, const Job* jobs, const uint32_t* offsets,
-                    uint32_t* sorted) {
+    k_scatter_items(const SortItem* items, const Job* jobs, const uint4* folded,
+                    const uint32_t* offsets, uint32_t* sorted) {
   __shared__ uint32_t cur[kRangeBuckets];
   const SortItem it = items[blockIdx.x];
   const Job j = jobs[it.job];
@@ -131,14 +175,15 @@ __global__ void __launch_bounds__(kSortThreads)
   __syncthreads();
   const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const Fr s = from_mont(load_fr(j.scalars, i));
+    uint32_t s[8];
+    const bool sneg = load_folded(folded, j.fold_base + i, s);
     uint32_t carry = 0;
     for (uint32_t w = 0; w < j.windows; ++w) {
       bool neg;
-      const uint32_t key = window_digit(s.v, w, j.c, carry, neg);
+      const uint32_t key = window_digit(s, w, j.c, carry, neg);
       if (key - b0 < span) {
         const uint32_t pos = atomicAdd(&cur[key - b0], 1u);
-        sorted[pos] = (uint32_t)(w * j.n + i) | (neg ? 0x80000000u : 0u);
+        sorted[pos] = (uint32_t)(w * j.n + i) | (neg != sneg ? 0x80000000u : 0u);
       }
     }
   }
@@ -766,22 +811,27 @@ void zero_u32(rt::Context& c, uint32_t* p, size_t n) {
 
 // Window bits.  All windows share one bucket set, so a window whose digits
 // span few values (the top window when c does not divide the scalar width
-// well) concentrates n entries in a handful of buckets.  c = 16 (16 full
-// windows for 255-bit scalars) and c = 13 (top window of 8 bits) avoid that,
-// and measure fastest from 2^14 and from 2^11 points on (autotuner sweep on
-// MI355X); smaller MSMs keep ~64 digits per bucket.
+// well) concentrates n entries in a handful of buckets.  For the folded
+// scalars (< 2^254): c = 17 (15 windows, top digits up to 59356 of 65536
+// buckets), 15 (17 windows, top window of 14 bits) and 13 (20 windows, 7 bits)
+// avoid that.  Measured on MI355X (device time, best task cap): 2^22 c = 17
+// 12.2 ms against c = 16 12.8 ms; 2^20 3.42 against 3.60; 2^18 c = 15 1.30
+// against 1.37; 2^16 c = 15 0.69 against 0.73 (16384 buckets reduce in
+// groups of 64); 2^12 c = 13 0.45 against 0.46 (c = 15).
 Plan choose_plan(size_t n, uint32_t override_c) {
   Plan p;
   uint32_t c = override_c;
   if (c == 0) {
-    if (n >= (size_t(1) << 14)) {
-      c = 16;
+    if (n >= (size_t(1) << 20)) {
+      c = 17;
+    } else if (n >= (size_t(1) << 15)) {
+      c = 15;
     } else if (n >= (size_t(1) << 11)) {
       c = 13;
     } else {
       c = 16;
       for (int it = 0; it < 3; ++it) {
-        const uint32_t w = (256 + c - 1) / c;
+        const uint32_t w = (255 + c - 1) / c;
         const double lg = std::log2(std::max<double>(1.0, (double)n * w / 64.0));
         const int cc = (int)std::lround(lg) + 1;
         c = (uint32_t)std::min(20, std::max(4, cc));
@@ -789,7 +839,7 @@ Plan choose_plan(size_t n, uint32_t override_c) {
     }
   }
   p.c = c;
-  p.windows = (256 + c - 1) / c;
+  p.windows = (255 + c - 1) / c;
   p.buckets = 1u << (c - 1);
   return p;
 }
@@ -797,8 +847,8 @@ Plan choose_plan(size_t n, uint32_t override_c) {
 // Entries per lane task by the largest MSM of a batch (autotuner sweep).
 uint32_t default_cap(size_t largest_n) {
   if (largest_n <= (size_t(1) << 12)) return 2;
-  if (largest_n <= (size_t(1) << 16)) return 4;
-  if (largest_n <= (size_t(1) << 18)) return 8;
+  if (largest_n < (size_t(1) << 14)) return 4;
+  if (largest_n < (size_t(1) << 18)) return 8;
   return 16;
 }
 
@@ -906,9 +956,13 @@ void Batch::add(const Layer& layer, const Fr* scalars) {
   e.layer = &layer;
   e.scalars = scalars;
   e.bucket_base = gbuckets_;
-  e.chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(64, layer.n / 4096));
-  e.cc_base = cc_total_;
+  // sort blocks: (bucket range, scalar chunk) pairs, about 512 per MSM with
+  // chunks of >= 1024 scalars
   const uint32_t B = layer.plan.buckets;
+  const size_t ranges = (B + kRangeBuckets - 1) / kRangeBuckets;
+  e.chunks = (uint32_t)std::max<size_t>(
+      1, std::min<size_t>((512 + ranges - 1) / ranges, layer.n / 1024));
+  e.cc_base = cc_total_;
   gbuckets_ += (B + kRed1 - 1) / kRed1 * kRed1;
   cc_total_ += (size_t)B * e.chunks;
   m_total_ += layer.n * layer.plan.windows;
@@ -938,13 +992,14 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   // job table and sort work items, uploaded together
   std::vector<Job> jobs(njobs);
   std::vector<SortItem> items;
-  size_t sum_threads_total = 0;
+  size_t sum_threads_total = 0, fold_total = 0;
   for (uint32_t j = 0; j < njobs; ++j) {
     const Entry& e = entries_[j];
     const Plan& p = e.layer->plan;
     jobs[j] = {e.scalars, static_cast<const G1Affine*>(e.layer->bases->ptr), (uint32_t)e.layer->n,
-               p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base, 0u,
-               0u, 0u, 0u, 0u};
+               p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base,
+               (uint32_t)fold_total, 0u, 0u, 0u, 0u};
+    fold_total += e.layer->n;
     // bucket-sum shape of this MSM as if alone: lanes per bucket while each
     // lane keeps >= 2 partials and the job's threads stay <= 131072; heavy
     // threshold three times its mean partials per lane
@@ -988,14 +1043,22 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   uint32_t* task_off = ntask + G;
   uint32_t* totals = task_off + G;  // [0] entries, [1] tasks
 
+  // folded scalars (fold_total <= m_total_ < 2^31)
+  rt::BufferRef folded_buf = c.allocate(std::max<size_t>(fold_total, 1) * 2 * sizeof(uint4));
+  const uint4* folded = static_cast<const uint4*>(folded_buf->ptr);
+  keep.push_back(folded_buf);
+
   mark(0);
+  hipLaunchKernelGGL(k_fold_scalars, dim3(grid_for(fold_total)), dim3(256), 0, c.stream, djobs,
+                     njobs, (uint32_t)fold_total, static_cast<uint4*>(folded_buf->ptr));
+  rt::check_launch();
   mark(1);
   hipLaunchKernelGGL(k_count_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0, c.stream,
-                     ditems, djobs, cc_counts);
+                     ditems, djobs, folded, cc_counts);
   rt::check_launch();
   exclusive_scan(c, cc_counts, cc_offsets, cc_total_, totals, keep);
   hipLaunchKernelGGL(k_scatter_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0,
-                     c.stream, ditems, djobs, cc_offsets, sorted);
+                     c.stream, ditems, djobs, folded, cc_offsets, sorted);
   rt::check_launch();
   hipLaunchKernelGGL(k_bucket_ranges, dim3(grid_for(G)), dim3(256), 0, c.stream, djobs, njobs, G,
                      cc_offsets, totals, starts, counts, owner);

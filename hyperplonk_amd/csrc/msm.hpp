@@ -8,9 +8,9 @@
 namespace hp {
 namespace msm {
 
-// Window plan.  Windows use signed digits in [-2^(c-1), 2^(c-1)], so
-// W = ceil(256 / c) windows cover every canonical Fr scalar with no final
-// carry.  The prepared layer stores 2^(c w) P_i for every window w, so all W
+// Window plan.  Windows use signed digits in [-2^(c-1), 2^(c-1)] of scalars
+// folded below (r - 1) / 2 < 2^254 (s > (r - 1) / 2 is taken as r - s against
+// the negated point), so W = ceil(255 / c) windows end with no carry.  The prepared layer stores 2^(c w) P_i for every window w, so all W
 // windows share one set of B = 2^(c-1) buckets and the bucket reduction runs
 // once per MSM rather than once per window.
 struct Plan {

@@ -19,7 +19,7 @@ def H(v):
     return int(v, 16)
 
 
-@pytest.mark.parametrize("window", [None, 4, 7])
+@pytest.mark.parametrize("window", [None, 4, 7, 15, 17])
 def test_golden_msm_cases(backend, golden, window, monkeypatch):
     if window:
         monkeypatch.setenv("HP_MI355X_MSM_WINDOW", str(window))
@@ -42,6 +42,34 @@ def test_golden_msm_cases(backend, golden, window, monkeypatch):
         m = backend.mle_from_evaluations(N, scalars + [0] * (size - n))
         got = backend.commit(pp, m)
         assert got == pf.point_from_hex(case["expected"]), (case["name"], window)
+
+
+@pytest.mark.parametrize("window", [None, 5, 15, 16, 17])
+def test_msm_scalars_around_half_modulus(backend, window, monkeypatch):
+    """Scalars above (r - 1) / 2 are folded to r - s against the negated point:
+    the values on both sides of the fold, the extremes and 2^254 +- 1, against
+    the closed form (sum a_i s_i) G over bases a_i G."""
+    if window:
+        monkeypatch.setenv("HP_MI355X_MSM_WINDOW", str(window))
+    half = (R - 1) // 2
+    scalars = [half - 1, half, half + 1, half + 2, R - 1, R - 2, 1, 2,
+               (1 << 254) - 1, 1 << 254, (1 << 254) + 1, (1 << 253) + 7,
+               half, half + 1, 3, R - 3]
+    coeffs = [5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 5, 5, 7, 7]
+    N = 4
+    pts = [pf.aff_mul(pf.G, a) for a in coeffs]
+    layers = [b"".join(pf.packet(p) for p in pts)]
+    for i in range(1, N + 1):
+        layers.append(b"".join(pf.packet(pf.G) for _ in range(1 << (N - i))))
+    pp = backend.prepare_prover_param(N, layers)
+    m = backend.mle_from_evaluations(N, scalars)
+    want = pf.aff_mul(pf.G, sum(a * s for a, s in zip(coeffs, scalars)) % R)
+    assert backend.commit(pp, m) == want
+    # a pair that cancels across the fold: s P + (r - s) P = identity
+    m2 = backend.mle_from_evaluations(N, [half, R - half] + [0] * 14)
+    pts2 = [pf.G, pf.G] + pts[2:]
+    pp2 = backend.prepare_prover_param(N, [b"".join(pf.packet(p) for p in pts2)] + layers[1:])
+    assert backend.commit(pp2, m2) is None
 
 
 def test_srs_generation_matches_oracle(backend, oracle, rng_factory):
