@@ -6,10 +6,11 @@
 //                folded below (r - 1) / 2 once; blocks (bucket range r,
 //                scalar chunk e) recompute every digit of their chunk (W
 //                windows), count those in their range with LDS atomics
-//                (16384 buckets per range), an exclusive scan turns the
-//                (bucket, chunk) counts into offsets, and a second pass
-//                scatters base references (w n + i | sign) into
-//                bucket-contiguous order.
+//                (16384 buckets per range, counters stored chunk-major), a
+//                prefix over each bucket's chunk counts and an exclusive scan
+//                of the bucket totals give every (bucket, chunk) its first
+//                slot, and a second pass scatters base references
+//                (w n + i | sign) into bucket-contiguous order.
 //   2. tasks     each bucket is cut into tasks of <= task_cap entries.
 //   3. accumulate one lane per task (<= cap entries of one bucket): XYZZ +=
 //                affine (8M + 2S); 64 independent accumulators per wavefront.
@@ -90,8 +91,9 @@ __device__ __forceinline__ uint32_t window_digit(const uint32_t* s, uint32_t w, 
 
 // Jobs of one batch share one global bucket space: job j owns global buckets
 // [bucket_base, bucket_base + buckets) (bucket_base a multiple of 256, the
-// padding buckets stay empty) and (bucket, chunk) counters
-// [cc_base, cc_base + buckets * chunks).  Counting and scattering run in
+// padding buckets stay empty) and (chunk, bucket) counters
+// [cc_base, cc_base + chunks * buckets), chunk-major so that the sort blocks
+// read and write them coalesced.  Counting and scattering run in
 // items (job, bucket range of <= kRangeBuckets, scalar chunk) so that the
 // per-block counters live in LDS; digits are recomputed from the scalars in
 // both passes, so no key/value arrays are ever written.
@@ -137,7 +139,7 @@ __device__ __forceinline__ bool load_folded(const uint4* folded, size_t i, uint3
   return (b.w >> 31) != 0;
 }
 
-// counts[cc_base + b * chunks + e] = digits of chunk e of the job in bucket b.
+// counts[cc_base + e * buckets + b] = digits of chunk e of the job in bucket b.
 __global__ void __launch_bounds__(kSortThreads)
     k_count_items(const SortItem* items, const Job* jobs, const uint4* folded, uint32_t* counts) {
   __shared__ uint32_t cnt[kRangeBuckets];
@@ -158,20 +160,52 @@ __global__ void __launch_bounds__(kSortThreads)
     }
   }
   __syncthreads();
-  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x)
-    counts[j.cc_base + (size_t)(b0 + b) * j.chunks + it.chunk] = cnt[b];
+  uint32_t* row = counts + j.cc_base + (size_t)it.chunk * j.buckets + b0;
+  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) row[b] = cnt[b];
+}
+
+// Per global bucket g: owner job, entry count, and the exclusive prefix of
+// its chunk counts (the chunk's first slot inside the bucket); padding
+// buckets count 0.
+__global__ void k_chunk_prefix(const Job* jobs, uint32_t njobs, uint32_t gbuckets,
+                               const uint32_t* cc_counts, uint32_t* cc_prefix, uint32_t* counts,
+                               uint32_t* owner) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < gbuckets;
+       g += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = njobs;  // last job with bucket_base <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (jobs[mid].bucket_base <= g) lo = mid; else hi = mid;
+    }
+    const Job j = jobs[lo];
+    const uint32_t b = g - j.bucket_base;
+    owner[g] = lo;
+    uint32_t run = 0;
+    if (b < j.buckets) {
+      const size_t base = j.cc_base + b;
+#pragma unroll 8
+      for (uint32_t e = 0; e < j.chunks; ++e) {
+        const size_t at = base + (size_t)e * j.buckets;
+        const uint32_t v = cc_counts[at];
+        cc_prefix[at] = run;
+        run += v;
+      }
+    }
+    counts[g] = run;
+  }
 }
 
 // sorted[offsets[...] + rank] = (w * n + i) | sign << 31  (index into job bases)
 __global__ void __launch_bounds__(kSortThreads)
     k_scatter_items(const SortItem* items, const Job* jobs, const uint4* folded,
-                    const uint32_t* offsets, uint32_t* sorted) {
+                    const uint32_t* cc_prefix, const uint32_t* starts, uint32_t* sorted) {
   __shared__ uint32_t cur[kRangeBuckets];
   const SortItem it = items[blockIdx.x];
   const Job j = jobs[it.job];
   const uint32_t b0 = it.b0, span = min(j.buckets - b0, kRangeBuckets);
-  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x)
-    cur[b] = offsets[j.cc_base + (size_t)(b0 + b) * j.chunks + it.chunk];
+  const uint32_t* row = cc_prefix + j.cc_base + (size_t)it.chunk * j.buckets + b0;
+  const uint32_t* first = starts + j.bucket_base + b0;
+  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) cur[b] = first[b] + row[b];
   __syncthreads();
   const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -186,34 +220,6 @@ __global__ void __launch_bounds__(kSortThreads)
         sorted[pos] = (uint32_t)(w * j.n + i) | (neg != sneg ? 0x80000000u : 0u);
       }
     }
-  }
-}
-
-// Global bucket g: start, count, owning job (padding buckets: count 0).
-__global__ void k_bucket_ranges(const Job* jobs, uint32_t njobs, uint32_t gbuckets,
-                                const uint32_t* offsets, const uint32_t* total, uint32_t* starts,
-                                uint32_t* counts, uint32_t* owner) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < gbuckets;
-       g += gridDim.x * blockDim.x) {
-    uint32_t lo = 0, hi = njobs;  // last job with bucket_base <= g
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (jobs[mid].bucket_base <= g) lo = mid; else hi = mid;
-    }
-    const Job j = jobs[lo];
-    const uint32_t b = g - j.bucket_base;
-    owner[g] = lo;
-    if (b >= j.buckets) {
-      starts[g] = 0;
-      counts[g] = 0;
-      continue;
-    }
-    const uint32_t s = offsets[j.cc_base + (size_t)b * j.chunks];
-    uint32_t e;
-    if (b + 1 < j.buckets) e = offsets[j.cc_base + (size_t)(b + 1) * j.chunks];
-    else e = lo + 1 < njobs ? offsets[jobs[lo + 1].cc_base] : *total;
-    starts[g] = s;
-    counts[g] = e - s;
   }
 }
 
@@ -1056,12 +1062,12 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   hipLaunchKernelGGL(k_count_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0, c.stream,
                      ditems, djobs, folded, cc_counts);
   rt::check_launch();
-  exclusive_scan(c, cc_counts, cc_offsets, cc_total_, totals, keep);
-  hipLaunchKernelGGL(k_scatter_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0,
-                     c.stream, ditems, djobs, folded, cc_offsets, sorted);
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(grid_for(G)), dim3(256), 0, c.stream, djobs, njobs, G,
+                     cc_counts, cc_offsets, counts, owner);
   rt::check_launch();
-  hipLaunchKernelGGL(k_bucket_ranges, dim3(grid_for(G)), dim3(256), 0, c.stream, djobs, njobs, G,
-                     cc_offsets, totals, starts, counts, owner);
+  exclusive_scan(c, counts, starts, G, totals, keep);  // totals[0]: entries
+  hipLaunchKernelGGL(k_scatter_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0,
+                     c.stream, ditems, djobs, folded, cc_offsets, starts, sorted);
   rt::check_launch();
   mark(2);
   hipLaunchKernelGGL(k_task_counts, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, G, cap,
