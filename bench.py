@@ -44,18 +44,19 @@ VALU_MUL_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 LIMB_PRODUCTS_PER_FQM = 300
 # XYZZ mixed addition = 8M + 2S Fq products.
 FQM_PER_MIXED_ADD = 10
-# HBM bytes of one 2^20 k_accumulate launch, from the FETCH_SIZE and
-# WRITE_SIZE passes calibrated on a 96 MiB device copy
-# (profiles/k_accumulate_msm_2_20_pmc.md); rocprofv3 counters cannot be read
-# from inside this process.
-ACCUMULATE_TRAFFIC_2_20 = 3.852e9
+# HBM bytes of one 2^20 k_accumulate launch (c = 17, task cap 32), from the
+# FETCH_SIZE and WRITE_SIZE passes calibrated on the in-place conversion of
+# 2^20 points (96 MiB read and written; profiles/k_accumulate_msm_2_20_pmc.md);
+# rocprofv3 counters cannot be read from inside this process.
+ACCUMULATE_TRAFFIC_2_20 = 3.877e9
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    # the GPU clock ramps up under sustained load: 20 commits (~70 ms) reach it
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--log-size", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log", type=int, default=18)

