@@ -621,8 +621,11 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_frac_poly(
     rt::BufferRef flag = c.allocate(sizeof(unsigned));
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.stream, static_cast<unsigned*>(flag->ptr));
     rt::check_launch();
-    // ~16 entries per thread amortise the one Fermat inversion per thread
-    const size_t threads = std::max<size_t>(1, std::min<size_t>(n / 16 + 1, 65536));
+    // one Fermat inversion per thread: small tables take one entry per thread
+    // (the chain is the inversion alone), large ones ~16 entries per thread
+    // with at least 16384 threads
+    const size_t threads = std::min<size_t>(
+        n, std::max<size_t>(16384, std::min<size_t>(n / 16 + 1, 65536)));
     hipLaunchKernelGGL(k_frac, dim3(rt::blocks_for(threads, 64)), dim3(64), 0, c.stream,
                        static_cast<const Fr* const*>(dn->ptr),
                        static_cast<const Fr* const*>(dd->ptr), count,
