@@ -124,6 +124,9 @@ constexpr int kJobsPerLaunch = 16;
 struct JobPack {
   RoundJob job[kJobsPerLaunch];
   uint32_t count;
+  // 1: multi-block jobs only store their block partials and k_finish_split,
+  // launched next, adds them; 0: the last block of each job (a ticket) adds them
+  uint32_t split;
 };
 
 // term[t] = coeff * prod_k (lo_k + t d_k) for t = 0..deg, streaming over the
@@ -199,9 +202,25 @@ __device__ __forceinline__ uint32_t job_of_block(const JobPack& pack) {
 // Block partials of a job, then the last block of the job adds them and
 // writes the canonical message.
 template <int D>
-__device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1]) {
+__device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1], bool split) {
   const uint32_t deg = job.degree;
+  if (job.blocks == 1) {
+    // One block holds the whole message: no block partials, no ticket and no
+    // device-scope fences (on this multi-XCD part each one writes back the
+    // L2, several microseconds apiece, which dominated the small rounds).
+    // Thread t reads back the value it wrote itself.
+    __shared__ __align__(16) Fr msg[D + 1];
+    block_reduce_store<D + 1>(acc, deg + 1, msg, 0);
+    if (threadIdx.x <= deg) {
+      store_fr(job.out, threadIdx.x, from_mont(msg[threadIdx.x]));
+      __threadfence_system();  // `out` may be pinned host memory read after the sync
+    }
+    return;
+  }
   block_reduce_store<D + 1>(acc, deg + 1, job.partial + (size_t)lblock * (deg + 1), 0);
+  // the kernel boundary orders the partials before k_finish_split reads them,
+  // with no device-scope fence or atomic here
+  if (split) return;
   __shared__ unsigned last;
   __threadfence();
   __syncthreads();
@@ -228,6 +247,29 @@ __device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1
     __threadfence_system();  // `out` may be pinned host memory read after the sync
   }
   if (threadIdx.x == 0) *job.ticket = 0;
+}
+
+// Second phase of a split launch: block j adds the block partials of job j of
+// the pack (strided shares of each point t, then one thread per t) and writes
+// its canonical message.  Single-block jobs finished in the round kernel.
+__global__ void __launch_bounds__(kThreads) k_finish_split(const JobPack pack) {
+  const RoundJob& job = pack.job[blockIdx.x];
+  if (job.blocks <= 1) return;
+  const uint32_t width = job.degree + 1, groups = kThreads / width;
+  const uint32_t t = threadIdx.x % width, g = threadIdx.x / width;
+  Fr s = Fr::zero();
+  if (g < groups)
+    for (uint32_t blk = g; blk < job.blocks; blk += groups)
+      s = s + load_fr(job.partial, (size_t)blk * width + t);
+  __shared__ Fr red[kThreads];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < width) {
+    Fr total = Fr::zero();
+    for (uint32_t k = 0; k < groups; ++k) total = total + red[k * width + threadIdx.x];
+    store_fr(job.out, threadIdx.x, from_mont(total));
+    __threadfence_system();  // `out` may be pinned host memory read after the sync
+  }
 }
 
 // Thread per pair (grid-strided): fold every distinct factor on the pair, then
@@ -271,7 +313,7 @@ __global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
   }
   Fr out[D + 1];
   to_stored<D>(acc, out);
-  finish_job<D>(job, lblock, out);
+  finish_job<D>(job, lblock, out, pack.split != 0);
 }
 
 // Thread per (product p, pair b), product-major, one item per thread: the
@@ -319,7 +361,7 @@ __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
   }
   Fr out[D + 1];
   to_stored<D>(pt.term, out);
-  finish_job<D>(job, lblock, out);
+  finish_job<D>(job, lblock, out, pack.split != 0);
 }
 
 // Thread per (product p, pair b, point t), t fastest, for the smallest
@@ -373,7 +415,7 @@ __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
     for (int k = 0; k <= D; ++k)
       if ((uint32_t)k == t) out[k] = v;
   }
-  finish_job<D>(job, lblock, out);
+  finish_job<D>(job, lblock, out, pack.split != 0);
 }
 
 // Any degree: message values for t in [t0, t0 + 8) only (host loops over t0).
@@ -612,11 +654,22 @@ const void* kernel_of(uint32_t d, uint32_t mode) {
   }
 }
 
+// Split launches: a round kernel whose multi-block jobs only store their
+// block partials, then k_finish_split (one block per job) adds them.  The
+// ticket path's two device-scope fences (each an L2 write-back on this
+// multi-XCD part) cost more than the second launch.
 void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
   for (PackLaunch& l : plan_packs(jobs)) {
+    l.pack.split = 1;
     void* args[] = {&l.pack};
     rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.mode), dim3(l.grid), dim3(sc::kThreads), args,
                                   0, c.stream));
+    bool multi = false;
+    for (uint32_t j = 0; j < l.pack.count; ++j) multi = multi || l.pack.job[j].blocks > 1;
+    if (multi) {
+      rt::hip_check(hipLaunchKernel(reinterpret_cast<const void*>(&sc::k_finish_split),
+                                    dim3(l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
+    }
   }
 }
 
