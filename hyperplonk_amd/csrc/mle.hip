@@ -168,6 +168,17 @@ __global__ void __launch_bounds__(kThreads) k_eval_dots(const DotJob* jobs, uint
   acc = wave_sum_fr(acc);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
   __syncthreads();
+  if (job.blocks == 1) {
+    // the block's sum is the value: no partial, ticket or device-scope fence
+    // (each fence writes back the L2 on this multi-XCD part)
+    if (threadIdx.x == 0) {
+      Fr s = lds[0];
+      for (unsigned w = 1; w < kThreads / 64; ++w) s = s + lds[w];
+      store_fr(job.out, 0, from_mont(s));
+      __threadfence_system();  // `out` is pinned host memory read after the sync
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
     Fr s = lds[0];
     for (unsigned w = 1; w < kThreads / 64; ++w) s = s + lds[w];
