@@ -1,16 +1,13 @@
 // Pippenger G1 MSM on gfx950 against a prepared, window-shifted SRS layer.
 //
 // Stages (all on the context stream; the host finishes the last level):
-//   1. sort      counting sort of the signed c-bit digits by bucket, without
-//                materialising digits: the scalars are made canonical and
-//                folded below (r - 1) / 2 once; blocks (bucket range r,
-//                scalar chunk e) recompute every digit of their chunk (W
-//                windows), count those in their range with LDS atomics
-//                (16384 buckets per range, counters stored chunk-major), a
-//                prefix over each bucket's chunk counts and an exclusive scan
-//                of the bucket totals give every (bucket, chunk) its first
-//                slot, and a second pass scatters base references
-//                (w n + i | sign) into bucket-contiguous order.
+//   1. sort      the scalars are made canonical and folded below (r - 1) / 2
+//                once; the signed c-bit digits are then sorted by bucket in
+//                two counting-sort passes: by partition of 256 buckets
+//                (blocks of scalar chunks recompute their digits and write
+//                8-byte records in long runs), then inside each partition
+//                by one block with LDS counters, leaving base references
+//                (w n + i | sign) in bucket-contiguous order.
 //   2. tasks     each bucket is cut into tasks of <= task_cap entries.
 //   3. accumulate one lane per task (<= cap entries of one bucket): XYZZ +=
 //                affine (8M + 2S); 64 independent accumulators per wavefront.
@@ -91,12 +88,19 @@ __device__ __forceinline__ uint32_t window_digit(const uint32_t* s, uint32_t w, 
 
 // Jobs of one batch share one global bucket space: job j owns global buckets
 // [bucket_base, bucket_base + buckets) (bucket_base a multiple of 256, the
-// padding buckets stay empty) and (chunk, bucket) counters
-// [cc_base, cc_base + chunks * buckets), chunk-major so that the sort blocks
-// read and write them coalesced.  Counting and scattering run in
-// items (job, bucket range of <= kRangeBuckets, scalar chunk) so that the
-// per-block counters live in LDS; digits are recomputed from the scalars in
-// both passes, so no key/value arrays are ever written.
+// padding buckets stay empty).  Every job's padded bucket range is cut into
+// kParts = 256 partitions of 2^part_bits buckets (part_bits = max(0, c - 9)),
+// global partition j * 256 + q.  The sort is two counting-sort passes:
+//   1. blocks (job, scalar chunk) recompute their digits and place 8-byte
+//      records (base reference, bucket within the partition) by partition:
+//      (chunk, partition) counters [cc_base, cc_base + chunks * 256),
+//      chunk-major, a prefix over each partition's chunks and a scan of the
+//      partition totals give every (partition, chunk) its first slot; chunks
+//      of ~2048 scalars leave runs of ~100 records per (partition, chunk), so
+//      the writes are coalesced;
+//   2. one block per partition counts its records per bucket in LDS and writes
+//      the base references in bucket order inside the partition's span, with
+//      the bucket starts, counts and owners.
 struct Job {
   const Fr* scalars;
   const G1Affine* bases;  // W window-shifted copies of the layer, window-major
@@ -106,12 +110,14 @@ struct Job {
   // buckets << lanes_log2); buckets with more than seq_max partials per lane
   // take the heavy path
   uint32_t lanes_log2, seq_max, sum_base, padded_buckets;  // padded: multiple of 256
+  uint32_t part_bits, pad0, pad1, pad2;  // buckets per sort partition: 2^part_bits
 };
 struct SortItem {
-  uint32_t job, b0, chunk, pad;
+  uint32_t job, chunk;
 };
 
-constexpr uint32_t kRangeBuckets = 16384;  // 64 KB of LDS counters per block
+constexpr uint32_t kPartsLog2 = 8, kParts = 1u << kPartsLog2;  // sort partitions per job
+constexpr uint32_t kMaxPartBuckets = (1u << 19) >> kPartsLog2;   // c <= 20
 constexpr unsigned kSortThreads = 1024;
 
 // Folded canonical scalars of every job of a batch, computed once for both
@@ -139,14 +145,13 @@ __device__ __forceinline__ bool load_folded(const uint4* folded, size_t i, uint3
   return (b.w >> 31) != 0;
 }
 
-// counts[cc_base + e * buckets + b] = digits of chunk e of the job in bucket b.
+// counts[cc_base + e * 256 + q] = digits of chunk e of the job in partition q.
 __global__ void __launch_bounds__(kSortThreads)
-    k_count_items(const SortItem* items, const Job* jobs, const uint4* folded, uint32_t* counts) {
-  __shared__ uint32_t cnt[kRangeBuckets];
+    k_part_count(const SortItem* items, const Job* jobs, const uint4* folded, uint32_t* counts) {
+  __shared__ uint32_t cnt[kParts];
   const SortItem it = items[blockIdx.x];
   const Job j = jobs[it.job];
-  const uint32_t b0 = it.b0, span = min(j.buckets - b0, kRangeBuckets);
-  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) cnt[b] = 0;
+  for (uint32_t q = threadIdx.x; q < kParts; q += blockDim.x) cnt[q] = 0;
   __syncthreads();
   const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -156,57 +161,51 @@ __global__ void __launch_bounds__(kSortThreads)
     for (uint32_t w = 0; w < j.windows; ++w) {
       bool neg;
       const uint32_t key = window_digit(s, w, j.c, carry, neg);
-      if (key - b0 < span) atomicAdd(&cnt[key - b0], 1u);  // kSkip never lands in range
+      if (key != kSkip) atomicAdd(&cnt[key >> j.part_bits], 1u);
     }
   }
   __syncthreads();
-  uint32_t* row = counts + j.cc_base + (size_t)it.chunk * j.buckets + b0;
-  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) row[b] = cnt[b];
+  uint32_t* row = counts + j.cc_base + (size_t)it.chunk * kParts;
+  for (uint32_t q = threadIdx.x; q < kParts; q += blockDim.x) row[q] = cnt[q];
 }
 
-// Per global bucket g: owner job, entry count, and the exclusive prefix of
-// its chunk counts (the chunk's first slot inside the bucket); padding
-// buckets count 0.
-__global__ void k_chunk_prefix(const Job* jobs, uint32_t njobs, uint32_t gbuckets,
-                               const uint32_t* cc_counts, uint32_t* cc_prefix, uint32_t* counts,
-                               uint32_t* owner) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < gbuckets;
-       g += gridDim.x * blockDim.x) {
-    uint32_t lo = 0, hi = njobs;  // last job with bucket_base <= g
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (jobs[mid].bucket_base <= g) lo = mid; else hi = mid;
-    }
-    const Job j = jobs[lo];
-    const uint32_t b = g - j.bucket_base;
-    owner[g] = lo;
-    uint32_t run = 0;
-    if (b < j.buckets) {
-      const size_t base = j.cc_base + b;
-#pragma unroll 8
-      for (uint32_t e = 0; e < j.chunks; ++e) {
-        const size_t at = base + (size_t)e * j.buckets;
-        const uint32_t v = cc_counts[at];
-        cc_prefix[at] = run;
-        run += v;
-      }
-    }
-    counts[g] = run;
+// One block per global partition: the first slot of every chunk inside it
+// (exclusive prefix over its chunk counts, one thread per chunk, chunks <=
+// kMaxChunks) and its record count.
+constexpr unsigned kMaxChunks = 512;
+__global__ void __launch_bounds__(kMaxChunks)
+    k_part_prefix(const Job* jobs, const uint32_t* cc_counts, uint32_t* cc_prefix,
+                  uint32_t* part_count) {
+  __shared__ uint32_t sc[kMaxChunks];
+  const uint32_t gq = blockIdx.x;
+  const Job& j = jobs[gq >> kPartsLog2];
+  const uint32_t e = threadIdx.x;
+  const size_t at = j.cc_base + (gq & (kParts - 1)) + (size_t)e * kParts;
+  const uint32_t v = e < j.chunks ? cc_counts[at] : 0u;
+  sc[e] = v;
+  __syncthreads();
+  for (unsigned d = 1; d < kMaxChunks; d <<= 1) {
+    const uint32_t add = e >= d ? sc[e - d] : 0u;
+    __syncthreads();
+    sc[e] += add;
+    __syncthreads();
   }
+  if (e < j.chunks) cc_prefix[at] = sc[e] - v;
+  if (e == kMaxChunks - 1) part_count[gq] = sc[e];
 }
 
-// sorted[offsets[...] + rank] = (w * n + i) | sign << 31  (index into job bases)
+// records[slot] = ((w n + i) | sign << 31, bucket within the partition)
 __global__ void __launch_bounds__(kSortThreads)
-    k_scatter_items(const SortItem* items, const Job* jobs, const uint4* folded,
-                    const uint32_t* cc_prefix, const uint32_t* starts, uint32_t* sorted) {
-  __shared__ uint32_t cur[kRangeBuckets];
+    k_part_scatter(const SortItem* items, const Job* jobs, const uint4* folded,
+                   const uint32_t* cc_prefix, const uint32_t* part_start, uint2* records) {
+  __shared__ uint32_t cur[kParts];
   const SortItem it = items[blockIdx.x];
   const Job j = jobs[it.job];
-  const uint32_t b0 = it.b0, span = min(j.buckets - b0, kRangeBuckets);
-  const uint32_t* row = cc_prefix + j.cc_base + (size_t)it.chunk * j.buckets + b0;
-  const uint32_t* first = starts + j.bucket_base + b0;
-  for (uint32_t b = threadIdx.x; b < span; b += blockDim.x) cur[b] = first[b] + row[b];
+  const uint32_t* row = cc_prefix + j.cc_base + (size_t)it.chunk * kParts;
+  const uint32_t* first = part_start + (size_t)it.job * kParts;
+  for (uint32_t q = threadIdx.x; q < kParts; q += blockDim.x) cur[q] = first[q] + row[q];
   __syncthreads();
+  const uint32_t low = (1u << j.part_bits) - 1u;
   const size_t i0 = (size_t)j.n * it.chunk / j.chunks, i1 = (size_t)j.n * (it.chunk + 1) / j.chunks;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     uint32_t s[8];
@@ -215,11 +214,69 @@ __global__ void __launch_bounds__(kSortThreads)
     for (uint32_t w = 0; w < j.windows; ++w) {
       bool neg;
       const uint32_t key = window_digit(s, w, j.c, carry, neg);
-      if (key - b0 < span) {
-        const uint32_t pos = atomicAdd(&cur[key - b0], 1u);
-        sorted[pos] = (uint32_t)(w * j.n + i) | (neg != sneg ? 0x80000000u : 0u);
+      if (key != kSkip) {
+        const uint32_t pos = atomicAdd(&cur[key >> j.part_bits], 1u);
+        records[pos] = make_uint2((uint32_t)(w * j.n + i) | (neg != sneg ? 0x80000000u : 0u),
+                                  key & low);
       }
     }
+  }
+}
+
+// One block per global partition: counting sort of its records by bucket
+// inside the partition's span [part_start, part_start + part_count), which is
+// exactly the union of its buckets' spans; writes each bucket's start, count
+// and owning job.
+__global__ void __launch_bounds__(kSortThreads)
+    k_part_sort(const Job* jobs, const uint32_t* part_start, const uint32_t* part_count,
+                const uint2* records, uint32_t* sorted, uint32_t* starts, uint32_t* counts,
+                uint32_t* owner) {
+  __shared__ uint32_t cnt[kMaxPartBuckets], cur[kMaxPartBuckets];
+  __shared__ uint32_t share[64];
+  const uint32_t gq = blockIdx.x, job = gq >> kPartsLog2;
+  const Job& j = jobs[job];
+  const uint32_t nb = 1u << j.part_bits;
+  const uint32_t s0 = part_start[gq], len = part_count[gq];
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) cnt[b] = 0;
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < len; x += blockDim.x) atomicAdd(&cnt[records[s0 + x].y], 1u);
+  __syncthreads();
+  // exclusive scan of the nb counters: 64 threads each own a span of
+  // ceil(nb / 64) counters, thread 0 scans the 64 span totals
+  const uint32_t span = (nb + 63) / 64;
+  if (threadIdx.x < 64) {
+    uint32_t t = 0;
+    for (uint32_t b = threadIdx.x * span; b < min(nb, (threadIdx.x + 1) * span); ++b) t += cnt[b];
+    share[threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = s0;
+    for (uint32_t k = 0; k < 64; ++k) {
+      const uint32_t t = share[k];
+      share[k] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    uint32_t run = share[threadIdx.x];
+    for (uint32_t b = threadIdx.x * span; b < min(nb, (threadIdx.x + 1) * span); ++b) {
+      cur[b] = run;
+      run += cnt[b];
+    }
+  }
+  __syncthreads();
+  const uint32_t g0 = j.bucket_base + ((gq & (kParts - 1)) << j.part_bits);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    starts[g0 + b] = cur[b];
+    counts[g0 + b] = cnt[b];
+    owner[g0 + b] = job;
+  }
+  __syncthreads();  // the starts are read before the slots advance
+  for (uint32_t x = threadIdx.x; x < len; x += blockDim.x) {
+    const uint2 e = records[s0 + x];
+    sorted[atomicAdd(&cur[e.y], 1u)] = e.x;
   }
 }
 
@@ -963,15 +1020,13 @@ void Batch::add(const Layer& layer, const Fr* scalars) {
   e.layer = &layer;
   e.scalars = scalars;
   e.bucket_base = gbuckets_;
-  // sort blocks: (bucket range, scalar chunk) pairs, about 512 per MSM with
-  // chunks of >= 1024 scalars
+  // sort blocks: scalar chunks of >= 1024 scalars, at most 512 per MSM
   const uint32_t B = layer.plan.buckets;
-  const size_t ranges = (B + kRangeBuckets - 1) / kRangeBuckets;
-  e.chunks = (uint32_t)std::max<size_t>(
-      1, std::min<size_t>((512 + ranges - 1) / ranges, layer.n / 1024));
+  const uint32_t padded = (B + kRed1 - 1) / kRed1 * kRed1;
+  e.chunks = (uint32_t)std::max<size_t>(1, std::min<size_t>(kMaxChunks, layer.n / 1024));
   e.cc_base = cc_total_;
-  gbuckets_ += (B + kRed1 - 1) / kRed1 * kRed1;
-  cc_total_ += (size_t)B * e.chunks;
+  gbuckets_ += padded;
+  cc_total_ += (size_t)kParts * e.chunks;
   m_total_ += layer.n * layer.plan.windows;
   largest_n_ = std::max(largest_n_, layer.n);
   // positions and counter indices are 32-bit on the device
@@ -1017,10 +1072,10 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
     jobs[j].seq_max =
         std::max<uint32_t>(kSeqMax, (uint32_t)std::ceil(3.0 * per_bucket / (double)(1u << l)));
     jobs[j].padded_buckets = (p.buckets + kRed1 - 1) / kRed1 * kRed1;
+    jobs[j].part_bits = p.c > 9 ? p.c - 9 : 0;  // 256 partitions of the padded range
     jobs[j].sum_base = sum_threads_total;
     sum_threads_total += (((size_t)jobs[j].padded_buckets << l) + 63) / 64 * 64;
-    for (uint32_t b0 = 0; b0 < p.buckets; b0 += kRangeBuckets)
-      for (uint32_t ch = 0; ch < e.chunks; ++ch) items.push_back({j, b0, ch, 0u});
+    for (uint32_t ch = 0; ch < e.chunks; ++ch) items.push_back({j, ch});
   }
   const size_t job_bytes = njobs * sizeof(Job), item_bytes = items.size() * sizeof(SortItem);
   rt::BufferRef table = c.allocate(job_bytes + item_bytes);
@@ -1040,7 +1095,9 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   std::vector<rt::BufferRef> keep;
   rt::BufferRef sorted_buf = c.allocate(std::max<size_t>(m_total_, 1) * sizeof(uint32_t));
   uint32_t* sorted = static_cast<uint32_t*>(sorted_buf->ptr);
-  rt::BufferRef meta = c.allocate((2 * cc_total_ + 5 * (size_t)G + 4) * sizeof(uint32_t));
+  const uint32_t P = njobs * kParts;  // sort partitions
+  rt::BufferRef meta =
+      c.allocate((2 * cc_total_ + 5 * (size_t)G + 2 * (size_t)P + 4) * sizeof(uint32_t));
   uint32_t* cc_counts = static_cast<uint32_t*>(meta->ptr);
   uint32_t* cc_offsets = cc_counts + cc_total_;
   uint32_t* starts = cc_offsets + cc_total_;
@@ -1048,7 +1105,13 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   uint32_t* owner = counts + G;
   uint32_t* ntask = owner + G;
   uint32_t* task_off = ntask + G;
-  uint32_t* totals = task_off + G;  // [0] entries, [1] tasks
+  uint32_t* part_count = task_off + G;
+  uint32_t* part_start = part_count + P;
+  uint32_t* totals = part_start + P;  // [0] entries, [1] tasks
+  // partition records, 8 bytes per non-zero digit (at most m_total_)
+  rt::BufferRef records_buf = c.allocate(std::max<size_t>(m_total_, 1) * sizeof(uint2));
+  uint2* records = static_cast<uint2*>(records_buf->ptr);
+  keep.push_back(records_buf);
 
   // folded scalars (fold_total <= m_total_ < 2^31)
   rt::BufferRef folded_buf = c.allocate(std::max<size_t>(fold_total, 1) * 2 * sizeof(uint4));
@@ -1060,15 +1123,18 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
                      njobs, (uint32_t)fold_total, static_cast<uint4*>(folded_buf->ptr));
   rt::check_launch();
   mark(1);
-  hipLaunchKernelGGL(k_count_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0, c.stream,
+  hipLaunchKernelGGL(k_part_count, dim3((unsigned)items.size()), dim3(kSortThreads), 0, c.stream,
                      ditems, djobs, folded, cc_counts);
   rt::check_launch();
-  hipLaunchKernelGGL(k_chunk_prefix, dim3(grid_for(G)), dim3(256), 0, c.stream, djobs, njobs, G,
-                     cc_counts, cc_offsets, counts, owner);
+  hipLaunchKernelGGL(k_part_prefix, dim3(P), dim3(kMaxChunks), 0, c.stream, djobs, cc_counts,
+                     cc_offsets, part_count);
   rt::check_launch();
-  exclusive_scan(c, counts, starts, G, totals, keep);  // totals[0]: entries
-  hipLaunchKernelGGL(k_scatter_items, dim3((unsigned)items.size()), dim3(kSortThreads), 0,
-                     c.stream, ditems, djobs, folded, cc_offsets, starts, sorted);
+  exclusive_scan(c, part_count, part_start, P, totals, keep);  // totals[0]: entries
+  hipLaunchKernelGGL(k_part_scatter, dim3((unsigned)items.size()), dim3(kSortThreads), 0,
+                     c.stream, ditems, djobs, folded, cc_offsets, part_start, records);
+  rt::check_launch();
+  hipLaunchKernelGGL(k_part_sort, dim3(P), dim3(kSortThreads), 0, c.stream, djobs, part_start,
+                     part_count, records, sorted, starts, counts, owner);
   rt::check_launch();
   mark(2);
   hipLaunchKernelGGL(k_task_counts, dim3(grid_for(G)), dim3(256), 0, c.stream, counts, G, cap,
