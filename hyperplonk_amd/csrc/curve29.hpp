@@ -102,6 +102,47 @@ HP_HD Xyzz29 add(const Xyzz29& p, const Xyzz29& q) {
   return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
 }
 
+// P + Q by two lanes (half = 0, 1) that both hold P and Q: each lane does
+// half of the independent products and `ex(v)` returns the partner lane's v
+// (every exchange is reached by both lanes of the pair together).  Per lane:
+// 5 products, 1 squaring and the Y3 sum against 12 products and 2 squarings
+// for add(); both lanes return the same point, equal to add(P, Q) as a group
+// element.  For latency-bound reductions whose wavefronts leave SIMDs idle.
+//   stage 1: lane 0 U1 = X1 ZZ2, S1 = Y1 ZZZ2, ZZ12 = ZZ1 ZZ2;
+//            lane 1 U2 = X2 ZZ1, S2 = Y2 ZZZ1, ZZZ12 = ZZZ1 ZZZ2
+//   stage 2: lane 0 PP = P^2, lane 1 RR = R^2
+//   stage 3: lane 0 Q = U1 PP, ZZ3 = ZZ12 PP; lane 1 PPP = P PP, ZZZ3 = ZZZ12 PPP
+//   stage 4: both X3 and Y3 (the bounds of add()).
+template <class Ex>
+HP_HD Xyzz29 add_pair(const Xyzz29& p, const Xyzz29& q, uint32_t half, Ex ex) {
+  if (q.is_identity()) return p;
+  if (p.is_identity()) return q;
+  const bool h = half != 0;
+  const Fq29 m1 = mul(h ? q.X : p.X, h ? p.ZZ : q.ZZ);     // U1 | U2
+  const Fq29 m2 = mul(h ? q.Y : p.Y, h ? p.ZZZ : q.ZZZ);   // S1 | S2
+  const Fq29 m3 = mul(h ? p.ZZZ : p.ZZ, h ? q.ZZZ : q.ZZ); // ZZ12 | ZZZ12
+  const Fq29 o1 = ex(m1), o2 = ex(m2);
+  const Fq29 U1 = h ? o1 : m1, U2 = h ? m1 : o1;
+  const Fq29 S1 = h ? o2 : m2, S2 = h ? m2 : o2;
+  const Fq29 P = sub2p(U2, U1);
+  const Fq29 R = sub2p(S2, S1);
+  if (is_zero_mod(P)) {
+    if (is_zero_mod(R)) return dbl(p);
+    return Xyzz29::identity();
+  }
+  const Fq29 m4 = sqr(h ? R : P);  // PP | RR
+  const Fq29 o4 = ex(m4);
+  const Fq29 PP = h ? o4 : m4, RR = h ? m4 : o4;
+  const Fq29 m5 = mul(h ? P : U1, PP);  // Q | PPP
+  const Fq29 o5 = ex(m5);
+  const Fq29 Qv = h ? o5 : m5, PPP = h ? m5 : o5;
+  const Fq29 m6 = mul(m3, h ? PPP : PP);  // ZZ3 | ZZZ3
+  const Fq29 o6 = ex(m6);
+  const Fq29 X3 = sub8p(RR, add_twice(PPP, Qv));
+  const Fq29 Y3 = mul_sum(R, sub16p(Qv, X3), sub2p(zero(), S1), PPP);
+  return {X3, Y3, h ? o6 : m6, h ? m6 : o6};
+}
+
 // a^(p - 2) (zero maps to zero); the one inversion per thread of the batch
 // normalisations
 HP_HD Fq29 inverse(const Fq29& a) {

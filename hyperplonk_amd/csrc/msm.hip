@@ -601,6 +601,48 @@ __global__ void __launch_bounds__(G) k_reduce_buckets(const Xyzz29* sums, uint32
   }
 }
 
+__device__ __forceinline__ Fq29 shfl_pair(const Fq29& v) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < f29::L; ++i) r.l[i] = __shfl_xor(v.l[i], 1, 64);
+  return r;
+}
+
+// The same level-1 reduction with two lanes per bucket (f29::add_pair): for
+// small batches, whose wavefronts leave most SIMDs idle, each of the
+// 2 log2 G dependent additions takes the per-lane share of its products.
+template <unsigned G>
+__global__ void __launch_bounds__(2 * G) k_reduce_buckets_pair(const Xyzz29* sums,
+                                                               uint32_t buckets, G1Xyzz* pair_a,
+                                                               G1Xyzz* pair_r) {
+  __shared__ Xyzz29 u[G];
+  const uint32_t t = threadIdx.x >> 1, h = threadIdx.x & 1u;  // bucket in the group, lane of the pair
+  const uint32_t b = blockIdx.x * G + t;
+  auto ex = [](const Fq29& v) { return shfl_pair(v); };
+  Xyzz29 v = b < buckets ? load_xyzz29(sums, b) : Xyzz29::identity();
+  if (h == 0) u[t] = v;
+  __syncthreads();
+  for (unsigned d = 1; d < G; d <<= 1) {
+    if (t + d < G) v = f29::add_pair(v, u[t + d], h, ex);  // uniform within a pair
+    __syncthreads();
+    if (h == 0) u[t] = v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) store_xyzz(pair_r, blockIdx.x, f29::to_xyzz(v));
+  // tree sum of U
+  for (unsigned d = G / 2; d > 0; d >>= 1) {
+    if (t < d) {
+      v = f29::add_pair(v, u[t + d], h, ex);
+      if (h == 0) u[t] = v;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    store_xyzz(pair_a, blockIdx.x, f29::to_xyzz(v));
+    __threadfence_system();  // the results may be pinned host memory read after the sync
+  }
+}
+
 // Prepared bases: 32-bit-limb Montgomery -> packed Fq29 (in place) and back.
 __global__ void k_bases_to29(G1Affine* pts, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -1184,8 +1226,8 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   const size_t groups = G / rg;
   mark(5);
   if (rg == kRedSmall)
-    hipLaunchKernelGGL(k_reduce_buckets<kRedSmall>, dim3((unsigned)groups), dim3(kRedSmall), 0,
-                       c.stream, sums, G, result, result + groups);
+    hipLaunchKernelGGL(k_reduce_buckets_pair<kRedSmall>, dim3((unsigned)groups),
+                       dim3(2 * kRedSmall), 0, c.stream, sums, G, result, result + groups);
   else
     hipLaunchKernelGGL(k_reduce_buckets<kRed1>, dim3((unsigned)groups), dim3(kRed1), 0, c.stream,
                        sums, G, result, result + groups);

@@ -13,12 +13,17 @@
 //     operands against the Fr product, reduce_est, canonical, is_zero_mod,
 //     inverse_fr against the 32-bit-limb inverse;
 //   * the host finish's Fq inverse (host_g1.hpp, binary extended Euclid)
-//     against the 32-bit-limb inverse, including 0, 1 and -1.
+//     against the 32-bit-limb inverse, including 0, 1 and -1;
+//   * the two-lane addition add_pair, its lanes played by two host threads,
+//     limb for limb against add() (general, P + P, P + (-P), identities).
 // Prints "FIELD29 PASS" or the failing checks.
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <condition_variable>
+#include <mutex>
 #include <random>
+#include <thread>
 
 #include "../../hyperplonk_amd/csrc/curve29.hpp"
 #include "../../hyperplonk_amd/csrc/host_g1.hpp"
@@ -75,6 +80,62 @@ static bool same_point(const G1Xyzz& a, const G1Xyzz& b) {
 }
 
 static Aff29 aff29(const G1Affine& a) { return {to29(a.x), to29(a.y)}; }
+
+// Two host threads stand for the two lanes of add_pair: an exchange publishes
+// the lane's value, waits for the partner's, and waits again before the slot
+// is reused.
+struct PairExchange {
+  std::mutex mu;
+  std::condition_variable cv;
+  Fq29 slot[2];
+  unsigned arrived = 0, generation = 0;
+  void barrier() {
+    std::unique_lock<std::mutex> lk(mu);
+    const unsigned gen = generation;
+    if (++arrived == 2) {
+      arrived = 0;
+      ++generation;
+      cv.notify_all();
+    } else {
+      cv.wait(lk, [&] { return generation != gen; });
+    }
+  }
+  Fq29 swap(unsigned half, const Fq29& v) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      slot[half] = v;
+    }
+    barrier();
+    Fq29 other;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      other = slot[half ^ 1u];
+    }
+    barrier();
+    return other;
+  }
+};
+
+static bool same_limbs(const Xyzz29& a, const Xyzz29& b) {
+  for (int i = 0; i < L; ++i)
+    if (a.X.l[i] != b.X.l[i] || a.Y.l[i] != b.Y.l[i] || a.ZZ.l[i] != b.ZZ.l[i] ||
+        a.ZZZ.l[i] != b.ZZZ.l[i])
+      return false;
+  return true;
+}
+
+// add_pair on two threads: both lanes must return add(p, q) limb for limb.
+static bool pair_add_matches(const Xyzz29& p, const Xyzz29& q) {
+  PairExchange ex;
+  Xyzz29 out[2];
+  std::thread t1([&] {
+    out[1] = add_pair(p, q, 1u, [&](const Fq29& v) { return ex.swap(1u, v); });
+  });
+  out[0] = add_pair(p, q, 0u, [&](const Fq29& v) { return ex.swap(0u, v); });
+  t1.join();
+  const Xyzz29 want = f29::add(p, q);
+  return same_limbs(out[0], want) && same_limbs(out[1], want);
+}
 
 int main() {
   // field
@@ -230,6 +291,14 @@ int main() {
     Xyzz29 nacc = acc;
     nacc.Y = sub4p(zero(), acc.Y);
     CHECK(f29::add(acc, nacc).is_identity() || ref.is_identity());
+    // the two-lane addition: general, doubling, cancelling and identity cases
+    if (it < 60) {
+      CHECK(pair_add_matches(acc, b29));
+      CHECK(pair_add_matches(acc, acc));
+      CHECK(pair_add_matches(acc, nacc));
+      CHECK(pair_add_matches(acc, Xyzz29::identity()));
+      CHECK(pair_add_matches(Xyzz29::identity(), b29));
+    }
   }
   // ---- Fr29: prescaled operands keep products in the stored form
   auto rand_fr = [] {
