@@ -445,17 +445,30 @@ __device__ __forceinline__ Xyzz29 shfl_xor_xyzz(const Xyzz29& v, unsigned mask) 
 // job: larger MSMs have more partials per bucket): each lane adds every
 // group-th partial, then a butterfly over the group.  Buckets with more than
 // seq_max partials per lane (outliers, several times the job's mean) are
-// appended to `heavy` instead and summed by the heavy kernels.
+// appended to `heavy` instead and summed by the heavy kernels.  kPair: every
+// lane of the group is a pair of threads adding with f29::add_pair (small
+// batches, whose wavefronts leave SIMDs idle; groups of <= 32 pairs, so a
+// group stays inside one wavefront); total_threads counts lanes.
 constexpr uint32_t kSeqMax = 8;
+__device__ __forceinline__ Fq29 shfl_pair(const Fq29& v) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < f29::L; ++i) r.l[i] = __shfl_xor(v.l[i], 1, 64);
+  return r;
+}
+
+template <bool kPair>
 __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
                                                      const uint32_t* ntask,
                                                      const Xyzz29* partials, const Job* jobs,
                                                      uint32_t njobs, uint32_t total_threads,
                                                      Xyzz29* sums, uint32_t* heavy,
                                                      uint32_t* heavy_count) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  // job owning this thread (last with sum_base <= gid); wave-uniform since
-  // every job's thread range is a multiple of 64 lanes
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t gid = kPair ? tid >> 1 : tid, h = kPair ? tid & 1u : 0u;
+  auto ex = [](const Fq29& v) { return shfl_pair(v); };
+  // job owning this lane (last with sum_base <= gid); wave-uniform since
+  // every job's lane range is a multiple of 64 lanes
   uint32_t lo = 0, hi = njobs;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) / 2;
@@ -472,18 +485,20 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
   if (live) {
     const uint32_t first = task_off[b], cnt = ntask[b];
     if (cnt > job.seq_max * group) {
-      if (lane == 0) heavy[atomicAdd(heavy_count, 1u)] = b;
+      if (lane == 0 && h == 0) heavy[atomicAdd(heavy_count, 1u)] = b;
       live = false;
     } else {
-      for (uint32_t k = lane; k < cnt; k += group)
-        s = f29::add(s, load_xyzz29(partials, first + k));
+      for (uint32_t k = lane; k < cnt; k += group) {
+        const Xyzz29 x = load_xyzz29(partials, first + k);
+        s = kPair ? f29::add_pair(s, x, h, ex) : f29::add(s, x);
+      }
     }
   }
   for (uint32_t m = 1; m < group; m <<= 1) {
-    const Xyzz29 o = shfl_xor_xyzz(s, m);
-    s = f29::add(s, o);
+    const Xyzz29 o = shfl_xor_xyzz(s, kPair ? 2 * m : m);
+    s = kPair ? f29::add_pair(s, o, h, ex) : f29::add(s, o);
   }
-  if (live && lane == 0) store_xyzz29(sums, b, s);
+  if (live && lane == 0 && h == 0) store_xyzz29(sums, b, s);
 }
 
 // Heavy buckets (many partials).  The first kHeavySplitSlots of the heavy
@@ -599,13 +614,6 @@ __global__ void __launch_bounds__(G) k_reduce_buckets(const Xyzz29* sums, uint32
     store_xyzz(pair_a, blockIdx.x, f29::to_xyzz(v));
     __threadfence_system();  // the results may be pinned host memory read after the sync
   }
-}
-
-__device__ __forceinline__ Fq29 shfl_pair(const Fq29& v) {
-  Fq29 r;
-#pragma unroll
-  for (int i = 0; i < f29::L; ++i) r.l[i] = __shfl_xor(v.l[i], 1, 64);
-  return r;
 }
 
 // The same level-1 reduction with two lanes per bucket (f29::add_pair): for
@@ -1205,9 +1213,19 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   uint32_t* heavy_count = static_cast<uint32_t*>(heavy_buf->ptr);
   uint32_t* heavy = heavy_count + 1;
   zero_u32(c, heavy_count, 1);
-  hipLaunchKernelGGL(k_bucket_sums, dim3((unsigned)((sum_threads_total + 255) / 256)), dim3(256), 0,
-                     c.stream, task_off, ntask, partials, djobs, njobs, (uint32_t)sum_threads_total,
-                     sums, heavy, heavy_count);
+  // pairs of threads per lane where the launch would leave SIMDs idle (at most
+  // two wavefronts per SIMD when doubled) and every lane group fits a
+  // wavefront as pairs
+  bool pair = 2 * sum_threads_total <= (size_t(1) << 17);
+  for (uint32_t j = 0; j < njobs; ++j) pair = pair && jobs[j].lanes_log2 <= 5;
+  if (pair)
+    hipLaunchKernelGGL(k_bucket_sums<true>, dim3((unsigned)((2 * sum_threads_total + 255) / 256)),
+                       dim3(256), 0, c.stream, task_off, ntask, partials, djobs, njobs,
+                       (uint32_t)sum_threads_total, sums, heavy, heavy_count);
+  else
+    hipLaunchKernelGGL(k_bucket_sums<false>, dim3((unsigned)((sum_threads_total + 255) / 256)),
+                       dim3(256), 0, c.stream, task_off, ntask, partials, djobs, njobs,
+                       (uint32_t)sum_threads_total, sums, heavy, heavy_count);
   rt::check_launch();
   rt::BufferRef heavy_parts = c.allocate((size_t)kHeavySplitSlots * kHeavySplit * sizeof(Xyzz29));
   Xyzz29* block_sums = static_cast<Xyzz29*>(heavy_parts->ptr);
