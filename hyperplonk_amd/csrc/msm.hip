@@ -586,6 +586,7 @@ __global__ void __launch_bounds__(kHeavyThreads)
 // 64 (12 dependent additions instead of 16, more groups for the host).
 constexpr unsigned kRed1 = 256;
 constexpr unsigned kRedSmall = 64;
+constexpr uint32_t kPairReduceMax = 65536;  // buckets of a batch reduced on two lanes each
 template <unsigned G>
 __global__ void __launch_bounds__(G) k_reduce_buckets(const Xyzz29* sums, uint32_t buckets,
                                                       G1Xyzz* pair_a, G1Xyzz* pair_r) {
@@ -1243,9 +1244,14 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   const unsigned rg = red_group();
   const size_t groups = G / rg;
   mark(5);
+  // two lanes per bucket while the doubled launch stays within two wavefronts
+  // per SIMD (the one-lane launch leaves SIMDs idle or is latency-bound)
   if (rg == kRedSmall)
     hipLaunchKernelGGL(k_reduce_buckets_pair<kRedSmall>, dim3((unsigned)groups),
                        dim3(2 * kRedSmall), 0, c.stream, sums, G, result, result + groups);
+  else if (G <= kPairReduceMax)
+    hipLaunchKernelGGL(k_reduce_buckets_pair<kRed1>, dim3((unsigned)groups), dim3(2 * kRed1), 0,
+                       c.stream, sums, G, result, result + groups);
   else
     hipLaunchKernelGGL(k_reduce_buckets<kRed1>, dim3((unsigned)groups), dim3(kRed1), 0, c.stream,
                        sums, G, result, result + groups);
