@@ -48,7 +48,7 @@ FQM_PER_MIXED_ADD = 10
 # FETCH_SIZE and WRITE_SIZE passes calibrated on the in-place conversion of
 # 2^20 points (96 MiB read and written; profiles/k_accumulate_msm_2_20_pmc.md);
 # rocprofv3 counters cannot be read from inside this process.
-ACCUMULATE_TRAFFIC_2_20 = 3.877e9
+ACCUMULATE_TRAFFIC_2_20 = 3.878e9
 
 
 def parse():
