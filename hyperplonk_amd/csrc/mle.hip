@@ -79,10 +79,21 @@ __global__ void k_fold_tail(const Fr* src, const Fr* point, size_t levels, Fr* o
 
 namespace {
 
-// eq table of one chunk of k <= 10 coordinates r[0..k):
+struct EqChunks {
+  const Fr* table[4];
+  uint32_t shift[4];
+  uint32_t bits[4];
+  uint32_t count;
+};
+
+// eq table of chunk blockIdx.x (k <= 10 coordinates r[shift .. shift + k)):
 // t[x] = prod_{i<k} (bit_i(x) ? r_i : 1 - r_i), built in LDS by prepending one
 // coordinate per level from the highest to the lowest (eq.rs:67-95 order).
-__global__ void k_eq_chunk(const Fr* r, size_t k, Fr* table) {
+// The chunks are independent: one launch, one block each.
+__global__ void k_eq_chunk(const Fr* r_all, EqChunks chunks) {
+  const Fr* r = r_all + chunks.shift[blockIdx.x];
+  const size_t k = chunks.bits[blockIdx.x];
+  Fr* table = const_cast<Fr*>(chunks.table[blockIdx.x]);
   __shared__ Fr buf[1024];
   if (threadIdx.x == 0) buf[0] = Fr::one();
   __syncthreads();
@@ -110,13 +121,6 @@ __global__ void k_eq_chunk(const Fr* r, size_t k, Fr* table) {
   }
   for (size_t i = threadIdx.x; i < len; i += blockDim.x) store_fr(table, i, buf[i]);
 }
-
-struct EqChunks {
-  const Fr* table[4];
-  uint32_t shift[4];
-  uint32_t bits[4];
-  uint32_t count;
-};
 
 // out[x] = prod_c chunk_c[(x >> shift_c) & mask_c]
 __global__ void k_eq_outer(EqChunks chunks, Fr* out, size_t n) {
@@ -341,14 +345,15 @@ void build_eq(rt::Context& c, const Fr* r, size_t nv, Fr* out) {
   Fr* t = static_cast<Fr*>(tables->ptr);
   off = 0;
   for (size_t k = 0; k < chunks; ++k) {
-    hipLaunchKernelGGL(k_eq_chunk, dim3(1), dim3(256), 0, c.stream, r + off, bits[k], t);
-    rt::check_launch();
     eq.table[k] = t;
     eq.shift[k] = static_cast<uint32_t>(off);
     eq.bits[k] = static_cast<uint32_t>(bits[k]);
     t += size_t(1) << bits[k];
     off += bits[k];
   }
+  if (chunks == 0) rt::fail_params();  // callers pass nv >= 1
+  hipLaunchKernelGGL(k_eq_chunk, dim3((unsigned)chunks), dim3(256), 0, c.stream, r, eq);
+  rt::check_launch();
   const size_t n = size_t(1) << nv;
   hipLaunchKernelGGL(k_eq_outer, dim3(grid(n)), dim3(kThreads), 0, c.stream, eq, out, n);
   rt::check_launch();

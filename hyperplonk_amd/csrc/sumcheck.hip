@@ -240,10 +240,13 @@ __device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1
   __shared__ Fr red[kThreads];
   red[threadIdx.x] = s;
   __syncthreads();
+  for (uint32_t hop = 1; hop < groups; hop <<= 1) {
+    if (g < groups && (g & (2 * hop - 1)) == 0 && g + hop < groups)
+      red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + hop * width];
+    __syncthreads();
+  }
   if (threadIdx.x < width) {
-    Fr total = Fr::zero();
-    for (uint32_t k = 0; k < groups; ++k) total = total + red[k * width + threadIdx.x];
-    store_fr(job.out, threadIdx.x, from_mont(total));
+    store_fr(job.out, threadIdx.x, from_mont(red[threadIdx.x]));
     __threadfence_system();  // `out` may be pinned host memory read after the sync
   }
   if (threadIdx.x == 0) *job.ticket = 0;
