@@ -345,6 +345,36 @@ __global__ void k_scan_add(uint32_t* out, size_t n, const uint32_t* tile_sums, u
   if (i == 0 && total_out) *total_out = tile_sums[tiles];
 }
 
+// Exclusive scan of n <= kSmallScan values by one block (kSmallScan /
+// kScanBlock consecutive values per thread), grand total to *total_out.
+constexpr unsigned kScanBlock = 1024, kSmallPer = 16, kSmallScan = kScanBlock * kSmallPer;
+__global__ void __launch_bounds__(kScanBlock)
+    k_scan_small(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_out) {
+  __shared__ uint32_t sh[kScanBlock];
+  const size_t base = (size_t)threadIdx.x * kSmallPer;
+  uint32_t v[kSmallPer], sum = 0;
+#pragma unroll
+  for (unsigned j = 0; j < kSmallPer; ++j) {
+    v[j] = base + j < n ? in[base + j] : 0u;
+    sum += v[j];
+  }
+  sh[threadIdx.x] = sum;
+  __syncthreads();
+  for (unsigned d = 1; d < kScanBlock; d <<= 1) {
+    const uint32_t add = threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint32_t run = sh[threadIdx.x] - sum;
+#pragma unroll
+  for (unsigned j = 0; j < kSmallPer; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+  if (threadIdx.x == kScanBlock - 1 && total_out) *total_out = sh[threadIdx.x];
+}
+
 // ---- tasks ----------------------------------------------------------------
 __global__ void k_task_counts(const uint32_t* counts, uint32_t buckets, uint32_t cap,
                               uint32_t* ntask) {
@@ -889,10 +919,17 @@ unsigned grid_for(size_t n, unsigned threads = 256, size_t cap = 65535) {
   return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, cap)));
 }
 
-// Exclusive scan of n uint32 values; optional grand total.  Tile sums are
-// scanned recursively when there are more than one tile's worth of tiles.
+// Exclusive scan of n uint32 values; optional grand total.  Up to
+// kSmallScan values (the sort partitions of a batch, the buckets of small
+// batches) in one launch of one block; beyond, tile sums are scanned
+// recursively when there are more than one tile's worth of tiles.
 void exclusive_scan(rt::Context& c, const uint32_t* in, uint32_t* out, size_t n,
                     uint32_t* total_out, std::vector<rt::BufferRef>& keep) {
+  if (n <= kSmallScan) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kScanBlock), 0, c.stream, in, out, n, total_out);
+    rt::check_launch();
+    return;
+  }
   const size_t tiles = (n + kScanTile - 1) / kScanTile;
   rt::BufferRef sums_buf = c.allocate((tiles + 1) * sizeof(uint32_t));
   keep.push_back(sums_buf);
