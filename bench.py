@@ -144,7 +144,8 @@ def main():
     if args.gpus != 1:
         print(
             "bench.py measures one GPU only: this build runs on a single MI355X and has no "
-            "multi-GPU path (see DESIGN.md for how the MSM would shard).",
+            "multi-GPU path (an MSM would shard by splitting its points across GPUs and "
+            "adding the per-GPU results).",
             file=sys.stderr,
         )
         sys.exit(2)
