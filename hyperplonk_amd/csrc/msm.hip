@@ -1045,6 +1045,10 @@ Plan choose_plan(size_t n, uint32_t override_c) {
 }
 
 // Entries per lane task by the largest MSM of a batch (autotuner sweep).
+// Bucket sums run on two lanes per addition up to this many lanes (two
+// wavefronts per SIMD); beyond it the one-lane launch issues as fast.
+constexpr size_t kPairLanes = size_t(1) << 17;
+
 uint32_t default_cap(size_t largest_n) {
   if (largest_n <= (size_t(1) << 12)) return 2;
   if (largest_n < (size_t(1) << 14)) return 4;
@@ -1192,6 +1196,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   std::vector<Job> jobs(njobs);
   std::vector<SortItem> items;
   size_t sum_threads_total = 0, fold_total = 0;
+  std::vector<double> per_bucket(njobs);
   for (uint32_t j = 0; j < njobs; ++j) {
     const Entry& e = entries_[j];
     const Plan& p = e.layer->plan;
@@ -1202,17 +1207,32 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
     // bucket-sum shape of this MSM as if alone: lanes per bucket while each
     // lane keeps >= 2 partials and the job's threads stay <= 131072; heavy
     // threshold three times its mean partials per lane
-    const double per_bucket = ((double)e.layer->n * p.windows / cap + 1.0) / p.buckets;
+    per_bucket[j] = ((double)e.layer->n * p.windows / cap + 1.0) / p.buckets;
     uint32_t l = 0;
-    while (l < 6 && ((size_t)p.buckets << (l + 1)) <= 131072 && (double)(2u << l) < per_bucket) ++l;
+    while (l < 6 && ((size_t)p.buckets << (l + 1)) <= 131072 && (double)(2u << l) < per_bucket[j])
+      ++l;
     jobs[j].lanes_log2 = l;
-    jobs[j].seq_max =
-        std::max<uint32_t>(kSeqMax, (uint32_t)std::ceil(3.0 * per_bucket / (double)(1u << l)));
     jobs[j].padded_buckets = (p.buckets + kRed1 - 1) / kRed1 * kRed1;
     jobs[j].part_bits = p.c > 9 ? p.c - 9 : 0;  // 256 partitions of the padded range
+    for (uint32_t ch = 0; ch < e.chunks; ++ch) items.push_back({j, ch});
+  }
+  // At most 32 lanes per bucket when that brings the two-lane launch within
+  // two wavefronts per SIMD: one more dependent addition per bucket, each at
+  // the latency of a two-lane addition instead of a one-lane one.
+  auto sum_threads = [&](uint32_t max_l) {
+    size_t t = 0;
+    for (uint32_t j = 0; j < njobs; ++j)
+      t += (((size_t)jobs[j].padded_buckets << std::min(jobs[j].lanes_log2, max_l)) + 63) / 64 * 64;
+    return t;
+  };
+  const uint32_t max_l = 2 * sum_threads(6) > kPairLanes && 2 * sum_threads(5) <= kPairLanes ? 5 : 6;
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const uint32_t l = std::min(jobs[j].lanes_log2, max_l);
+    jobs[j].lanes_log2 = l;
+    jobs[j].seq_max =
+        std::max<uint32_t>(kSeqMax, (uint32_t)std::ceil(3.0 * per_bucket[j] / (double)(1u << l)));
     jobs[j].sum_base = sum_threads_total;
     sum_threads_total += (((size_t)jobs[j].padded_buckets << l) + 63) / 64 * 64;
-    for (uint32_t ch = 0; ch < e.chunks; ++ch) items.push_back({j, ch});
   }
   const size_t job_bytes = njobs * sizeof(Job), item_bytes = items.size() * sizeof(SortItem);
   rt::BufferRef table = c.allocate(job_bytes + item_bytes);
@@ -1303,7 +1323,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   // pairs of threads per lane where the launch would leave SIMDs idle (at most
   // two wavefronts per SIMD when doubled) and every lane group fits a
   // wavefront as pairs
-  bool pair = 2 * sum_threads_total <= (size_t(1) << 17);
+  bool pair = 2 * sum_threads_total <= kPairLanes;
   for (uint32_t j = 0; j < njobs; ++j) pair = pair && jobs[j].lanes_log2 <= 5;
   if (pair)
     hipLaunchKernelGGL(k_bucket_sums<true>, dim3((unsigned)((2 * sum_threads_total + 255) / 256)),
