@@ -1044,13 +1044,14 @@ Plan choose_plan(size_t n, uint32_t override_c) {
   return p;
 }
 
-// Entries per lane task by the largest MSM of a batch (autotuner sweep).
 // Bucket sums run on two lanes per addition up to this many lanes (two
 // wavefronts per SIMD); beyond it the one-lane launch issues as fast.
 constexpr size_t kPairLanes = size_t(1) << 17;
 
+// Entries per lane task by the largest MSM of a batch (autotuner sweep; for
+// batches of 2^12-point MSMs, 4 beats 2 and 8 in whole proofs: the shorter
+// bucket-sum chains outweigh the longer lane tasks).
 uint32_t default_cap(size_t largest_n) {
-  if (largest_n <= (size_t(1) << 12)) return 2;
   if (largest_n < (size_t(1) << 14)) return 4;
   if (largest_n < (size_t(1) << 18)) return 8;
   if (largest_n < (size_t(1) << 20)) return 16;
@@ -1185,7 +1186,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
   };
   const uint32_t njobs = (uint32_t)entries_.size();
   if (njobs == 0) return;
-  // lane tasks: entries per lane by the largest MSM (2 .. 16), or the
+  // lane tasks: entries per lane by the largest MSM (4 .. 32), or the
   // context's tuned cap, or an explicit one
   uint32_t cap = default_cap(largest_n_);
   if (cap_override_)
