@@ -44,11 +44,14 @@ VALU_MUL_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 LIMB_PRODUCTS_PER_FQM = 300
 # XYZZ mixed addition = 8M + 2S Fq products.
 FQM_PER_MIXED_ADD = 10
-# HBM bytes of one 2^20 k_accumulate launch (c = 17, task cap 32), from the
-# FETCH_SIZE and WRITE_SIZE passes calibrated on the in-place conversion of
-# 2^20 points (96 MiB read and written; profiles/k_accumulate_msm_2_20_pmc.md);
-# rocprofv3 counters cannot be read from inside this process.
-ACCUMULATE_TRAFFIC_2_20 = 3.878e9
+# HBM bytes of one 2^20 k_accumulate launch by (window bits, task cap), from
+# the FETCH_SIZE and WRITE_SIZE passes calibrated on the in-place conversion
+# of 2^20 points (96 MiB read and written; profiles/k_accumulate_msm_2_20_pmc.md);
+# rocprofv3 counters cannot be read from inside this process.  The autotuner
+# picks c = 17 with cap 16 or 32 (within 1 % of each other) on these boxes.
+ACCUMULATE_TRAFFIC_2_20 = {(17, 32): 3.878e9, (17, 16): 3.622e9}
+# task cap of the built-in plan at 2^20
+DEFAULT_CAP_2_20 = 32
 
 
 def parse():
@@ -63,6 +66,8 @@ def parse():
     ap.add_argument("--extra-sizes", default="16,18")
     ap.add_argument("--skip-workloads", action="store_true", help="MSM headline only")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in window rule, no autotuning")
+    ap.add_argument("--cap", type=int, default=0,
+                    help="lane task cap of the headline size with the built-in window rule (no autotuning)")
     ap.add_argument("--serve-full", action="store_true",
                     help="serve the full 2000-request trace (nv 10..20) instead of the short one")
     return ap.parse_args()
@@ -158,12 +163,15 @@ def main():
     be = Mi355xBackend(0)
     rng = SplitMix64Stream(0xB1512381)
     tuning = None
-    if not args.no_tune:
+    if args.cap:
+        be.msm_set_tuning([{"log_n": log_n, "window": 0, "cap": args.cap}])
+    elif not args.no_tune:
         # window bits and task cap for the headline size, timed on this device
         s = time.time()
         tuning = be.msm_autotune([log_n])[0]
         tuning["seconds"] = round(time.time() - s, 2)
         print(f"[bench] autotune 2^{log_n}: {tuning}", file=sys.stderr, flush=True)
+    cap_used = tuning["cap"] if tuning else (args.cap or (DEFAULT_CAP_2_20 if log_n == 20 else None))
     t = fr_from_bytes(rng.fr_bytes(log_n).tobytes())
     t0 = time.time()
     pp = be.generate_prover_param(log_n, t)
@@ -251,6 +259,7 @@ def main():
             f"device-generated prepared SRS (N={log_n}, seeded trapdoor), affine result",
             "msm_log_size": log_n,
             "window_bits": plan["c"],
+            "task_cap": cap_used,
             "windows": plan["windows"],
             "buckets": plan["buckets"],
             "nonzero_digits": plan["entries"],
@@ -264,7 +273,7 @@ def main():
             "peak": round(VALU_MUL_PEAK_T, 2),
             "unit": "T u32-limb-products/s",
             "frac": round(achieved / VALU_MUL_PEAK_T, 4),
-            "traffic": ACCUMULATE_TRAFFIC_2_20 if log_n == 20 else None,
+            "traffic": (ACCUMULATE_TRAFFIC_2_20.get((plan["c"], cap_used)) if log_n == 20 else None),
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/k_accumulate_msm_2_20_pmc.md",
             "kernel_ms": round(acc_ms, 4),
             "stage_ms": {k: round(v, 4) for k, v in stages.items()},
