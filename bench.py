@@ -37,13 +37,18 @@ sys.path.insert(0, ROOT)
 
 # Reference-published G1 MSM 2^20 (prepared SRS, Ascend910B3): 1.113355917 s.
 REFERENCE_MSM_2_20_SECONDS = 1.113355917
-# gfx950: 256 CUs x 64 lanes x 2.4 GHz, one v_mad_u64_u32 (u32 x u32 -> u64 + u64)
-# per lane per cycle.
+# MI355X published figures: 256 CUs at up to 2.4 GHz, each CU issuing 64 lanes of
+# one full-rate VALU operation per cycle, so 256 x 64 x 2.4e9 u32 x u32 -> u64
+# multiply-adds (v_mad_u64_u32) per second; HBM3E 8 TB/s.  The line also carries
+# both rates measured on the device (copy kernel, multiply-add chain kernel).
 VALU_MUL_PEAK_T = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_BYTES = 8e12
 # Fq Montgomery product on 12 x 32-bit limbs: 2 * 12^2 + 12 limb products.
 LIMB_PRODUCTS_PER_FQM = 300
-# XYZZ mixed addition = 8M + 2S Fq products.
-FQM_PER_MIXED_ADD = 10
+# Fixed work convention for one (point, window): a mixed addition counted as
+# 7M + 4S = 11 Fq products (the executed XYZZ formula takes 8M + 2S, so this
+# convention credits the kernel with 10 % more work than it does).
+FQM_PER_MIXED_ADD = 11
 # HBM bytes of one 2^20 k_accumulate launch by (window bits, task cap), from
 # the FETCH_SIZE and WRITE_SIZE passes calibrated on the in-place conversion
 # of 2^20 points (96 MiB read and written; profiles/k_accumulate_msm_2_20_pmc.md);
@@ -91,7 +96,7 @@ def run_json(cmd, timeout):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
-def workloads(args, be, rng):
+def workloads(args, be, rng, peaks):
     out = {}
     # fold (fix_variables by the lowest variable) of a 2^24 table: 2 reads + 1 write of 32 B per output
     from hyperplonk_amd.fields import fr_from_bytes
@@ -105,8 +110,10 @@ def workloads(args, be, rng):
         "ms": round(ms, 4),
         "bytes": moved,
         "hbm_GB_per_s": round(moved / (ms * 1e-3) / 1e9, 1),
-        "hbm_peak_GB_per_s": 8000,
-        "frac": round(moved / (ms * 1e-3) / 8e12, 4),
+        "hbm_peak_GB_per_s": HBM_PEAK_BYTES / 1e9,
+        "frac": round(moved / (ms * 1e-3) / HBM_PEAK_BYTES, 4),
+        "hbm_copy_GB_per_s_measured": round(peaks["hbm_bytes_per_s"] / 1e9, 1),
+        "frac_of_measured_copy": round(moved / (ms * 1e-3) / peaks["hbm_bytes_per_s"], 4),
     }
     del table
     if not os.path.exists(PROVE_BENCH):
@@ -117,7 +124,7 @@ def workloads(args, be, rng):
         "jellyfish_nv12": run_json([PROVE_BENCH, "prove", "12", "jellyfish", "20", "3", "2"], 300),
     }
     if os.path.exists(PROTOCOL_TEST):
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         cpu = run_json([PROTOCOL_TEST, "cpu-prove", "12", "vanilla", "3", str(threads)], 600)
         cpu["kind"] = "port"
         cpu["sample"] = "3 proofs of the nv-12 vanilla circuit on the CPU oracle backend (MSMs threaded)"
@@ -135,6 +142,33 @@ def workloads(args, be, rng):
     else:
         out["serving"] = run_json([PROVE_BENCH, "serve", "300", "auto", "10", "16", "0.3", "16", "5"], 600)
     return out
+
+
+def host_threads() -> int:
+    """Worker threads for the CPU baselines: the cores this process may run on,
+    at most 16 (one GPU's share of the host on these machines)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def host_identity() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cores": avail}
 
 
 def reference_conventional_fqm(log_n: int) -> float:
@@ -202,6 +236,10 @@ def main():
     acc_ms = stages["accumulate"]
     acc_products = plan["entries"] * FQM_PER_MIXED_ADD * LIMB_PRODUCTS_PER_FQM
     achieved = acc_products / (acc_ms * 1e-3) / 1e12
+    # roofline denominators measured on this device
+    peaks = be.device_peaks()
+    print(f"[bench] measured peaks: {peaks}", file=sys.stderr, flush=True)
+    mad_peak_t = peaks["mad_per_s"] / 1e12
     conventional = reference_conventional_fqm(log_n) * LIMB_PRODUCTS_PER_FQM
     whole_msm_rate = conventional / (total / args.steps) / 1e12
 
@@ -223,7 +261,7 @@ def main():
         from oracle import oracle
 
         lg = args.cpu_sample_log
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         layer = be.download_layer(pp, log_n - lg)
         sample = rng.fr_bytes(1 << lg)
         s = time.perf_counter()
@@ -237,9 +275,10 @@ def main():
             "kind": "port",
             "sample": f"one G1 MSM of 2^{lg} points (SRS layer {log_n - lg}), oracle Pippenger, "
             f"{threads} threads, {dt:.2f} s; GPU result byte-identical: {got == want}",
+            "host": host_identity(),
         }
 
-    extra_workloads = None if args.skip_workloads else workloads(args, be, rng)
+    extra_workloads = None if args.skip_workloads else workloads(args, be, rng, peaks)
 
     line = {
         "metric": "MSM pts/s at 2^20",
@@ -273,6 +312,12 @@ def main():
             "peak": round(VALU_MUL_PEAK_T, 2),
             "unit": "T u32-limb-products/s",
             "frac": round(achieved / VALU_MUL_PEAK_T, 4),
+            "work_convention": f"non-zero digits x {FQM_PER_MIXED_ADD} Fq products (7M + 4S mixed addition) "
+            f"x {LIMB_PRODUCTS_PER_FQM} u32 limb products",
+            "peak_measured": round(mad_peak_t, 2),
+            "frac_of_measured": round(achieved / mad_peak_t, 4),
+            "peak_measured_kernel": "k_mad_u64: 16 independent v_mad_u64_u32 chains per lane on every CU",
+            "hbm_copy_bytes_per_s_measured": round(peaks["hbm_bytes_per_s"], 1),
             "traffic": (ACCUMULATE_TRAFFIC_2_20.get((plan["c"], cap_used)) if log_n == 20 else None),
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/k_accumulate_msm_2_20_pmc.md",
             "kernel_ms": round(acc_ms, 4),

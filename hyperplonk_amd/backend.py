@@ -218,6 +218,14 @@ class Mi355xBackend:
                                                      _fr_scalars([r]), reps, ctypes.byref(ms)), "fold profile")
         return ms.value
 
+    def device_peaks(self, copy_bytes: int = 1 << 30, reps: int = 5) -> dict:
+        """Extension: HBM copy rate (read + write bytes/s) and u32 x u32 -> u64
+        multiply-add rate measured on this device (roofline denominators)."""
+        hbm, mad = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.hp_mi355x_v1_device_peaks(self.context, copy_bytes, reps, ctypes.byref(hbm), ctypes.byref(mad)),
+              "device_peaks")
+        return {"hbm_bytes_per_s": hbm.value, "mad_per_s": mad.value, "copy_bytes": copy_bytes}
+
     def build_eq_x_r(self, r: Sequence[int]) -> Mle:
         if not r:
             raise InvalidParameters("r length is 0")

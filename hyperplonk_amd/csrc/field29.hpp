@@ -299,7 +299,8 @@ HP_HD F29<P> canonical(const F29<P>& a) {
 }
 
 // a - k p for k = floor(a / p) - 1 (estimated from the top 58 bits), clamped
-// at 0: a representative below 3 p of a value below 2^(29 L - 1)
+// at 0: a representative below 3 p of a value below 2^(29 L) with normalised
+// limbs (for Fr, values up to 70 r; the sumcheck accumulators stay below 41 r)
 template <class P>
 HP_HD F29<P> reduce_est(const F29<P>& a) {
   constexpr int L = P::L;

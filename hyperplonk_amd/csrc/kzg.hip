@@ -59,7 +59,9 @@ using namespace hp;
 namespace {
 
 kzg::Params* bind(HpMi355xContext* ctx, rt::Context& c, const HpMi355xKzgParams* params) {
-  if (params == nullptr || params->ctx != ctx || !c.resources.count(params)) return nullptr;
+  // membership first: nothing is read through a handle this context does not hold
+  if (params == nullptr || !c.holds(params, rt::Context::Kind::kzg_params)) return nullptr;
+  if (params->ctx != ctx) return nullptr;
   return params->impl.get();
 }
 
@@ -90,7 +92,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_create(HpMi355xCon
       if (v >= 4 && v <= 20) p->window_override = static_cast<uint32_t>(v);
     }
     auto* handle = new HpMi355xKzgParams{ctx, p};
-    c.resources[handle] = p;
+    c.resources[handle] = {rt::Context::Kind::kzg_params, p};
     *out = handle;
     return rt::ok();
   });
@@ -193,7 +195,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer_com
 
 extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_destroy(HpMi355xContext* ctx,
                                                                       HpMi355xKzgParams* params) {
-  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+  return hp_guarded_release(ctx, [&](rt::Context& c) -> HpMi355xStatus {
     if (!bind(ctx, c, params)) return rt::invalid_parameters();
     c.resources.erase(params);
     delete params;

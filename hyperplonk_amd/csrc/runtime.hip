@@ -21,6 +21,8 @@
 namespace hp {
 namespace rt {
 
+thread_local Context* g_call_context = nullptr;
+
 namespace {
 std::mutex g_contexts_mutex;
 std::unordered_set<HpMi355xContext*> g_contexts;
@@ -390,14 +392,29 @@ extern "C" HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext* ctx) {
   if (c.results_host) (void)hipHostFree(c.results_host);
   if (c.pinned_free) (void)hipEventDestroy(c.pinned_free);
   (void)hipStreamDestroy(c.stream);
+  const bool poisoned = c.poisoned;
+  const int32_t first_error = c.first_error;
   delete ctx;
   if (drained != hipSuccess) return rt::invalid_state(static_cast<int32_t>(drained));
+  // everything is released; a context that failed earlier still reports its
+  // first error (bridge_runtime.cpp:1317)
+  if (poisoned) return rt::invalid_state(first_error);
   return rt::ok();
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_debug_inject_failure(HpMi355xContext* ctx, uint32_t after,
+                                                            int32_t native_code) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (after != 0 && native_code == 0) return rt::invalid_parameters();
+    c.inject_after = after;
+    c.inject_code = native_code;
+    return rt::ok();
+  });
 }
 
 extern "C" HpMi355xStatus hp_mi355x_v1_buffer_retire(HpMi355xContext* ctx,
                                                      HpMi355xDeviceAddress address) {
-  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+  return hp_guarded_release(ctx, [&](rt::Context& c) -> HpMi355xStatus {
     auto it = c.registry.find(address);
     if (address == nullptr || it == c.registry.end()) return rt::invalid_parameters();
     c.registry.erase(it);  // states holding references keep the memory alive

@@ -31,7 +31,9 @@ namespace rt {
 
 constexpr size_t kFrBytes = 32;
 constexpr size_t kG1Bytes = 97;
-constexpr size_t kMaxNumVars = 40;
+// Tables of at most 2^31 entries (num_vars < 32): device-side lengths, pair
+// counts and grid sizes are 32-bit.
+constexpr size_t kMaxNumVars = 32;
 
 inline HpMi355xStatus ok() { return {HP_MI355X_STATUS_SUCCESS, 0}; }
 inline HpMi355xStatus invalid_parameters(int32_t code = 0) {
@@ -56,9 +58,6 @@ struct Failure {
   throw Failure{invalid_state(static_cast<int32_t>(hipErrorOutOfMemory)), false};
 }
 
-inline void hip_check(hipError_t e) {
-  if (e != hipSuccess) fail_state(static_cast<int32_t>(e));
-}
 
 // Canonical little-endian field element check (value < modulus).
 bool canonical_fr(const uint8_t* bytes);
@@ -107,8 +106,20 @@ struct Context {
   hipStream_t stream = nullptr;
   std::unique_ptr<Allocator> alloc;
   std::unordered_map<void*, BufferRef> registry;  // live MLE handles
-  // Live prover states and KZG parameters (released before memory on destroy).
-  std::unordered_map<const void*, std::shared_ptr<void>> resources;
+  // Live prover states and KZG parameters (released before memory on destroy),
+  // keyed by handle address with the handle's kind, so that a handle is
+  // checked by lookup here before anything is read through it, and a freed
+  // handle whose address was reused by a handle of another kind is refused.
+  enum class Kind : uint8_t { kzg_params = 1, sumcheck_state = 2, internal = 3 };
+  struct Resource {
+    Kind kind;
+    std::shared_ptr<void> object;
+  };
+  std::unordered_map<const void*, Resource> resources;
+  bool holds(const void* handle, Kind kind) const {
+    auto it = resources.find(handle);
+    return it != resources.end() && it->second.kind == kind;
+  }
   bool poisoned = false;
   int32_t first_error = 0;
   // MSM tuning by log2 of the layer size (0 = the built-in rule): window bits
@@ -140,6 +151,28 @@ struct Context {
     if (!poisoned) first_error = code ? code : -1;
     poisoned = true;
   }
+  // Failure injection (hp_mi355x_v1_debug_inject_failure): the inject_after-th
+  // checked HIP call of this context from now on reports inject_code as its
+  // error, after the call itself completed normally on the device.
+  uint32_t inject_after = 0;
+  int32_t inject_code = 0;
+};
+
+// The context of the ABI call running on this thread (set by hp_guarded).
+extern thread_local Context* g_call_context;
+
+inline void hip_check(hipError_t e) {
+  Context* c = g_call_context;
+  if (e == hipSuccess && c != nullptr && c->inject_after != 0 && --c->inject_after == 0)
+    e = static_cast<hipError_t>(c->inject_code);
+  if (e != hipSuccess) fail_state(static_cast<int32_t>(e));
+}
+
+// Sets g_call_context for the lifetime of one ABI call.
+struct CallScope {
+  Context* saved;
+  explicit CallScope(Context* c) : saved(g_call_context) { g_call_context = c; }
+  ~CallScope() { g_call_context = saved; }
 };
 
 bool context_live(HpMi355xContext* ctx);
@@ -173,6 +206,7 @@ HpMi355xStatus hp_guarded(HpMi355xContext* ctx, F&& body) noexcept {
   if (ctx == nullptr || !hp::rt::context_live(ctx)) return hp::rt::invalid_parameters();
   hp::rt::Context& c = ctx->impl;
   if (c.poisoned) return hp::rt::invalid_state(c.first_error);
+  hp::rt::CallScope scope(&c);
   try {
     if (hipSetDevice(c.device) != hipSuccess) return hp::rt::invalid_state();
     return body(c);
@@ -181,6 +215,22 @@ HpMi355xStatus hp_guarded(HpMi355xContext* ctx, F&& body) noexcept {
     return f.status;
   } catch (...) {
     c.poison(-2);
+    return hp::rt::invalid_state(-2);
+  }
+}
+
+// Run a release entry point (buffer_retire, params and state destroy): as
+// hp_guarded, but also on a poisoned context, since releasing host-side
+// ownership must succeed after a failure was reported (the owner's drop path
+// cannot recover from a refused release).  Device memory goes back to the
+// context's cache and is freed by context_destroy after its drain.
+template <class F>
+HpMi355xStatus hp_guarded_release(HpMi355xContext* ctx, F&& body) noexcept {
+  if (ctx == nullptr || !hp::rt::context_live(ctx)) return hp::rt::invalid_parameters();
+  hp::rt::Context& c = ctx->impl;
+  try {
+    return body(c);
+  } catch (...) {
     return hp::rt::invalid_state(-2);
   }
 }

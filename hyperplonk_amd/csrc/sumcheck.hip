@@ -179,7 +179,10 @@ __device__ __forceinline__ Fr29 load29(const Fr* base, size_t i) {
   return f29::unpack_fr(load_fr(base, i));
 }
 
-// accumulators stay below 3 r + 4 x 3.3 r < 35 r between reductions
+// Between reductions an accumulator holds a reduced value (< 3 r) plus up to
+// four product terms; a product of one factor ends at its first factor's
+// terms (< 9.3 r), so the sum stays below 3 r + 4 x 9.3 r < 41 r < 2^261, the
+// input bound of reduce_est.
 template <int D>
 __device__ __forceinline__ void reduce_acc(Fr29 (&acc)[D + 1], uint32_t deg) {
 #pragma unroll
@@ -724,8 +727,12 @@ bool round_graphs_enabled() {
 RoundGraphCache& graph_cache(rt::Context& c) {
   auto it = c.resources.find(&g_round_graph_key);
   if (it == c.resources.end())
-    it = c.resources.emplace(&g_round_graph_key, std::make_shared<RoundGraphCache>()).first;
-  return *static_cast<RoundGraphCache*>(it->second.get());
+    it = c.resources
+             .emplace(&g_round_graph_key,
+                      rt::Context::Resource{rt::Context::Kind::internal,
+                                            std::make_shared<RoundGraphCache>()})
+             .first;
+  return *static_cast<RoundGraphCache*>(it->second.object.get());
 }
 
 hipKernelNodeParams node_params(PackLaunch& l, void** args) {
@@ -807,8 +814,10 @@ void run_jobs_graph(rt::Context& c, std::vector<sc::RoundJob>& jobs,
 // Checks of one round call before any side effect (bridge_sumcheck.cpp:519-550).
 HpMi355xStatus check_round(HpMi355xContext* ctx, rt::Context& c, HpMi355xSumCheckState* state,
                            const uint8_t* challenge, const uint8_t* out, size_t count) {
-  if (state == nullptr || state->ctx != ctx || !c.resources.count(state))
+  // membership first: nothing is read through a handle this context does not hold
+  if (state == nullptr || !c.holds(state, rt::Context::Kind::sumcheck_state))
     return rt::invalid_parameters();
+  if (state->ctx != ctx) return rt::invalid_parameters();
   const sc::State& s = *state->impl;
   if (s.failed) return rt::invalid_state(s.failure_code);
   if (out == nullptr) return rt::invalid_parameters();
@@ -1003,7 +1012,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     rt::hip_check(hipMemsetAsync(s->ticket->ptr, 0, sizeof(unsigned), c.stream));
     c.sync();
     auto* handle = new HpMi355xSumCheckState{ctx, s};
-    c.resources[handle] = s;
+    c.resources[handle] = {rt::Context::Kind::sumcheck_state, s};
     *out = handle;
     return rt::ok();
   });
@@ -1129,11 +1138,11 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
 
 extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_destroy(HpMi355xContext* ctx,
                                                                      HpMi355xSumCheckState* state) {
-  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
-    if (state == nullptr || state->ctx != ctx) return rt::invalid_parameters();
-    auto it = c.resources.find(state);
-    if (it == c.resources.end()) return rt::invalid_parameters();
-    c.resources.erase(it);
+  return hp_guarded_release(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (state == nullptr || !c.holds(state, rt::Context::Kind::sumcheck_state))
+      return rt::invalid_parameters();
+    if (state->ctx != ctx) return rt::invalid_parameters();
+    c.resources.erase(state);
     delete state;
     return rt::ok();
   });

@@ -23,8 +23,12 @@
  * InvalidParameters.  The first sumcheck round requires a null challenge,
  * later rounds a non-null one; ordering errors and exhaustion return
  * InvalidState without advancing the state.  A failed device execution
- * poisons the state (and the context for context-wide failures); later calls
- * on it return InvalidState.  No C++ exception crosses this interface.
+ * poisons the prover state it was advancing and the context; later calls on
+ * them return InvalidState with the first error's code, while buffer_retire
+ * and the params / state destroys still release their handle (the owner's
+ * drop path), and context_destroy releases everything and returns that
+ * first error.  Handles are checked against the context's live set before
+ * anything is read through them.  No C++ exception crosses this interface.
  */
 #ifndef HP_MI355X_H
 #define HP_MI355X_H
@@ -243,6 +247,23 @@ HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed(
 HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed(
     HpMi355xContext *ctx, const HpMi355xKzgParams *params, size_t layer_index,
     uint8_t *compressed, size_t point_count);
+/* Roofline denominators measured on this device (no counterpart in the
+ * reference): *out_hbm_bytes_per_s from a streaming copy of copy_bytes (read
+ * plus write bytes over the kernel time), *out_mad_per_s from a u32 x u32 ->
+ * u64 multiply-add chain kernel (v_mad_u64_u32, the field kernels' limb
+ * product) on every CU; best of `reps` timed runs each (HIP events on the
+ * context stream).  copy_bytes: 1 MiB .. 16 GiB, a multiple of 16. */
+HpMi355xStatus hp_mi355x_v1_device_peaks(HpMi355xContext *ctx, size_t copy_bytes, uint32_t reps,
+                                         double *out_hbm_bytes_per_s, double *out_mad_per_s);
+/* Failure injection for tests of the poisoning contract (no counterpart in the
+ * reference bridge, whose tests force device errors through the runtime):
+ * the `after`-th HIP call this context checks from now on (1 = the next one)
+ * completes normally on the device but is reported as failed with
+ * `native_code` (non-zero), exactly as a real device failure would be: the
+ * call returns INVALID_STATE with that code, a prover state it was advancing
+ * is poisoned, and so is the context.  after = 0 disarms. */
+HpMi355xStatus hp_mi355x_v1_debug_inject_failure(HpMi355xContext *ctx, uint32_t after,
+                                                 int32_t native_code);
 #ifdef __cplusplus
 }
 #endif

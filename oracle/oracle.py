@@ -163,10 +163,19 @@ def msm_pippenger_packets(points97: bytes, scalars32: bytes, n: int, threads: in
 
 
 # ---------------------------------------------------------------------- MLE
-def mle_evaluate(table: Sequence[int], point: Sequence[int]) -> int:
+def _table_bytes(table) -> bytes:
+    """A table as canonical LE32 bytes: raw bytes (or a uint8 array) pass through."""
+    if isinstance(table, (bytes, bytearray)):
+        return bytes(table)
+    if hasattr(table, "tobytes") and getattr(table, "dtype", None) is not None and table.dtype.itemsize == 1:
+        return table.tobytes()
+    return fr_bytes(table)
+
+
+def mle_evaluate(table, point: Sequence[int]) -> int:
     nv = len(point)
     out = ctypes.create_string_buffer(32)
-    _check(lib().oracle_mle_evaluate(fr_bytes(table), nv, fr_bytes(point), out), "evaluate")
+    _check(lib().oracle_mle_evaluate(_table_bytes(table), nv, fr_bytes(point), out), "evaluate")
     return fr_ints(out.raw)[0]
 
 
@@ -255,7 +264,7 @@ class SumCheckState:
         off_arr = (ctypes.c_size_t * len(offsets))(*offsets)
         idx_arr = (ctypes.c_size_t * max(len(indices), 1))(*indices)
         status = ctypes.c_int(0)
-        flat = b"".join(fr_bytes(t) for t in tables)
+        flat = b"".join(_table_bytes(t) for t in tables)
         self._h = lib().oracle_sumcheck_create(
             nv,
             max_degree,
@@ -288,7 +297,7 @@ def kzg_open_packets(layers97: bytes, table: Sequence[int], point: Sequence[int]
     nv = len(point)
     quot = ctypes.create_string_buffer(97 * max(nv, 1))
     ev = ctypes.create_string_buffer(32)
-    st = lib().oracle_kzg_open(layers97, fr_bytes(table), nv, fr_bytes(point), threads, quot, ev)
+    st = lib().oracle_kzg_open(layers97, _table_bytes(table), nv, fr_bytes(point), threads, quot, ev)
     _check(st, "kzg open")
     return [quot.raw[97 * j : 97 * (j + 1)] for j in range(nv)], fr_ints(ev.raw)[0]
 

@@ -13,7 +13,7 @@ from hyperplonk_amd.virtual_polynomial import encode_products
 def test_library_exports_every_header_symbol():
     lib = abi.load()
     declared = abi.header_symbols()
-    assert len(declared) == 32
+    assert len(declared) == 34
     for name in declared:
         assert hasattr(lib, name), name
     # the 20 entry points of the reference bridge (bridge.h:43-62) are all present
@@ -42,6 +42,24 @@ def test_null_and_unknown_context_are_rejected_without_a_device():
     assert st.kind == abi.INVALID_PARAMETERS and out.value is None
     st = lib.hp_mi355x_v1_context_create(0, None)
     assert st.kind == abi.INVALID_PARAMETERS
+
+
+def test_static_archive_links_into_a_c_program():
+    """libhp_mi355x.a links into a plain C program with the HIP runtime and the
+    C++ runtime only (the link the reference's `kind = "static"` binding makes,
+    backend/src/ascend910b3/ffi.rs:34); the program's calls are argument checks
+    that run before any device use."""
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prog = os.path.join(root, "_build", "static_link_test")
+    if not os.path.exists(prog):
+        subprocess.run(["make", "-C", os.path.join(root, "tests", "cpp"), prog], check=True, capture_output=True)
+    libs = subprocess.run(["ldd", prog], capture_output=True, text=True).stdout
+    assert "libhp_mi355x" not in libs
+    res = subprocess.run([prog], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "STATIC_LINK PASS" in res.stdout, (res.returncode, res.stdout, res.stderr)
 
 
 class FakeMle:

@@ -18,3 +18,13 @@ def test_cpp_host_mirror_known_answers():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "BACKEND_TEST PASS" in res.stdout
+
+
+def test_static_archive_program_known_answers():
+    """A C program linked against libhp_mi355x.a (no shared library), the way
+    the reference's Rust glue links its bridge statically, runs the evaluate
+    and sumcheck known answers on the GPU (tests/cpp/static_gpu_test.c)."""
+    prog = os.path.join(ROOT, "_build", "static_gpu_test")
+    assert os.path.exists(prog), "build the test programs first (make -C tests/cpp)"
+    res = subprocess.run([prog], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and "STATIC_GPU PASS" in res.stdout, (res.returncode, res.stdout, res.stderr)
