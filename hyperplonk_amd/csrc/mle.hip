@@ -8,6 +8,7 @@
 // low-bit-first: index bit i <-> variable i <-> point[i].
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "field29.hpp"
@@ -304,18 +305,22 @@ void fold(rt::Context& c, const Fr* src, Fr* dst, const Fr* r, size_t half) {
 // Evaluate a table of 2^nv entries at a device-resident point; result in out[0].
 // Folds ping-pong between two scratch tables (a fold cannot run in place on
 // the device: thread b would overwrite entry b while thread b/2 still reads it).
-void evaluate(rt::Context& c, const Fr* table, size_t nv, const Fr* point, Fr* out) {
+void evaluate(rt::Context& c, const Fr* table, size_t nv, const Fr* point, Fr* out,
+              Fr* fold_scratch) {
   if (nv == 0) {
     rt::hip_check(hipMemcpyAsync(out, table, rt::kFrBytes, hipMemcpyDeviceToDevice, c.stream));
     return;
   }
   const Fr* src = table;
   size_t level = 0;
-  rt::BufferRef scratch;
+  rt::BufferRef held;
   if (nv > 10) {
     const size_t a = size_t(1) << (nv - 1), b = size_t(1) << (nv - 2);
-    scratch = c.allocate((a + b) * rt::kFrBytes);
-    Fr* bufs[2] = {static_cast<Fr*>(scratch->ptr), static_cast<Fr*>(scratch->ptr) + a};
+    if (fold_scratch == nullptr) {
+      held = c.allocate((a + b) * rt::kFrBytes);
+      fold_scratch = static_cast<Fr*>(held->ptr);
+    }
+    Fr* bufs[2] = {fold_scratch, fold_scratch + a};
     while (nv - level > 10) {
       const size_t half = size_t(1) << (nv - level - 1);
       Fr* dst = bufs[level & 1];
@@ -475,6 +480,125 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate(HpMi355xContext* ctx,
     mle::evaluate(c, static_cast<const Fr*>(table->ptr), num_vars,
                   static_cast<const Fr*>(dpoint->ptr), static_cast<Fr*>(result->ptr));
     rt::download_fr(c, static_cast<const Fr*>(result->ptr), 1, out_evaluation);
+    return rt::ok();
+  });
+}
+
+// ---- evaluate_mle without allocation -------------------------------------
+//
+// One file-scope device array per GPU holds everything the evaluation needs
+// beyond its input table and output: the point (Montgomery form), the value
+// and the two fold tables (2^(nv-1) and 2^(nv-2) entries) for up to
+// HP_MI355X_NOALLOC_MAX_NUM_VARS variables.  Every word is written before it
+// is read within a call, so nothing has to start at zero.  All calls on one
+// GPU share the array: calls that overlap on one GPU (two contexts of the
+// device driven from two threads) race.
+namespace hp {
+namespace {
+
+constexpr size_t kNoAllocMaxNumVars = HP_MI355X_NOALLOC_MAX_NUM_VARS;
+constexpr size_t kNoAllocPointFr = 32;  // point coordinates (num_vars <= 31)
+constexpr size_t kNoAllocScratchFr = kNoAllocPointFr + 2 + (size_t(1) << (kNoAllocMaxNumVars - 1)) +
+                                     (size_t(1) << (kNoAllocMaxNumVars - 2));
+static_assert(sizeof(Fr) == 32, "Fr is eight 32-bit words");
+
+// 32-byte words; host code never names this array as a device address: it
+// uses the per-device pointer from hipGetSymbolAddress below.
+static __device__ __align__(256) uint4 scratch[2 * kNoAllocScratchFr];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+int32_t g_noalloc_error[kNoAllocMaxDevices] = {};
+bool g_noalloc_done[kNoAllocMaxDevices] = {};
+
+struct PointArg {
+  Fr v[kNoAllocPointFr];
+};
+
+// the point (Montgomery form, by value) into the scratch point slots
+__global__ void k_store_point(PointArg p, uint32_t count, Fr* dst) {
+  if (threadIdx.x < count) store_fr(dst, threadIdx.x, p.v[threadIdx.x]);
+}
+
+// canonical value of a Montgomery cell
+__global__ void k_canonical_value(const Fr* src, Fr* dst) {
+  if (threadIdx.x == 0) store_fr(dst, 0, from_mont(load_fr(src, 0)));
+}
+
+}  // namespace
+
+namespace mle {
+void noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (g_noalloc_done[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    g_noalloc_error[device] = static_cast<int32_t>(e);
+    ptr = nullptr;
+  }
+  g_noalloc_ptr[device] = ptr;
+  g_noalloc_done[device] = true;
+}
+}  // namespace mle
+
+}  // namespace hp
+
+namespace {
+// This device's scratch array, or a failure status.
+HpMi355xStatus noalloc_scratch(int device, Fr** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kNoAllocMaxDevices) return rt::invalid_state();
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_done[device]) return rt::invalid_state();
+  if (g_noalloc_ptr[device] == nullptr) return rt::invalid_state(g_noalloc_error[device]);
+  *out = static_cast<Fr*>(g_noalloc_ptr[device]);
+  return rt::ok();
+}
+}  // namespace
+
+extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate_noalloc(HpMi355xContext* ctx,
+                                                            HpMi355xDeviceAddress polynomial,
+                                                            size_t num_vars, const uint8_t* point,
+                                                            uint8_t* out_evaluation) {
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    // the checks of mle_evaluate, then the size limit of the static scratch
+    if (out_evaluation == nullptr) return rt::invalid_parameters();
+    if (point == nullptr && num_vars != 0) return rt::invalid_parameters();
+    if (!dims_ok(num_vars)) return rt::invalid_parameters();
+    rt::BufferRef table = c.lookup(polynomial, num_vars);
+    if (!table) return rt::invalid_parameters();
+    if (num_vars && !rt::canonical_fr_array(point, num_vars)) return rt::invalid_parameters();
+    if (num_vars > kNoAllocMaxNumVars)
+      return rt::invalid_parameters(HP_MI355X_NATIVE_EXCEEDS_SCRATCH);
+    Fr* base = nullptr;
+    const HpMi355xStatus st = noalloc_scratch(c.device, &base);
+    if (!rt::is_ok(st)) return st;
+    Fr* dpoint = base;
+    Fr* value = base + kNoAllocPointFr;       // Montgomery
+    Fr* canonical = value + 1;
+    Fr* folds = base + kNoAllocPointFr + 2;   // 2^(max-1) + 2^(max-2) entries
+    PointArg arg;
+    for (size_t i = 0; i < num_vars; ++i) {
+      Fr v;
+      std::memcpy(v.v, point + rt::kFrBytes * i, rt::kFrBytes);
+      arg.v[i] = to_mont(v);
+    }
+    for (size_t i = num_vars; i < kNoAllocPointFr; ++i) arg.v[i] = Fr::zero();
+    if (num_vars) {
+      hipLaunchKernelGGL(k_store_point, dim3(1), dim3(64), 0, c.stream, arg,
+                         static_cast<uint32_t>(num_vars), dpoint);
+      rt::check_launch();
+    }
+    mle::evaluate(c, static_cast<const Fr*>(table->ptr), num_vars, dpoint, value, folds);
+    hipLaunchKernelGGL(k_canonical_value, dim3(1), dim3(64), 0, c.stream, value, canonical);
+    rt::check_launch();
+    rt::hip_check(hipMemcpyAsync(out_evaluation, canonical, rt::kFrBytes, hipMemcpyDeviceToHost,
+                                 c.stream));
+    c.sync();
     return rt::ok();
   });
 }

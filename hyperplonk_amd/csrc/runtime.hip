@@ -16,6 +16,7 @@
 #include <unordered_set>
 
 #include "kernels_common.hpp"
+#include "mle.hpp"
 #include "runtime.hpp"
 
 namespace hp {
@@ -358,6 +359,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_context_create(int32_t device_id, HpMi355
     ctx->impl.device = device_id;
     rt::hip_check(hipStreamCreateWithFlags(&ctx->impl.stream, hipStreamNonBlocking));
     ctx->impl.alloc = std::make_unique<rt::Allocator>(ctx->impl.stream);
+    mle::noalloc_scratch_setup(device_id);  // once per device, this device current
   } catch (const rt::Failure& f) {
     if (ctx) {
       if (ctx->impl.stream) (void)hipStreamDestroy(ctx->impl.stream);

@@ -247,6 +247,22 @@ HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_upload_layer_compressed(
 HpMi355xStatus hp_mi355x_v1_multilinear_kzg_params_download_layer_compressed(
     HpMi355xContext *ctx, const HpMi355xKzgParams *params, size_t layer_index,
     uint8_t *compressed, size_t point_count);
+/* evaluate_mle that allocates nothing (no counterpart in the reference bridge;
+ * for callers that cannot allocate device or pinned memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Same arguments, checks, statuses and value as hp_mi355x_v1_mle_evaluate;
+ * the point, the fold tables and the value live in one static device array
+ * per GPU, sized for num_vars <= HP_MI355X_NOALLOC_MAX_NUM_VARS (24 MiB);
+ * larger tables return INVALID_PARAMETERS with native_code
+ * HP_MI355X_NATIVE_EXCEEDS_SCRATCH.  Every call on a GPU shares that array:
+ * calls that overlap on one GPU (contexts of the same device driven from
+ * several threads) race. */
+#define HP_MI355X_NOALLOC_MAX_NUM_VARS 20
+#define HP_MI355X_NATIVE_EXCEEDS_SCRATCH 1001
+HpMi355xStatus hp_mi355x_v1_mle_evaluate_noalloc(HpMi355xContext *ctx,
+                                                 HpMi355xDeviceAddress polynomial,
+                                                 size_t num_vars, const uint8_t *point,
+                                                 uint8_t *out_evaluation);
 /* Roofline denominators measured on this device (no counterpart in the
  * reference): *out_hbm_bytes_per_s from a streaming copy of copy_bytes (read
  * plus write bytes over the kernel time), *out_mad_per_s from a u32 x u32 ->
