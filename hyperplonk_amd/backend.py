@@ -456,10 +456,9 @@ class Mi355xBackend:
         )
         return fr_from_bytes(out.raw)
 
-    def kzg_batch(self, pp: PreparedProverParam, ops: Sequence[tuple]) -> List[tuple]:
-        """Commits ("commit", mle) and opens ("open", mle, point) with all their
-        MSMs in one batch (extension: multilinear_kzg_batch).  Returns packets
-        per op: (commitment,) or (quotient packets, evaluation)."""
+    _KzgRequestType = _KzgRequest
+
+    def _kzg_requests(self, ops: Sequence[tuple]):
         reqs = (_KzgRequest * max(len(ops), 1))()
         keep = []
         for i, op in enumerate(ops):
@@ -485,7 +484,10 @@ class Mi355xBackend:
                 r.point = ctypes.addressof(pbuf) if pbuf is not None else None
                 r.out_evaluation = ctypes.addressof(ev)
             r.out_points = ctypes.addressof(out)
-        check(self.lib.hp_mi355x_v1_multilinear_kzg_batch(self.context, pp.raw, reqs, len(ops)), "kzg batch")
+        return reqs, keep
+
+    @staticmethod
+    def _kzg_results(ops: Sequence[tuple], keep) -> List[tuple]:
         res = []
         for op, k in zip(ops, keep):
             if op[0] == "commit":
@@ -494,6 +496,23 @@ class Mi355xBackend:
                 nv = op[1].num_vars
                 res.append(([k[0].raw[97 * j : 97 * (j + 1)] for j in range(nv)], fr_from_bytes(k[1].raw)[0]))
         return res
+
+    def kzg_batch(self, pp: PreparedProverParam, ops: Sequence[tuple]) -> List[tuple]:
+        """Commits ("commit", mle) and opens ("open", mle, point) with all their
+        MSMs in one batch (extension: multilinear_kzg_batch).  Returns packets
+        per op: (commitment,) or (quotient packets, evaluation)."""
+        reqs, keep = self._kzg_requests(ops)
+        check(self.lib.hp_mi355x_v1_multilinear_kzg_batch(self.context, pp.raw, reqs, len(ops)), "kzg batch")
+        return self._kzg_results(ops, keep)
+
+    def kzg_batch_submit(self, pp: PreparedProverParam, ops: Sequence[tuple]) -> "KzgPending":
+        """kzg_batch queued beside later calls (extension: batch_submit); the
+        returned object's wait() gives the same results."""
+        reqs, keep = self._kzg_requests(ops)
+        raw = ctypes.c_void_p()
+        check(self.lib.hp_mi355x_v1_multilinear_kzg_batch_submit(self.context, pp.raw, reqs, len(ops),
+                                                                  ctypes.byref(raw)), "kzg batch submit")
+        return KzgPending(self, pp, list(ops), reqs, keep, raw.value)
 
     def prove_rounds_batch(self, states: Sequence[ProverState], challenges: Sequence[Optional[int]]) -> List[List[int]]:
         """One round of each distinct state in shared launches (extension)."""
@@ -511,3 +530,23 @@ class Mi355xBackend:
             "sumcheck round batch",
         )
         return [fr_from_bytes(o.raw) for o in outs]
+
+
+class KzgPending:
+    """A submitted KZG batch: its request and output buffers stay alive here
+    until wait() (called once) writes and returns the results."""
+
+    def __init__(self, be: Mi355xBackend, pp: PreparedProverParam, ops, reqs, keep, raw: int):
+        self._be, self._pp, self._ops, self._reqs, self._keep, self._raw = be, pp, ops, reqs, keep, raw
+
+    def wait(self) -> List[tuple]:
+        if not self._raw:
+            raise InvalidParameters("batch already waited for")
+        raw, self._raw = self._raw, None
+        check(self._be.lib.hp_mi355x_v1_multilinear_kzg_batch_wait(self._be.context, raw), "kzg batch wait")
+        return Mi355xBackend._kzg_results(self._ops, self._keep)
+
+    def __del__(self):
+        if getattr(self, "_raw", None) and self._be._ctx.raw:
+            self._be.lib.hp_mi355x_v1_multilinear_kzg_batch_wait(self._be.context, self._raw)
+            self._raw = None

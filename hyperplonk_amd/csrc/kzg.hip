@@ -54,6 +54,32 @@ struct HpMi355xKzgParams {
   std::shared_ptr<hp::kzg::Params> impl;
 };
 
+namespace hp {
+namespace kzg {
+// An asynchronous batch: its work runs on the context's side stream; every
+// buffer it reads or writes is held here until the wait.
+struct Ticket {
+  std::shared_ptr<Params> params;  // the layers the batch reads
+  std::vector<rt::BufferRef> hold;
+  msm::Batch batch;
+  G1Xyzz* slots_host = nullptr;  // pinned: level-1 results of every MSM
+  Fr* values_host = nullptr;     // pinned: opened values, stored form
+  std::vector<uint8_t*> packet_outs, value_outs;
+  hipEvent_t done = nullptr;
+  ~Ticket() {
+    if (slots_host) (void)hipHostFree(slots_host);
+    if (values_host) (void)hipHostFree(values_host);
+    if (done) (void)hipEventDestroy(done);
+  }
+};
+}  // namespace kzg
+}  // namespace hp
+
+struct HpMi355xKzgTicket {
+  HpMi355xContext* ctx;
+  std::shared_ptr<hp::kzg::Ticket> impl;
+};
+
 using namespace hp;
 
 namespace {
@@ -340,6 +366,39 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
   });
 }
 
+// Every request of a batch, before any work: arguments, then layer readiness.
+static HpMi355xStatus check_requests(rt::Context& c, const kzg::Params& p,
+                                     const HpMi355xKzgRequest* requests, size_t count,
+                                     std::vector<rt::BufferRef>& tables) {
+  const size_t N = p.num_vars;
+  for (size_t i = 0; i < count; ++i) {
+    const HpMi355xKzgRequest& r = requests[i];
+    if (r.kind != HP_MI355X_KZG_COMMIT && r.kind != HP_MI355X_KZG_OPEN)
+      return rt::invalid_parameters();
+    if (r.num_vars > N) return rt::invalid_parameters();
+    tables[i] = c.lookup(r.polynomial, r.num_vars);
+    if (!tables[i]) return rt::invalid_parameters();
+    if (r.kind == HP_MI355X_KZG_COMMIT) {
+      if (r.out_points == nullptr) return rt::invalid_parameters();
+    } else {
+      if (r.out_evaluation == nullptr) return rt::invalid_parameters();
+      if (r.num_vars > 0 && (r.point == nullptr || r.out_points == nullptr))
+        return rt::invalid_parameters();
+      if (r.num_vars && !rt::canonical_fr_array(r.point, r.num_vars)) return rt::invalid_parameters();
+    }
+  }
+  for (size_t i = 0; i < count; ++i) {
+    const HpMi355xKzgRequest& r = requests[i];
+    if (r.kind == HP_MI355X_KZG_COMMIT) {
+      if (!p.ready[N - r.num_vars]) return rt::invalid_state();
+    } else {
+      for (size_t j = 0; j < r.num_vars; ++j)
+        if (!p.ready[N - r.num_vars + 1 + j]) return rt::invalid_state();
+    }
+  }
+  return rt::ok();
+}
+
 // Commits and opens of many polynomials with all their MSMs in one batch.
 extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext* ctx,
                                                              const HpMi355xKzgParams* params,
@@ -352,31 +411,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext* ct
     if (requests == nullptr) return rt::invalid_parameters();
     const size_t N = p->num_vars;
     std::vector<rt::BufferRef> tables(count);
-    for (size_t i = 0; i < count; ++i) {
-      const HpMi355xKzgRequest& r = requests[i];
-      if (r.kind != HP_MI355X_KZG_COMMIT && r.kind != HP_MI355X_KZG_OPEN)
-        return rt::invalid_parameters();
-      if (r.num_vars > N) return rt::invalid_parameters();
-      tables[i] = c.lookup(r.polynomial, r.num_vars);
-      if (!tables[i]) return rt::invalid_parameters();
-      if (r.kind == HP_MI355X_KZG_COMMIT) {
-        if (r.out_points == nullptr) return rt::invalid_parameters();
-      } else {
-        if (r.out_evaluation == nullptr) return rt::invalid_parameters();
-        if (r.num_vars > 0 && (r.point == nullptr || r.out_points == nullptr))
-          return rt::invalid_parameters();
-        if (r.num_vars && !rt::canonical_fr_array(r.point, r.num_vars)) return rt::invalid_parameters();
-      }
-    }
-    for (size_t i = 0; i < count; ++i) {
-      const HpMi355xKzgRequest& r = requests[i];
-      if (r.kind == HP_MI355X_KZG_COMMIT) {
-        if (!p->ready[N - r.num_vars]) return rt::invalid_state();
-      } else {
-        for (size_t j = 0; j < r.num_vars; ++j)
-          if (!p->ready[N - r.num_vars + 1 + j]) return rt::invalid_state();
-      }
-    }
+    const HpMi355xStatus st = check_requests(c, *p, requests, count, tables);
+    if (!rt::is_ok(st)) return st;
     msm::Batch batch;
     std::vector<rt::BufferRef> keep;
     std::vector<const Fr*> cells;
@@ -403,6 +439,110 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext* ct
         std::memcpy(packet_outs[j], packets.data() + rt::kG1Bytes * j, rt::kG1Bytes);
     }
     download_values(c, cells, value_outs.data());
+    return rt::ok();
+  });
+}
+
+// The requests of multilinear_kzg_batch enqueued on the context's side stream
+// behind everything already queued on the context stream; the call returns
+// at once and the results land in the requests' outputs at the wait, so
+// other work (sumcheck rounds) overlaps the MSMs on the device.
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch_submit(
+    HpMi355xContext* ctx, const HpMi355xKzgParams* params, const HpMi355xKzgRequest* requests,
+    size_t count, HpMi355xKzgTicket** out) {
+  if (out) *out = nullptr;
+  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (out == nullptr) return rt::invalid_parameters();
+    kzg::Params* p = bind(ctx, c, params);
+    if (!p) return rt::invalid_parameters();
+    if (count > 0 && requests == nullptr) return rt::invalid_parameters();
+    std::vector<rt::BufferRef> tables(count);
+    const HpMi355xStatus st = check_requests(c, *p, requests, count, tables);
+    if (!rt::is_ok(st)) return st;
+    auto t = std::make_shared<kzg::Ticket>();
+    t->params = std::static_pointer_cast<kzg::Params>(c.resources.at(params).object);
+    t->hold = tables;
+    const size_t N = p->num_vars;
+    hipStream_t side = c.side_stream();
+    {
+      hipEvent_t ready = nullptr;
+      rt::hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+      const hipError_t r1 = hipEventRecord(ready, c.stream);
+      const hipError_t r2 = r1 == hipSuccess ? hipStreamWaitEvent(side, ready, 0) : r1;
+      (void)hipEventDestroy(ready);
+      rt::hip_check(r2);
+    }
+    rt::StreamSwap swap(c, side);
+    std::vector<const Fr*> cells;
+    for (size_t i = 0; i < count; ++i) {
+      const HpMi355xKzgRequest& r = requests[i];
+      const Fr* table = static_cast<const Fr*>(tables[i]->ptr);
+      if (r.kind == HP_MI355X_KZG_COMMIT) {
+        t->batch.add(p->layers[N - r.num_vars], table);
+        t->packet_outs.push_back(r.out_points);
+      } else {
+        const Fr* value = nullptr;
+        enqueue_open(c, *p, table, r.num_vars, r.point, t->batch, t->hold, &value);
+        for (size_t j = 0; j < r.num_vars; ++j) t->packet_outs.push_back(r.out_points + rt::kG1Bytes * j);
+        cells.push_back(value);
+        t->value_outs.push_back(r.out_evaluation);
+      }
+    }
+    if (t->batch.size()) {
+      const size_t slots = std::max<size_t>(t->batch.result_slots(), 1);
+      rt::hip_check(hipHostMalloc(reinterpret_cast<void**>(&t->slots_host), slots * sizeof(G1Xyzz),
+                                  hipHostMallocDefault));
+      t->batch.enqueue(c, t->slots_host, nullptr, &t->hold);
+    }
+    if (!cells.empty()) {
+      const size_t k = cells.size();
+      rt::BufferRef ptrs = c.allocate(k * sizeof(void*));
+      uint8_t* stage = c.staging(k * sizeof(void*));
+      std::memcpy(stage, cells.data(), k * sizeof(void*));
+      rt::hip_check(hipMemcpyAsync(ptrs->ptr, stage, k * sizeof(void*), hipMemcpyHostToDevice, c.stream));
+      c.staging_used();
+      rt::BufferRef vals = c.allocate(k * rt::kFrBytes);
+      hipLaunchKernelGGL(k_gather_values, dim3(1), dim3(256), 0, c.stream,
+                         static_cast<const Fr* const*>(ptrs->ptr), k, static_cast<Fr*>(vals->ptr));
+      rt::check_launch();
+      rt::hip_check(hipHostMalloc(reinterpret_cast<void**>(&t->values_host), k * rt::kFrBytes,
+                                  hipHostMallocDefault));
+      rt::hip_check(hipMemcpyAsync(t->values_host, vals->ptr, k * rt::kFrBytes, hipMemcpyDeviceToHost,
+                                   c.stream));
+      t->hold.push_back(std::move(ptrs));
+      t->hold.push_back(std::move(vals));
+    }
+    rt::hip_check(hipEventCreateWithFlags(&t->done, hipEventDisableTiming));
+    rt::hip_check(hipEventRecord(t->done, c.stream));
+    auto* handle = new HpMi355xKzgTicket{ctx, t};
+    c.resources[handle] = {rt::Context::Kind::kzg_ticket, t};
+    *out = handle;
+    return rt::ok();
+  });
+}
+
+// Waits for a submitted batch and writes its results (once per ticket; the
+// ticket is released).
+extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch_wait(HpMi355xContext* ctx,
+                                                                  HpMi355xKzgTicket* ticket) {
+  return hp_guarded<false>(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (ticket == nullptr || !c.holds(ticket, rt::Context::Kind::kzg_ticket))
+      return rt::invalid_parameters();
+    if (ticket->ctx != ctx) return rt::invalid_parameters();
+    kzg::Ticket& t = *ticket->impl;
+    rt::hip_check(hipEventSynchronize(t.done));
+    if (t.batch.size()) {
+      std::vector<uint8_t> packets(t.batch.size() * rt::kG1Bytes);
+      t.batch.finish_packets(t.slots_host, packets.data());
+      for (size_t j = 0; j < t.packet_outs.size(); ++j)
+        std::memcpy(t.packet_outs[j], packets.data() + rt::kG1Bytes * j, rt::kG1Bytes);
+    }
+    for (size_t i = 0; i < t.value_outs.size(); ++i) {
+      const Fr v = from_mont(t.values_host[i]);
+      std::memcpy(t.value_outs[i], v.v, rt::kFrBytes);
+    }
+    c.resources.erase(ticket);
+    delete ticket;
     return rt::ok();
   });
 }

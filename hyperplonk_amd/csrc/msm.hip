@@ -1180,7 +1180,8 @@ void Batch::add(const Layer& layer, const Fr* scalars) {
 unsigned Batch::red_group() const { return gbuckets_ <= 16384 ? kRedSmall : kRed1; }
 size_t Batch::result_slots() const { return 2 * (size_t)(gbuckets_ / red_group()); }
 
-void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
+void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev,
+                    std::vector<rt::BufferRef>* hold) const {
   auto mark = [&](int i) {
     if (ev) rt::hip_check(hipEventRecord(ev->marks[i], c.stream));
   };
@@ -1368,6 +1369,11 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev) const {
     if (ev->entries_out)
       rt::hip_check(hipMemcpyAsync(ev->entries_out, totals, sizeof(uint32_t),
                                    hipMemcpyDeviceToDevice, c.stream));
+  }
+  if (hold) {
+    for (rt::BufferRef* b : {&table, &sorted_buf, &meta, &tasks_buf, &partials_buf, &sums_buf, &heavy_buf})
+      hold->push_back(*b);
+    hold->insert(hold->end(), keep.begin(), keep.end());
   }
 }
 

@@ -60,7 +60,10 @@ class Batch {
   void set_cap(uint32_t cap) { cap_override_ = cap; }
   size_t result_slots() const;
   unsigned red_group() const;  // buckets per level-1 reduction group
-  void enqueue(rt::Context& c, G1Xyzz* result, StageEvents* events = nullptr) const;
+  // `hold`, when given, receives every device buffer the launches use, for
+  // callers whose stream is not the context's (the buffers must outlive it)
+  void enqueue(rt::Context& c, G1Xyzz* result, StageEvents* events = nullptr,
+               std::vector<rt::BufferRef>* hold = nullptr) const;
   // After the result slots are copied to the host: one 97-byte packet per MSM.
   void finish_packets(const G1Xyzz* host_slots, uint8_t* out97) const;
 

@@ -175,6 +175,11 @@ void Context::staging_used() {
 
 void Context::sync() { hip_check(hipStreamSynchronize(stream)); }
 
+hipStream_t Context::side_stream() {
+  if (side == nullptr) hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  return side;
+}
+
 // ---- boundary conversions ----------------------------------------------
 namespace {
 __global__ void k_fr_to_mont(Fr* data, size_t n) {
@@ -386,14 +391,26 @@ extern "C" HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext* ctx) {
   }
   rt::Context& c = ctx->impl;
   (void)hipSetDevice(c.device);
-  const hipError_t drained = hipStreamSynchronize(c.stream);
+  if (c.quiesce != nullptr) {
+    try {
+      c.quiesce(c);  // stops persistent round loops (they wait on the host)
+    } catch (...) {
+    }
+  }
+  hipError_t drained = hipStreamSynchronize(c.stream);
+  if (c.side != nullptr) {
+    const hipError_t e = hipStreamSynchronize(c.side);
+    if (drained == hipSuccess) drained = e;
+  }
   c.resources.clear();
+  if (c.release_tails != nullptr) c.release_tails(c);
   c.registry.clear();
   c.alloc.reset();  // frees every cached block (after the drain)
   if (c.pinned) (void)hipHostFree(c.pinned);
   if (c.results_host) (void)hipHostFree(c.results_host);
   if (c.pinned_free) (void)hipEventDestroy(c.pinned_free);
   (void)hipStreamDestroy(c.stream);
+  if (c.side != nullptr) (void)hipStreamDestroy(c.side);
   const bool poisoned = c.poisoned;
   const int32_t first_error = c.first_error;
   delete ctx;

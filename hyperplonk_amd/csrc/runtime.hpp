@@ -110,7 +110,7 @@ struct Context {
   // keyed by handle address with the handle's kind, so that a handle is
   // checked by lookup here before anything is read through it, and a freed
   // handle whose address was reused by a handle of another kind is refused.
-  enum class Kind : uint8_t { kzg_params = 1, sumcheck_state = 2, internal = 3 };
+  enum class Kind : uint8_t { kzg_params = 1, sumcheck_state = 2, internal = 3, kzg_ticket = 4 };
   struct Resource {
     Kind kind;
     std::shared_ptr<void> object;
@@ -147,10 +147,20 @@ struct Context {
   uint8_t* results_host = nullptr;
   size_t results_bytes = 0;
   void sync();
+  // A second stream for work that overlaps the context stream (asynchronous
+  // KZG batches), created on first use; context_destroy drains both.
+  hipStream_t side = nullptr;
+  hipStream_t side_stream();
   void poison(int32_t code) {
     if (!poisoned) first_error = code ? code : -1;
     poisoned = true;
   }
+  // Persistent sumcheck round loops (sumcheck.hip) running on `stream`: while
+  // any is active, work enqueued by another entry point would wait behind it,
+  // so hp_guarded calls `quiesce` first (it stops them and drains the stream).
+  void (*quiesce)(Context&) = nullptr;
+  void* tails = nullptr;  // owned by sumcheck.hip (released by `release_tails`)
+  void (*release_tails)(Context&) = nullptr;
   // Failure injection (hp_mi355x_v1_debug_inject_failure): the inject_after-th
   // checked HIP call of this context from now on reports inject_code as its
   // error, after the call itself completed normally on the device.
@@ -167,6 +177,15 @@ inline void hip_check(hipError_t e) {
     e = static_cast<hipError_t>(c->inject_code);
   if (e != hipSuccess) fail_state(static_cast<int32_t>(e));
 }
+
+// Points the context's stream at `s` for one scope: everything the library
+// enqueues meanwhile (copies, kernels, staging events) goes to `s`.
+struct StreamSwap {
+  Context& c;
+  hipStream_t saved;
+  StreamSwap(Context& ctx, hipStream_t s) : c(ctx), saved(ctx.stream) { c.stream = s; }
+  ~StreamSwap() { c.stream = saved; }
+};
 
 // Sets g_call_context for the lifetime of one ABI call.
 struct CallScope {
@@ -201,7 +220,9 @@ struct HpMi355xContext {
 
 // Run `body` as an ABI call: validate the context, select its device, map
 // library failures to statuses and poison the context on device errors.
-template <class F>
+// kQuiesce: stop the context's persistent round loops before the body
+// enqueues work (every entry point but the sumcheck rounds, which manage them).
+template <bool kQuiesce = true, class F>
 HpMi355xStatus hp_guarded(HpMi355xContext* ctx, F&& body) noexcept {
   if (ctx == nullptr || !hp::rt::context_live(ctx)) return hp::rt::invalid_parameters();
   hp::rt::Context& c = ctx->impl;
@@ -209,6 +230,7 @@ HpMi355xStatus hp_guarded(HpMi355xContext* ctx, F&& body) noexcept {
   hp::rt::CallScope scope(&c);
   try {
     if (hipSetDevice(c.device) != hipSuccess) return hp::rt::invalid_state();
+    if (kQuiesce && c.quiesce != nullptr) c.quiesce(c);
     return body(c);
   } catch (const hp::rt::Failure& f) {
     if (f.poison) c.poison(f.status.native_code);

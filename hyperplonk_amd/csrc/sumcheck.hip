@@ -19,8 +19,10 @@
 // (backend/src/ascend910b3/native/bridge_sumcheck.cpp:210-331 create,
 // :519-550 round).
 #include <algorithm>
+#include <chrono>
 #include <memory>
 #include <map>
+#include <thread>
 #include <deque>
 #include <cstdlib>
 #include <cstring>
@@ -141,30 +143,27 @@ struct JobPack {
 // first factor's terms < 9.3r, every later product < 3.3r (R / r = 70.7).
 using f29::Fr29;
 
+// Every job of a launch has degree D exactly (launches group jobs by degree),
+// so the point count is a compile-time constant.
 template <int D>
 struct ProductTerms {
   Fr29 term[D + 1];
-  __device__ __forceinline__ void first(const Fr29& coeff, const Fr29& lo, const Fr29& hi,
-                                        uint32_t deg) {
+  __device__ __forceinline__ void first(const Fr29& coeff, const Fr29& lo, const Fr29& hi) {
     Fr29 line = f29::mul(coeff, lo);
     const Fr29 d = f29::mul(coeff, f29::sub1p(hi, lo));
 #pragma unroll
     for (int t = 0; t <= D; ++t) {
-      if (t <= (int)deg) {
-        term[t] = line;
-        line = f29::add(line, d);
-      }
+      term[t] = line;
+      if (t < D) line = f29::add(line, d);
     }
   }
-  __device__ __forceinline__ void times(const Fr29& lo, const Fr29& hi, uint32_t deg) {
+  __device__ __forceinline__ void times(const Fr29& lo, const Fr29& hi) {
     Fr29 line = lo;
     const Fr29 d = f29::sub1p(hi, lo);
 #pragma unroll
     for (int t = 0; t <= D; ++t) {
-      if (t <= (int)deg) {
-        term[t] = f29::mul(term[t], line);
-        line = f29::add(line, d);
-      }
+      term[t] = f29::mul(term[t], line);
+      if (t < D) line = f29::add(line, d);
     }
   }
 };
@@ -184,10 +183,9 @@ __device__ __forceinline__ Fr29 load29(const Fr* base, size_t i) {
 // terms (< 9.3 r), so the sum stays below 3 r + 4 x 9.3 r < 41 r < 2^261, the
 // input bound of reduce_est.
 template <int D>
-__device__ __forceinline__ void reduce_acc(Fr29 (&acc)[D + 1], uint32_t deg) {
+__device__ __forceinline__ void reduce_acc(Fr29 (&acc)[D + 1]) {
 #pragma unroll
-  for (int t = 0; t <= D; ++t)
-    if (t <= (int)deg) acc[t] = f29::reduce_est(acc[t]);
+  for (int t = 0; t <= D; ++t) acc[t] = f29::reduce_est(acc[t]);
 }
 
 template <int D>
@@ -206,7 +204,7 @@ __device__ __forceinline__ uint32_t job_of_block(const JobPack& pack) {
 // writes the canonical message.
 template <int D>
 __device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1], bool split) {
-  const uint32_t deg = job.degree;
+  constexpr uint32_t deg = D;
   if (job.blocks == 1) {
     // One block holds the whole message: no block partials, no ticket and no
     // device-scope fences (on this multi-XCD part each one writes back the
@@ -288,7 +286,6 @@ template <int D>
 __global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
-  const uint32_t deg = job.degree;
   const size_t stride = (size_t)job.blocks * kThreads;
   Fr29 acc[D + 1];
 #pragma unroll
@@ -309,17 +306,16 @@ __global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
       const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
       ProductTerms<D> pt;
       const Fr* t0 = tbl[job.slots[begin]];
-      pt.first(load29(job.coeffs29, p), load29(t0, 2 * b), load29(t0, 2 * b + 1), deg);
+      pt.first(load29(job.coeffs29, p), load29(t0, 2 * b), load29(t0, 2 * b + 1));
       for (uint32_t e = begin + 1; e < end; ++e) {
         const Fr* t = tbl[job.slots[e]];
-        pt.times(load29(t, 2 * b), load29(t, 2 * b + 1), deg);
+        pt.times(load29(t, 2 * b), load29(t, 2 * b + 1));
       }
 #pragma unroll
-      for (int t = 0; t <= D; ++t)
-        if (t <= (int)deg) acc[t] = f29::add(acc[t], pt.term[t]);
-      if ((p & 3) == 3) reduce_acc<D>(acc, deg);
+      for (int t = 0; t <= D; ++t) acc[t] = f29::add(acc[t], pt.term[t]);
+      if ((p & 3) == 3) reduce_acc<D>(acc);
     }
-    reduce_acc<D>(acc, deg);
+    reduce_acc<D>(acc);
   }
   Fr out[D + 1];
   to_stored<D>(acc, out);
@@ -334,7 +330,6 @@ template <int D>
 __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
-  const uint32_t deg = job.degree;
   const size_t items = (size_t)job.pairs * job.products;
   const size_t it = (size_t)lblock * kThreads + threadIdx.x;
   ProductTerms<D> pt;
@@ -364,9 +359,9 @@ __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
         hi = load29(t, 2 * b + 1);
       }
       if (e == begin)
-        pt.first(load29(job.coeffs29, p), lo, hi, deg);
+        pt.first(load29(job.coeffs29, p), lo, hi);
       else
-        pt.times(lo, hi, deg);
+        pt.times(lo, hi);
     }
   }
   Fr out[D + 1];
@@ -382,7 +377,7 @@ template <int D>
 __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
-  const uint32_t deg = job.degree, width = deg + 1;
+  constexpr uint32_t width = D + 1;
   const size_t items = (size_t)job.pairs * job.products * width;
   const size_t it = (size_t)lblock * kThreads + threadIdx.x;
   Fr out[D + 1];
@@ -426,6 +421,215 @@ __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
       if ((uint32_t)k == t) out[k] = v;
   }
   finish_job<D>(job, lblock, out, pack.split != 0);
+}
+
+// ---- persistent round loops for the last (small) rounds -------------------
+//
+// Once a state's table is small, a round's arithmetic takes a few
+// microseconds and a launch per round (launch, end-of-kernel write-back, the
+// host waking on the stream) costs several times that.  One block per state
+// then runs every remaining round in a loop and talks to the host through a
+// mailbox in pinned host memory: it publishes each message with a
+// system-scope release and waits for the host's next challenge, with a bounded
+// wait (a wall-clock limit and an iteration cap) and an abort flag, so every
+// wave always reaches the exit: all rounds done, abort from the host, or the
+// wait limit (the host then relaunches from the round it is at).
+constexpr unsigned kTailThreads = 512;
+constexpr uint32_t kTailRunning = 0, kTailDone = 1, kTailTimedOut = 2, kTailAborted = 3;
+constexpr uint32_t kTailMaxWidth = 9;  // degree <= 8
+// metadata limits of a state that runs its rounds in the loop (kept in LDS)
+constexpr uint32_t kTailMaxProducts = 32, kTailMaxEntries = 128, kTailMaxFactors = 64;
+
+struct TailJob {
+  const Fr* const* tables[3];  // inputs, fold slot A, fold slot B (device pointer arrays)
+  const Fr* coeffs29;
+  const uint32_t* offsets;
+  const uint32_t* slots;
+  const uint8_t* owner;
+  uint32_t products, degree, num_vars, first_round;
+  uint32_t entries, factors, pad0, pad1;
+  uint32_t challenge[8];  // stored (Montgomery) form of the first round's challenge
+};
+
+struct TailMailbox {
+  alignas(64) TailJob job;  // written by the host before the launch
+  // host -> device
+  alignas(64) uint32_t challenge_round;  // r: `challenge` holds round r's challenge
+  uint32_t abort;                        // 1: stop at the next wait
+  alignas(64) uint32_t challenge[8];
+  // device -> host
+  alignas(64) uint32_t message_round;  // r + 1: `message` holds round r's message
+  uint32_t status;                     // kTailRunning / Done / TimedOut / Aborted
+  alignas(64) uint32_t message[kTailMaxWidth * 8];  // stored (Montgomery) form
+};
+
+// Host-written words are read with relaxed system-scope loads (they bypass the
+// caches): an acquire would also invalidate the L2, and every table load of
+// the next round would then go to HBM.  The words after a flag are loaded
+// only once the flag's load has returned.
+__device__ __forceinline__ uint32_t sys_load_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void sys_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void sys_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Block k of the launch runs the state of boxes[k] from job.first_round to
+// its last round (or until it stops).  Each round: thread per (product p,
+// pair b, point t) items as in k_round_points, strided over the block; the
+// fold writes of round k are read by other threads in round k + 1 after the
+// block barrier (one workgroup, one CU).
+__global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const* boxes,
+                                                             uint64_t wait_ticks,
+                                                             uint32_t spin_cap) {
+  TailMailbox* box = boxes[blockIdx.x];
+  __shared__ TailJob job;
+  __shared__ uint32_t s_ch[8];
+  __shared__ uint32_t s_stop;
+  __shared__ Fr lds[kTailThreads / 64][kTailMaxWidth];
+  __shared__ Fr msg[kTailMaxWidth];
+  // the state's metadata, kept in LDS for all its rounds (one global load
+  // fewer per factor in every item)
+  __shared__ uint32_t s_off[kTailMaxProducts + 1];
+  __shared__ uint32_t s_slot[kTailMaxEntries];
+  __shared__ uint8_t s_owner[kTailMaxEntries];
+  __shared__ Fr s_coeff[kTailMaxProducts];
+  __shared__ const Fr* s_tab[3][kTailMaxFactors];
+  if (threadIdx.x < sizeof(TailJob) / 4)
+    reinterpret_cast<uint32_t*>(&job)[threadIdx.x] =
+        reinterpret_cast<const uint32_t*>(&box->job)[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x < 8) s_ch[threadIdx.x] = job.challenge[threadIdx.x];
+  if (threadIdx.x <= job.products) s_off[threadIdx.x] = job.offsets[threadIdx.x];
+  if (threadIdx.x < job.products) s_coeff[threadIdx.x] = load_fr(job.coeffs29, threadIdx.x);
+  if (threadIdx.x < job.entries) {
+    s_slot[threadIdx.x] = job.slots[threadIdx.x];
+    s_owner[threadIdx.x] = job.owner[threadIdx.x];
+  }
+  if (threadIdx.x < job.factors)
+    for (int t = 0; t < 3; ++t) s_tab[t][threadIdx.x] = job.tables[t][threadIdx.x];
+  __syncthreads();
+  const uint32_t n = job.num_vars, width = job.degree + 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ Fr29 s_c29;
+  for (uint32_t k = job.first_round;; ++k) {
+    if (threadIdx.x == 0) {
+      Fr ch;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) ch.v[i] = s_ch[i];
+      s_c29 = f29::prescale_fr(ch);  // once per round, for every thread
+    }
+    __syncthreads();
+    const Fr29 c29 = s_c29;
+    const size_t pairs = size_t(1) << (n - k - 1);
+    const bool fold = k > 0;
+    const Fr* const* src = k <= 1 ? s_tab[0] : s_tab[1 + ((k - 2) & 1)];
+    const Fr* const* dst = k == 0 ? nullptr : s_tab[1 + ((k - 1) & 1)];
+    // lazy Fr29 sums (each term < 3.3 r; reduced every 16 terms, below 56 r)
+    Fr29 acc29[kTailMaxWidth];
+#pragma unroll
+    for (uint32_t t = 0; t < kTailMaxWidth; ++t) acc29[t] = f29::zero_t<f29::FrP>();
+    uint32_t terms = 0;
+    const size_t items = pairs * job.products * width;
+    for (size_t it = threadIdx.x; it < items; it += kTailThreads) {
+      const uint32_t t = static_cast<uint32_t>(it % width);
+      const size_t pb = it / width;
+      const uint32_t p = static_cast<uint32_t>(pb / pairs);
+      const size_t b = pb - (size_t)p * pairs;
+      const uint32_t begin = s_off[p], end = s_off[p + 1];
+      Fr29 term = f29::unpack_fr(s_coeff[p]);
+      for (uint32_t e = begin; e < end; ++e) {
+        const uint32_t f = s_slot[e];
+        Fr29 lo, hi;
+        if (fold) {
+          const Fr* o = src[f];
+          const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+          const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+          lo = fold29(c29, a0, a1);
+          hi = fold29(c29, a2, a3);
+          if (t == 0 && s_owner[e]) {
+            store_fr(const_cast<Fr*>(dst[f]), 2 * b, f29::pack_fr_canonical(lo));
+            store_fr(const_cast<Fr*>(dst[f]), 2 * b + 1, f29::pack_fr_canonical(hi));
+          }
+        } else {
+          const Fr* tb = src[f];
+          lo = load29(tb, 2 * b);
+          hi = load29(tb, 2 * b + 1);
+        }
+        const Fr29 d = f29::sub1p(hi, lo);
+        Fr29 line = lo;
+        for (uint32_t q = 0; q < t; ++q) line = f29::add(line, d);
+        term = f29::mul(term, line);
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kTailMaxWidth; ++q)
+        if (q == t) acc29[q] = f29::add(acc29[q], term);
+      if ((++terms & 15u) == 0) {
+#pragma unroll
+        for (uint32_t q = 0; q < kTailMaxWidth; ++q) acc29[q] = f29::reduce_est(acc29[q]);
+      }
+    }
+    Fr acc[kTailMaxWidth];
+#pragma unroll
+    for (uint32_t t = 0; t < kTailMaxWidth; ++t)
+      acc[t] = t < width ? f29::pack_fr(f29::reduce_est(acc29[t])) : Fr::zero();
+    // message: wavefront sums, then one thread per point t
+#pragma unroll
+    for (uint32_t t = 0; t < kTailMaxWidth; ++t) {
+      if (t < width) {
+        const Fr w = wave_sum(acc[t]);
+        if (lane == 0) lds[wave][t] = w;
+      }
+    }
+    __syncthreads();  // also orders this round's fold writes before the next round's reads
+    if (threadIdx.x < width) {
+      Fr sum = Fr::zero();
+#pragma unroll
+      for (int w = 0; w < (int)(kTailThreads / 64); ++w) sum = sum + lds[w][threadIdx.x];
+      msg[threadIdx.x] = sum;  // stored form: the host converts to canonical
+    }
+    __syncthreads();
+    const bool last = k + 1 == n;
+    // wave 0 writes the message words (one store instruction per 64 words),
+    // then its lane 0 publishes the round: the release waits for the wave's
+    // own stores
+    if (threadIdx.x < 64) {
+      for (uint32_t i = threadIdx.x; i < width * 8; i += 64)
+        sys_store(&box->message[i], msg[i / 8].v[i % 8]);
+      if (threadIdx.x == 0) {
+        if (last) sys_store(&box->status, kTailDone);
+        sys_release(&box->message_round, k + 1);
+      }
+    }
+    if (last) return;
+    // the challenge of round k + 1 (bounded wait; abort from the host)
+    if (threadIdx.x == 0) {
+      uint32_t stop = kTailRunning;
+      const uint64_t start = static_cast<uint64_t>(wall_clock64());
+      for (uint32_t spin = 0;; ++spin) {
+        if (sys_load_relaxed(&box->abort) != 0) {
+          stop = kTailAborted;
+          break;
+        }
+        if (sys_load_relaxed(&box->challenge_round) == k + 1) {
+          for (int i = 0; i < 8; ++i) s_ch[i] = sys_load_relaxed(&box->challenge[i]);
+          break;
+        }
+        if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
+          stop = kTailTimedOut;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (stop != kTailRunning) sys_release(&box->status, stop);
+      s_stop = stop;
+    }
+    __syncthreads();
+    if (s_stop != kTailRunning) return;
+  }
 }
 
 // Any degree: message values for t in [t0, t0 + 8) only (host loops over t0).
@@ -502,9 +706,14 @@ struct State {
   const Fr* coeffs29 = nullptr;  // coefficients times 2^(5 m_p) (Fr29 product chains)
   const uint8_t* owner = nullptr;
   const Fr** table_ptrs[3] = {nullptr, nullptr, nullptr};  // inputs, slot A, slot B
-  size_t distinct = 0, product_count = 0;
+  size_t distinct = 0, product_count = 0, index_count = 0;
   rt::BufferRef partial, result, ticket;
   Fr* slot_base[2] = {nullptr, nullptr};
+  // persistent round loop: this state's mailbox (pinned host memory, from the
+  // context's pool) and whether its block may still wait on the host
+  TailMailbox* box = nullptr;
+  TailMailbox* dbox = nullptr;
+  bool tail_active = false;
 };
 
 }  // namespace sc
@@ -681,6 +890,249 @@ void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
                                     dim3(l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
     }
   }
+}
+
+// ---- persistent round loops: host side -------------------------------------
+//
+// A context keeps a pool of mailboxes and the list of states whose blocks may
+// still wait for a challenge.  At most one round-loop launch is active per
+// context: a new one (or any other work on the context stream) first stops
+// the running one (abort flags, then a stream drain).
+struct TailPool {
+  std::vector<sc::TailMailbox*> chunks;  // pinned blocks of kChunk mailboxes
+  std::vector<std::pair<sc::TailMailbox*, sc::TailMailbox*>> free;  // host, device
+  std::vector<sc::State*> active;
+  sc::TailMailbox** ptrs = nullptr;  // pinned: device mailbox pointers of a launch
+  sc::TailMailbox** dptrs = nullptr;
+  size_t ptrs_cap = 0;
+  uint64_t wait_ticks = 0;
+  static constexpr size_t kChunk = 64;
+};
+
+// The device waits this long for the host's next challenge before its block
+// stops (the host then relaunches from its round); HP_MI355X_SC_TAIL_WAIT_US.
+uint64_t tail_wait_us() {
+  static const uint64_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_TAIL_WAIT_US");
+    return e ? static_cast<uint64_t>(std::strtoull(e, nullptr, 0)) : uint64_t(100000);
+  }();
+  return v;
+}
+
+// A state runs its rounds in the loop once a round has at most this many
+// (product, pair, point) items: HP_MI355X_SC_TAIL_ITEMS for rounds of several
+// states at once (round_batch; default 4096: measured ~5 % faster on the
+// 64-sumcheck batch), HP_MI355X_SC_TAIL_SINGLE_ITEMS for one state's rounds
+// (default 0, off: one block per round computes no faster than the launch
+// path's several, and the handshake saves about what the launch costs, so
+// measured no gain alone); 0 turns the loop off.
+size_t tail_items(bool batch) {
+  static const size_t batch_v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_TAIL_ITEMS");
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(4096);
+  }();
+  static const size_t single_v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_TAIL_SINGLE_ITEMS");
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(0);
+  }();
+  return batch ? batch_v : single_v;
+}
+
+constexpr size_t kTailMaxStates = 128;
+
+bool tail_ok(const sc::State& s, bool batch) {
+  if (s.max_degree > kMaxJobDegree || s.product_count == 0 || s.round >= s.num_vars) return false;
+  if (s.product_count > sc::kTailMaxProducts || s.index_count > sc::kTailMaxEntries ||
+      s.distinct > sc::kTailMaxFactors)
+    return false;
+  const size_t pairs = size_t(1) << (s.num_vars - s.round - 1);
+  return pairs * s.product_count * (s.max_degree + 1) <= tail_items(batch);
+}
+
+void quiesce_tails(rt::Context& c) {
+  auto* pool = static_cast<TailPool*>(c.tails);
+  if (pool == nullptr || pool->active.empty()) return;
+  for (sc::State* s : pool->active) __atomic_store_n(&s->box->abort, 1u, __ATOMIC_RELEASE);
+  for (sc::State* s : pool->active) s->tail_active = false;
+  pool->active.clear();
+  rt::hip_check(hipStreamSynchronize(c.stream));  // every block has seen its abort and exited
+}
+
+void release_tail_pool(rt::Context& c) {
+  auto* pool = static_cast<TailPool*>(c.tails);
+  if (pool == nullptr) return;
+  for (sc::TailMailbox* m : pool->chunks) (void)hipHostFree(m);
+  if (pool->ptrs) (void)hipHostFree(pool->ptrs);
+  delete pool;
+  c.tails = nullptr;
+  c.quiesce = nullptr;
+  c.release_tails = nullptr;
+}
+
+TailPool& tail_pool(rt::Context& c) {
+  if (c.tails == nullptr) {
+    auto* pool = new TailPool();
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) != hipSuccess || khz <= 0) {
+      (void)hipGetLastError();
+      khz = 100000;  // 100 MHz
+    }
+    pool->wait_ticks = tail_wait_us() * static_cast<uint64_t>(khz) / 1000;
+    c.tails = pool;
+    c.quiesce = &quiesce_tails;
+    c.release_tails = &release_tail_pool;
+  }
+  return *static_cast<TailPool*>(c.tails);
+}
+
+void tail_mailbox(TailPool& pool, sc::State& s) {
+  if (s.box) return;
+  if (pool.free.empty()) {
+    void* raw = nullptr;
+    rt::hip_check(hipHostMalloc(&raw, TailPool::kChunk * sizeof(sc::TailMailbox),
+                                hipHostMallocCoherent | hipHostMallocMapped));
+    pool.chunks.push_back(static_cast<sc::TailMailbox*>(raw));
+    void* draw = nullptr;
+    rt::hip_check(hipHostGetDevicePointer(&draw, raw, 0));
+    auto* h = static_cast<sc::TailMailbox*>(raw);
+    auto* d = static_cast<sc::TailMailbox*>(draw);
+    for (size_t i = 0; i < TailPool::kChunk; ++i) pool.free.emplace_back(h + i, d + i);
+  }
+  s.box = pool.free.back().first;
+  s.dbox = pool.free.back().second;
+  pool.free.pop_back();
+}
+
+void tail_return_mailbox(rt::Context& c, sc::State& s) {
+  if (s.box == nullptr) return;
+  auto* pool = static_cast<TailPool*>(c.tails);
+  if (pool) pool->free.emplace_back(s.box, s.dbox);
+  s.box = s.dbox = nullptr;
+}
+
+// One launch, one block per state, from each state's current round with its
+// challenge (stored form; ignored at round 0).  Precondition: no active loop.
+void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>& states,
+                  const std::vector<Fr>& challenges) {
+  if (pool.ptrs_cap < states.size()) {
+    if (pool.ptrs) rt::hip_check(hipHostFree(pool.ptrs));
+    pool.ptrs = nullptr;
+    pool.ptrs_cap = 0;
+    const size_t cap = std::max<size_t>(states.size(), 64);
+    void* raw = nullptr;
+    rt::hip_check(hipHostMalloc(&raw, cap * sizeof(void*), hipHostMallocCoherent | hipHostMallocMapped));
+    pool.ptrs = static_cast<sc::TailMailbox**>(raw);
+    void* draw = nullptr;
+    rt::hip_check(hipHostGetDevicePointer(&draw, raw, 0));
+    pool.dptrs = static_cast<sc::TailMailbox**>(draw);
+    pool.ptrs_cap = cap;
+  }
+  for (size_t i = 0; i < states.size(); ++i) {
+    sc::State& s = *states[i];
+    tail_mailbox(pool, s);
+    sc::TailJob& j = s.box->job;
+    for (int t = 0; t < 3; ++t) j.tables[t] = s.table_ptrs[t];
+    j.coeffs29 = s.coeffs29;
+    j.offsets = s.prods.offsets;
+    j.slots = s.prods.slots;
+    j.owner = s.owner;
+    j.products = static_cast<uint32_t>(s.product_count);
+    j.degree = static_cast<uint32_t>(s.max_degree);
+    j.num_vars = static_cast<uint32_t>(s.num_vars);
+    j.first_round = static_cast<uint32_t>(s.round);
+    j.entries = static_cast<uint32_t>(s.index_count);
+    j.factors = static_cast<uint32_t>(s.distinct);
+    std::memcpy(j.challenge, challenges[i].v, sizeof j.challenge);
+    s.box->challenge_round = 0;
+    s.box->abort = 0;
+    s.box->message_round = 0;
+    s.box->status = sc::kTailRunning;
+    pool.ptrs[i] = s.dbox;
+  }
+  __atomic_thread_fence(__ATOMIC_SEQ_CST);
+  const uint32_t spin_cap = 50000000u;
+  hipLaunchKernelGGL(sc::k_round_tail, dim3(static_cast<unsigned>(states.size())), dim3(sc::kTailThreads),
+                     0, c.stream, pool.dptrs, pool.wait_ticks, spin_cap);
+  rt::check_launch();
+  for (sc::State* s : states) {
+    s->tail_active = true;
+    pool.active.push_back(s);
+  }
+}
+
+void tail_post(sc::State& s, const Fr& challenge) {
+  std::memcpy(s.box->challenge, challenge.v, sizeof s.box->challenge);
+  __atomic_store_n(&s.box->challenge_round, static_cast<uint32_t>(s.round), __ATOMIC_RELEASE);
+}
+
+// Wait for the message of the state's current round: true with it in `out`
+// (width canonical Fr), false if the block stopped first (wait limit or abort).
+bool tail_wait(rt::Context& c, sc::State& s, uint8_t* out) {
+  const uint32_t want = static_cast<uint32_t>(s.round) + 1;
+  const auto start = std::chrono::steady_clock::now();
+  for (uint32_t spin = 0;; ++spin) {
+    if (__atomic_load_n(&s.box->message_round, __ATOMIC_ACQUIRE) == want) break;
+    const uint32_t st = __atomic_load_n(&s.box->status, __ATOMIC_ACQUIRE);
+    if (st == sc::kTailTimedOut || st == sc::kTailAborted) {
+      // the block may have published this round just before stopping
+      if (__atomic_load_n(&s.box->message_round, __ATOMIC_ACQUIRE) == want) break;
+      return false;
+    }
+    if ((spin & 1023u) == 1023u) {
+      if (std::chrono::steady_clock::now() - start > std::chrono::seconds(60)) {
+        // the device never answered: stop it and report a device failure
+        quiesce_tails(c);
+        rt::fail_state(-4);
+      }
+      std::this_thread::yield();
+    } else {
+      __builtin_ia32_pause();
+    }
+  }
+  // stored (Montgomery) form -> canonical
+  for (size_t t = 0; t <= s.max_degree; ++t) {
+    Fr v;
+    std::memcpy(v.v, s.box->message + 8 * t, rt::kFrBytes);
+    v = from_mont(v);
+    std::memcpy(out + rt::kFrBytes * t, v.v, rt::kFrBytes);
+  }
+  return true;
+}
+
+// After a state's round came back through its loop.
+void tail_advance(TailPool& pool, sc::State& s) {
+  s.round += 1;
+  if (s.round == s.num_vars) {  // its block returned after the last message
+    s.tail_active = false;
+    pool.active.erase(std::remove(pool.active.begin(), pool.active.end(), &s), pool.active.end());
+  }
+}
+
+// Rounds of `states` (all tail_ok) through the loop: post or launch, wait.
+// Returns false (nothing advanced, no loop active) when a block stopped
+// before answering twice in a row; the caller then uses the launch path.
+bool tail_rounds(rt::Context& c, const std::vector<sc::State*>& states, const std::vector<Fr>& chs,
+                 const std::vector<uint8_t*>& outs) {
+  TailPool& pool = tail_pool(c);
+  bool launch = false;
+  for (sc::State* s : states) launch = launch || !s->tail_active;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (launch) {
+      quiesce_tails(c);
+      launch_tails(c, pool, states, chs);
+    } else {
+      for (size_t i = 0; i < states.size(); ++i) tail_post(*states[i], chs[i]);
+    }
+    bool all = true;
+    for (size_t i = 0; i < states.size() && all; ++i) all = tail_wait(c, *states[i], outs[i]);
+    if (all) {
+      for (sc::State* s : states) tail_advance(pool, *s);
+      return true;
+    }
+    launch = true;  // a block stopped (wait limit): relaunch every state once
+  }
+  quiesce_tails(c);
+  return false;
 }
 
 // ---- round launches as hipGraphs, replayed with new challenges ----------
@@ -932,6 +1384,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     s->inputs = distinct;
     s->distinct = distinct.size();
     s->product_count = product_count;
+    s->index_count = index_count;
     const size_t F = distinct.size();
     if (F > 0 && num_vars >= 2) {
       const size_t a = size_t(1) << (num_vars - 1), b = size_t(1) << (num_vars - 2);
@@ -1021,7 +1474,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
 extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
     HpMi355xContext* ctx, HpMi355xSumCheckState* state, const uint8_t* challenge_or_null,
     uint8_t* out_evaluations, size_t evaluation_count) {
-  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+  return hp_guarded<false>(ctx, [&](rt::Context& c) -> HpMi355xStatus {
     const HpMi355xStatus st =
         check_round(ctx, c, state, challenge_or_null, out_evaluations, evaluation_count);
     if (st.kind != HP_MI355X_STATUS_SUCCESS) return st;
@@ -1029,6 +1482,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
     try {
       const size_t width = s.max_degree + 1;
       const Fr ch = challenge_of(s, challenge_or_null);
+      if (tail_ok(s, false) && tail_rounds(c, {&s}, {ch}, {out_evaluations})) return rt::ok();
+      quiesce_tails(c);
       if (s.max_degree <= kMaxJobDegree && round_graphs_enabled()) {
         std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr)};
         std::vector<uint8_t> out(width * rt::kFrBytes);
@@ -1065,7 +1520,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
     HpMi355xContext* ctx, HpMi355xSumCheckState* const* states,
     const uint8_t* const* challenges_or_null, uint8_t* const* out_evaluations,
     const size_t* evaluation_counts, size_t count) {
-  return hp_guarded(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+  return hp_guarded<false>(ctx, [&](rt::Context& c) -> HpMi355xStatus {
     if (count == 0) return rt::ok();
     if (states == nullptr || out_evaluations == nullptr || evaluation_counts == nullptr)
       return rt::invalid_parameters();
@@ -1086,6 +1541,21 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
       total += ss[i]->max_degree + 1;
     }
     try {
+      // every block of a loop launch must be resident at once (one block of
+      // 512 threads per CU): at most 128 states
+      const bool batch = count > 1;
+      bool all_tail = count <= kTailMaxStates;
+      for (size_t i = 0; i < count; ++i) all_tail = all_tail && tail_ok(*ss[i], batch);
+      if (all_tail) {
+        std::vector<Fr> chs(count);
+        std::vector<uint8_t*> outs(count);
+        for (size_t i = 0; i < count; ++i) {
+          chs[i] = challenge_of(*ss[i], challenges_or_null ? challenges_or_null[i] : nullptr);
+          outs[i] = out_evaluations[i];
+        }
+        if (tail_rounds(c, ss, chs, outs)) return rt::ok();
+      }
+      quiesce_tails(c);
       bool all_jobs = true;
       for (size_t i = 0; i < count; ++i) all_jobs = all_jobs && ss[i]->max_degree <= kMaxJobDegree;
       if (all_jobs && round_graphs_enabled()) {
@@ -1142,6 +1612,17 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_destroy(HpMi355xCon
     if (state == nullptr || !c.holds(state, rt::Context::Kind::sumcheck_state))
       return rt::invalid_parameters();
     if (state->ctx != ctx) return rt::invalid_parameters();
+    sc::State& s = *state->impl;
+    if (s.tail_active) {
+      // its block may still wait on the host: stop the loop before the
+      // state's memory and mailbox are recycled
+      try {
+        quiesce_tails(c);
+      } catch (const rt::Failure& f) {
+        c.poison(f.status.native_code);
+      }
+    }
+    tail_return_mailbox(c, s);
     c.resources.erase(state);
     delete state;
     return rt::ok();

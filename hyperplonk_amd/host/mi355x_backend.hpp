@@ -421,6 +421,60 @@ class Mi355xBackend {
     return res;
   }
 
+  // extension: the same batch, submitted to run beside later calls; the
+  // results are written by wait() (call it exactly once)
+  class KzgPending {
+   public:
+    KzgPending() = default;
+    KzgPending(KzgPending&&) = default;
+    KzgPending& operator=(KzgPending&&) = default;
+    std::vector<KzgResult> wait() {
+      if (!ticket_) invalid("batch already waited for");
+      const auto lock = ctx_->guard();
+      HpMi355xKzgTicket* t = ticket_;
+      ticket_ = nullptr;
+      check(hp_mi355x_v1_multilinear_kzg_batch_wait(ctx_->raw(), t), "kzg_batch_wait");
+      return std::move(*res_);
+    }
+    ~KzgPending() {
+      if (ticket_) {
+        const auto lock = ctx_->guard();
+        (void)hp_mi355x_v1_multilinear_kzg_batch_wait(ctx_->raw(), ticket_);
+      }
+    }
+
+   private:
+    friend class Mi355xBackend;
+    std::shared_ptr<Context> ctx_;
+    HpMi355xKzgTicket* ticket_ = nullptr;
+    std::unique_ptr<std::vector<KzgResult>> res_;  // stable output storage
+  };
+  KzgPending kzg_batch_submit(const PreparedProverParam& pp, const std::vector<KzgOp>& ops) const {
+    const auto lock = ctx_->guard();
+    KzgPending pending;
+    pending.ctx_ = ctx_;
+    pending.res_ = std::make_unique<std::vector<KzgResult>>(ops.size());
+    std::vector<KzgResult>& res = *pending.res_;
+    std::vector<HpMi355xKzgRequest> reqs(ops.size());
+    for (size_t i = 0; i < ops.size(); ++i) {
+      const KzgOp& op = ops[i];
+      own(op.polynomial);
+      const size_t nv = op.polynomial.num_vars();
+      if (op.open && op.point.size() != nv) invalid("polynomial num_vars does not match point length");
+      res[i].points.resize(op.open ? nv : 1);
+      reqs[i].kind = op.open ? HP_MI355X_KZG_OPEN : HP_MI355X_KZG_COMMIT;
+      reqs[i].polynomial = op.polynomial.address();
+      reqs[i].num_vars = nv;
+      reqs[i].point = op.open && nv ? op.point[0].data() : nullptr;
+      reqs[i].out_points = res[i].points.empty() ? nullptr : res[i].points[0].data();
+      reqs[i].out_evaluation = op.open ? res[i].value.data() : nullptr;
+    }
+    check(hp_mi355x_v1_multilinear_kzg_batch_submit(ctx_->raw(), pp.raw(), reqs.data(), reqs.size(),
+                                                    &pending.ticket_),
+          "kzg_batch_submit");
+    return pending;
+  }
+
   // extension: one round of several states in shared launches
   std::vector<std::vector<Fr>> prove_rounds_batch(std::vector<ProverState*>& states,
                                                   const std::vector<std::optional<Fr>>& challenges) const {

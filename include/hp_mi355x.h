@@ -233,6 +233,22 @@ HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch(HpMi355xContext *ctx,
                                                   const HpMi355xKzgParams *params,
                                                   const HpMi355xKzgRequest *requests,
                                                   size_t count);
+/* Asynchronous multilinear_kzg_batch: the same checks (all before any work),
+ * then the MSMs are queued on a second stream of the context, behind the work
+ * already queued on the context, and the call returns at once with a ticket;
+ * other calls on the context (sumcheck rounds of other proofs) then overlap
+ * the MSMs on the device.  The requests' output buffers must stay valid until
+ * batch_wait, which waits for the work, writes every output (identical to
+ * multilinear_kzg_batch) and releases the ticket; call it once per ticket.
+ * Input tables and parameters stay alive until then even if released.  Used
+ * by the batched prover to overlap commitments with sumchecks. */
+typedef struct HpMi355xKzgTicket HpMi355xKzgTicket;
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch_submit(HpMi355xContext *ctx,
+                                                         const HpMi355xKzgParams *params,
+                                                         const HpMi355xKzgRequest *requests,
+                                                         size_t count, HpMi355xKzgTicket **out);
+HpMi355xStatus hp_mi355x_v1_multilinear_kzg_batch_wait(HpMi355xContext *ctx,
+                                                       HpMi355xKzgTicket *ticket);
 /* Canonical SRS files (MultilinearUniversalParams serialize_compressed, the
  * format hyperplonk/examples/full_prove_bench.rs:665-760 loads and writes):
  * fill one layer from point_count compressed G1 points (48 bytes each:

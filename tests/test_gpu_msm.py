@@ -264,3 +264,53 @@ def test_autotune_changes_plans_not_results(backend, oracle, rng_factory):
             backend.msm_set_tuning([{"log_n": 6, "window": 3, "cap": 4}])
     finally:
         backend.msm_set_tuning([{"log_n": 6, "window": 0, "cap": 0}, {"log_n": 9, "window": 0, "cap": 0}])
+
+
+def test_kzg_batch_submit_overlaps_and_matches(backend, oracle, rng_factory, srs20):
+    """An asynchronous KZG batch (side stream) gives the packets and values of
+    the synchronous batch and of the oracle while sumcheck rounds, table
+    uploads and a retire of one of its tables run on the context meanwhile;
+    invalid requests are refused before any work and a ticket is waited for
+    once."""
+    import ctypes
+
+    from hyperplonk_amd import VirtualPolynomial, abi
+
+    N, t, pp = srs20
+    rng = rng_factory(0xA5C)
+    nvs = [16, 3, 12, 0, 9]
+    tabs = [rng.fr_bytes(1 << nv) for nv in nvs]
+    mles = [backend.mle_from_evaluations(nv, tb) for nv, tb in zip(nvs, tabs)]
+    ops = [("commit", mles[0]), ("open", mles[1], fr_from_bytes(rng.fr_bytes(3).tobytes())),
+           ("commit", mles[2]), ("open", mles[3], []), ("commit", mles[4])]
+    want = backend.kzg_batch(pp, ops)
+    ticket = backend.kzg_batch_submit(pp, ops)
+    # other work on the context while the batch runs
+    nv = 10
+    st_tabs = [rng.fr_bytes(1 << nv) for _ in range(3)]
+    st_mles = [backend.mle_from_evaluations(nv, tb) for tb in st_tabs]
+    vp = VirtualPolynomial(nv)
+    vp.add_mle_list([st_mles[0], st_mles[1]], 3)
+    vp.add_mle_list([st_mles[2]], 5)
+    st = backend.prover_init(vp)
+    ost = oracle.SumCheckState(nv, 2, st_tabs, [(3, [0, 1]), (5, [2])])
+    ch = None
+    srng = pf.SplitMix64(0xA5D)
+    for _ in range(nv):
+        assert backend.prove_round_and_update_state(st, ch) == ost.round(ch)
+        ch = srng.nonboolean()
+    # a table of the pending batch retired: the ticket keeps its memory alive
+    assert backend.lib.hp_mi355x_v1_buffer_retire(backend.context, mles[2].address).kind == 0
+    mles[2].address = 0
+    got = ticket.wait()
+    assert got == want
+    assert got[0][0] == oracle.msm_pippenger_packets(backend.download_layer(pp, N - 16), tabs[0].tobytes(),
+                                                     1 << 16, 16)
+    with pytest.raises(InvalidParameters):
+        ticket.wait()
+    # a bad request: nothing is submitted
+    out = ctypes.c_void_p()
+    req = (backend._KzgRequestType * 1)()
+    req[0].kind = 7
+    st_ = backend.lib.hp_mi355x_v1_multilinear_kzg_batch_submit(backend.context, pp.raw, req, 1, ctypes.byref(out))
+    assert st_.kind == abi.INVALID_PARAMETERS and out.value is None

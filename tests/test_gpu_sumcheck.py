@@ -297,3 +297,108 @@ def test_round_graph_replay_gives_same_messages(degree):
         assert res.returncode == 0, res.stderr
         digests.append(json.loads(res.stdout.strip().splitlines()[-1])["messages_sha3_8"])
     assert digests[0] == digests[1]
+
+
+@pytest.mark.parametrize("degree", ["2", "5"])
+def test_round_loops_give_the_same_messages(degree):
+    """The persistent round loops change no message: whole n = 12 sumchecks
+    of one state with the loops off (default for one state), on for rounds of
+    at most 4096 items, taking every round from the first (a large item
+    limit), and with a zero device wait limit (every block stops right after
+    its message, so each round relaunches the loop from the state's round)."""
+    import json
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prog = os.path.join(root, "_build", "prove_bench")
+    digests = []
+    for env_extra in ({}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096"},
+                      {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "100000000"},
+                      {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096", "HP_MI355X_SC_TAIL_WAIT_US": "0"}):
+        env = dict(os.environ, **env_extra)
+        res = subprocess.run([prog, "sumcheck", "12", degree, "2", "3"], capture_output=True, text=True,
+                             timeout=120, env=env)
+        assert res.returncode == 0, (env_extra, res.stderr[-2000:])
+        digests.append(json.loads(res.stdout.strip().splitlines()[-1])["messages_sha3_8"])
+    assert len(set(digests)) == 1, digests
+
+
+def test_round_loops_interleaved_states_and_other_work(backend, oracle, rng_factory):
+    """Three states advanced two at a time in alternating pairs (each batch
+    stops the running loop and starts one for its pair), one state alone
+    between them (the launch path), and other work on the context between
+    rounds (which stops the running loop), each round byte-exact against the
+    oracle; a state destroyed while its loop waits, then the context keeps
+    working."""
+    nv = 9
+    rng = rng_factory(0x7A11)
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(4)]
+    mles = [backend.mle_from_evaluations(nv, t) for t in tabs]
+    specs = [[(3, [0, 1]), (R - 5, [2])], [(7, [1, 2, 3]), (2, [0, 0])], [(5, [3, 3, 1])]]
+    states, osts = [], []
+    for prods in specs:
+        vp = VirtualPolynomial(nv)
+        for c, idx in prods:
+            vp.add_mle_list([mles[i] for i in idx], c)
+        states.append(backend.prover_init(vp))
+        osts.append(oracle.SumCheckState(nv, vp.aux_info.max_degree, tabs, prods))
+    srng = pf.SplitMix64(0x7A12)
+    ch = [None, None, None]
+    for rnd in range(nv):
+        for group in ((0, 1), (2,)) if rnd % 2 == 0 else ((1, 2), (0,)):
+            got = backend.prove_rounds_batch([states[i] for i in group], [ch[i] for i in group])
+            for i, msg in zip(group, got):
+                assert msg == osts[i].round(ch[i]), (i, rnd)
+                ch[i] = srng.nonboolean()
+        if rnd % 3 == 1:
+            pt = [srng.field() for _ in range(nv)]
+            assert backend.evaluate_mle(mles[0], pt) == oracle.mle_evaluate(tabs[0], pt)
+    # states stopped in the middle of their loop
+    sts, osts2 = [], []
+    for prods in ([(11, [0, 3])], [(4, [1, 2])]):
+        vp = VirtualPolynomial(nv)
+        for c, idx in prods:
+            vp.add_mle_list([mles[i] for i in idx], c)
+        sts.append(backend.prover_init(vp))
+        osts2.append(oracle.SumCheckState(nv, 2, tabs, prods))
+    c = [None, None]
+    for _ in range(nv - 2):
+        got = backend.prove_rounds_batch(sts, c)
+        assert got == [osts2[0].round(c[0]), osts2[1].round(c[1])]
+        c = [srng.nonboolean(), srng.nonboolean()]
+    del sts
+    assert backend.evaluate_mle(mles[1], [1] * nv) == oracle.mle_evaluate(tabs[1], [1] * nv)
+
+
+def test_round_loops_batches_gain_members(backend, oracle, rng_factory):
+    """round_batch through the loops: states join the batch's loop in later
+    rounds (the running loop is stopped and relaunched with every state), all
+    messages byte-exact against the oracle."""
+    rng = rng_factory(0x7A13)
+    specs = [(5, [(1, [0, 1])]), (8, [(5, [0, 1, 2]), (R - 1, [2])]), (6, [(3, [0, 0, 1, 1, 2])]),
+             (7, [(2, [0, 1])]), (9, [(4, [2, 1])])]
+    states, osts = [], []
+    for nv, prods in specs:
+        tabs = [rng.fr_bytes(1 << nv) for _ in range(3)]
+        mles = [backend.mle_from_evaluations(nv, t) for t in tabs]
+        vp = VirtualPolynomial(nv)
+        for c, idx in prods:
+            vp.add_mle_list([mles[i] for i in idx], c)
+        states.append(backend.prover_init(vp))
+        osts.append(oracle.SumCheckState(nv, vp.aux_info.max_degree, tabs, prods))
+    srng = pf.SplitMix64(0x7A14)
+    chal = [None] * len(specs)
+    rounds = [0] * len(specs)
+    step = 0
+    while True:
+        # states join one by one over the first steps
+        live = [i for i, (nv, _) in enumerate(specs) if rounds[i] < nv and i <= step]
+        if not live:
+            break
+        got = backend.prove_rounds_batch([states[i] for i in live], [chal[i] for i in live])
+        for i, msg in zip(live, got):
+            assert msg == osts[i].round(chal[i]), (i, rounds[i])
+            rounds[i] += 1
+            chal[i] = srng.nonboolean()
+        step += 1
