@@ -20,9 +20,10 @@ oracle prover as its CPU baseline), `sumcheck` (n = 20 whole proofs, degree 2
 and 5), `batch` (13 uneven commits + 64 sumchecks, serial vs batched, and the
 standalone MSM sizes 2^16..2^22), `srs` (device SRS generation with the
 canonical file write, then the file read back and decompressed on the device,
-nv 20) and `serving` (a Poisson request trace served
-with continuous batching vs serially: proofs/s and p50/p99 latency; the full
-2000-request, nv 10..20 trace with --serve-full).
+nv 20) and `serving` (the 2000-request, nv 10..20 Poisson prover-service
+stream at 0.8 x the measured batched capacity, served with continuous batching
+under the measured HBM budget and again serially: proofs/s and p50/p99
+latency; a 300-request nv 10..16 trace with --serve-short).
 """
 import argparse
 import json
@@ -73,8 +74,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="use the built-in window rule, no autotuning")
     ap.add_argument("--cap", type=int, default=0,
                     help="lane task cap of the headline size with the built-in window rule (no autotuning)")
-    ap.add_argument("--serve-full", action="store_true",
-                    help="serve the full 2000-request trace (nv 10..20) instead of the short one")
+    ap.add_argument("--serve-short", action="store_true",
+                    help="serve a 300-request nv 10..16 trace instead of the 2000-request nv 10..20 stream")
     return ap.parse_args()
 
 
@@ -137,10 +138,16 @@ def workloads(args, be, rng, peaks):
     # SRS: device generation + canonical file write, then file read + device decompression (nv 20)
     with tempfile.TemporaryDirectory() as tmp:
         out["srs"] = run_json([PROVE_BENCH, "srs", "20", os.path.join(tmp, "srs_nv20.bin")], 300)
-    if args.serve_full:
-        out["serving"] = run_json([PROVE_BENCH, "serve", "2000", "auto", "10", "20", "0.3", "16", "5"], 3000)
-    else:
+    if args.serve_short:
         out["serving"] = run_json([PROVE_BENCH, "serve", "300", "auto", "10", "16", "0.3", "16", "5"], 600)
+        out["serving"]["trace"] = "reduced: 300 requests, nv 10..16"
+    else:
+        # the prover-service stream: 2000 Poisson requests (30 % jellyfish, nv 10..20) at
+        # 0.8 x the measured batched capacity, SRS resident for nv <= 22, admission by
+        # slots and the measured HBM budget, batched vs serial FIFO on the same trace
+        out["serving"] = run_json([PROVE_BENCH, "serve", "2000", "cap", "10", "20", "0.3", "16", "5",
+                                   "auto", "22", "single"], 900)
+        out["serving"]["trace"] = "2000 requests, nv 10..20, 30 % jellyfish, SRS nv 22"
     return out
 
 

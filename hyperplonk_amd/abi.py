@@ -66,6 +66,7 @@ SIGNATURES = {
     "hp_mi355x_v1_mle_fold_profile": [_vp, _vp, _sz, _u8p, _sz, ctypes.POINTER(ctypes.c_float)],
     "hp_mi355x_v1_mle_evaluate_batch": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_vp), _u8p, _sz],
     "hp_mi355x_v1_debug_inject_failure": [_vp, ctypes.c_uint32, ctypes.c_int32],
+    "hp_mi355x_v1_memory_info": [_vp, _szp, _szp, _szp, _szp, ctypes.c_int32],
     "hp_mi355x_v1_multilinear_kzg_batch_submit": [_vp, _vp, _vp, _sz, _vpp],
     "hp_mi355x_v1_multilinear_kzg_batch_wait": [_vp, _vp],
     "hp_mi355x_v1_mle_evaluate_noalloc": [_vp, _vp, _sz, _u8p, _u8p],

@@ -81,6 +81,8 @@ void* Allocator::acquire(size_t bytes, size_t* capacity) {
     it->second.pop_back();
     cached_ -= it->first;
     *capacity = it->first;
+    live_ += it->first;
+    peak_ = std::max(peak_, live_);
     return p;
   }
   if (cached_ > limit_) trim();
@@ -96,6 +98,8 @@ void* Allocator::acquire(size_t bytes, size_t* capacity) {
     }
   }
   *capacity = size;
+  live_ += size;
+  peak_ = std::max(peak_, live_);
   return p;
 }
 
@@ -103,6 +107,7 @@ void Allocator::release(void* ptr, size_t capacity) {
   if (!ptr) return;
   free_[capacity].push_back(ptr);
   cached_ += capacity;
+  live_ -= std::min(live_, capacity);
 }
 
 void Allocator::trim() {
@@ -419,6 +424,23 @@ extern "C" HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext* ctx) {
   // first error (bridge_runtime.cpp:1317)
   if (poisoned) return rt::invalid_state(first_error);
   return rt::ok();
+}
+
+extern "C" HpMi355xStatus hp_mi355x_v1_memory_info(HpMi355xContext* ctx, size_t* device_free,
+                                                   size_t* device_total, size_t* context_live,
+                                                   size_t* context_peak, int32_t reset_peak) {
+  return hp_guarded<false>(ctx, [&](rt::Context& c) -> HpMi355xStatus {
+    if (device_free == nullptr || device_total == nullptr || context_live == nullptr ||
+        context_peak == nullptr)
+      return rt::invalid_parameters();
+    rt::hip_check(hipMemGetInfo(device_free, device_total));
+    // blocks in the context's cache are free for the context's own use
+    *device_free += c.alloc->cached_bytes();
+    *context_live = c.alloc->live_bytes();
+    *context_peak = c.alloc->peak_bytes();
+    if (reset_peak) c.alloc->reset_peak();
+    return rt::ok();
+  });
 }
 
 extern "C" HpMi355xStatus hp_mi355x_v1_debug_inject_failure(HpMi355xContext* ctx, uint32_t after,

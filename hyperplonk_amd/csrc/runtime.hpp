@@ -78,12 +78,17 @@ class Allocator {
   void release(void* ptr, size_t capacity);
   void trim();  // synchronises, then frees every cached block
   size_t cached_bytes() const { return cached_; }
+  // bytes handed out and not yet released, and their high-water mark
+  size_t live_bytes() const { return live_; }
+  size_t peak_bytes() const { return peak_; }
+  void reset_peak() { peak_ = live_; }
 
  private:
   static size_t round_up(size_t bytes);
   hipStream_t stream_;
   std::map<size_t, std::vector<void*>> free_;
   size_t cached_ = 0;
+  size_t live_ = 0, peak_ = 0;
   size_t limit_ = size_t(4) << 30;
 };
 

@@ -475,6 +475,19 @@ class Mi355xBackend {
     return pending;
   }
 
+  // extension: device memory and this context's live / peak bytes
+  struct MemoryInfo {
+    size_t device_free = 0, device_total = 0, live = 0, peak = 0;
+  };
+  MemoryInfo memory_info(bool reset_peak = false) const {
+    const auto lock = ctx_->guard();
+    MemoryInfo m;
+    check(hp_mi355x_v1_memory_info(ctx_->raw(), &m.device_free, &m.device_total, &m.live, &m.peak,
+                                   reset_peak ? 1 : 0),
+          "memory_info");
+    return m;
+  }
+
   // extension: one round of several states in shared launches
   std::vector<std::vector<Fr>> prove_rounds_batch(std::vector<ProverState*>& states,
                                                   const std::vector<std::optional<Fr>>& challenges) const {

@@ -15,13 +15,19 @@
 //             states in shared launches) and overlapped (the commit batch
 //             submitted to the side stream while the 64 sumchecks run);
 //             outputs must be bit-identical
-//   serve     <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>
+//   serve     <requests> <rate|0|auto|cap> <nv_min> <nv_max> <jelly_frac> <slots> <seed>
+//             [<budget bytes|auto> [<srs nv> [single]]]
 //             a request trace (Poisson arrivals at `rate` per second; 0 = all at
-//             once; auto = 0.8 x the serial capacity; both capacities measured
-//             on a closed-loop probe of the same request mix), then again at a
-//             high load (0.9 x the batched capacity, above the serial one) served with continuous batching and again serially
-//             (FIFO, one proof at a time); latency = completion - arrival;
-//             every proof of the two runs must be identical
+//             once; auto = 0.8 x the serial capacity, cap = 0.8 x the batched
+//             capacity; both capacities measured on a closed-loop probe of the
+//             same request mix), then (unless `single`) again at a high load
+//             (0.9 x the batched capacity, above the serial one), each served
+//             with continuous batching and again serially (FIFO, one proof at
+//             a time); latency = completion - arrival; every proof of the two
+//             runs must be identical.  Admission by slots and an HBM budget
+//             (bytes, or auto: 90 % of the free device memory) against
+//             per-class footprints measured on the device first; one SRS of
+//             max(nv_max, srs nv) variables serves every request
 //   srs       <nv> <path>
 //             device SRS generation from a seeded trapdoor, compression on the
 //             device and the canonical file write (load_or_generate_srs), then
@@ -338,10 +344,16 @@ static double busy_ms(const serve::ServeReport& rep) {
   return t;
 }
 
+struct ServeOptions {
+  size_t budget = 0;   // 0: measured (90 % of the free device memory)
+  size_t srs_nv = 0;   // SRS size (at least nv_max): one resident SRS serves every nv (trim)
+  bool single = false; // only the first rate (no high-load pair)
+};
+
 static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, size_t nv_max, double jelly,
-                     size_t slots, uint64_t seed) {
+                     size_t slots, uint64_t seed, const ServeOptions& opt) {
   mi355x::Mi355xBackend be(0);
-  const size_t N = nv_max;
+  const size_t N = std::max(nv_max, opt.srs_nv);
   SplitMix64 trng(seed ^ 0x5352535f54415553ULL);
   std::vector<Fr> t;
   for (size_t i = 0; i < N; ++i) t.push_back(trng.field());
@@ -366,6 +378,13 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   t0 = Clock::now();
   const serve::CircuitPool payloads(all, cfg.variants, cfg.gen_threads);
   const double pool_ms = ms_since(t0);
+  // admission: per-class device footprints measured on this device, and the budget
+  t0 = Clock::now();
+  cfg.footprints = serve::measure_footprints(be, pp, vk, all, payloads);
+  const double footprint_ms = ms_since(t0);
+  cfg.hbm_budget = opt.budget;
+  size_t max_footprint = 0;
+  for (const auto& kv : cfg.footprints) max_footprint = std::max(max_footprint, kv.second);
   // closed-loop probes (everything at t = 0): the capacity of each mode
   double capacity_serial = 0;
   {
@@ -377,8 +396,11 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
     capacity_serial = probe / srep.span_s;
   }
   // auto: a fixed rate the serial prover sustains (0.8 x its capacity), so
-  // both modes keep up and the latencies compare like for like
-  rate = rate_arg == "auto" ? 0.8 * capacity_serial : std::atof(rate_arg.c_str());
+  // both modes keep up and the latencies compare like for like; cap: 0.8 x
+  // the batched capacity (the prover-service stream's definition)
+  rate = rate_arg == "auto"  ? 0.8 * capacity_serial
+         : rate_arg == "cap" ? 0.8 * capacity
+                             : std::atof(rate_arg.c_str());
   trace = serve::make_trace(count, rate, nv_min, nv_max, jelly, seed);
   serve::ServeConfig batched = cfg;
   const auto brep = serve::run_trace(be, pp, vk, trace, payloads, batched);
@@ -400,15 +422,21 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
   const Summary b = summarize(trace, brep), s = summarize(trace, srep);
   // high load: 0.9 x the batched capacity, above what the serial prover sustains
   const double rate_hi = 0.9 * capacity;
-  const auto trace_hi = serve::make_trace(count, rate_hi, nv_min, nv_max, jelly, seed);
-  const auto brep_hi = serve::run_trace(be, pp, vk, trace_hi, payloads, batched);
-  const auto srep_hi = serve::run_trace(be, pp, vk, trace_hi, payloads, serial);
+  std::vector<serve::Request> trace_hi;
+  serve::ServeReport brep_hi, srep_hi;
   size_t identical_hi = 0;
-  for (size_t i = 0; i < trace_hi.size(); ++i)
-    identical_hi += brep_hi.outcomes[i].error.empty() &&
-                    brep_hi.outcomes[i].proof == srep_hi.outcomes[i].proof &&
-                    brep_hi.outcomes[i].proof == brep.outcomes[i].proof;
-  const Summary bh = summarize(trace_hi, brep_hi), sh = summarize(trace_hi, srep_hi);
+  Summary bh, sh;
+  if (!opt.single) {
+    trace_hi = serve::make_trace(count, rate_hi, nv_min, nv_max, jelly, seed);
+    brep_hi = serve::run_trace(be, pp, vk, trace_hi, payloads, batched);
+    srep_hi = serve::run_trace(be, pp, vk, trace_hi, payloads, serial);
+    for (size_t i = 0; i < trace_hi.size(); ++i)
+      identical_hi += brep_hi.outcomes[i].error.empty() &&
+                      brep_hi.outcomes[i].proof == srep_hi.outcomes[i].proof &&
+                      brep_hi.outcomes[i].proof == brep.outcomes[i].proof;
+    bh = summarize(trace_hi, brep_hi);
+    sh = summarize(trace_hi, srep_hi);
+  }
   size_t jelly_n = 0;
   for (const auto& r : trace) jelly_n += r.jellyfish;
   std::printf(
@@ -425,7 +453,10 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
       "\"high_load\": {\"rate_per_s\": %.3f, \"batched\": {\"proofs_per_s\": %.3f, \"p50_ms\": %.2f, "
       "\"p99_ms\": %.2f, \"mean_ms\": %.2f, \"errors\": %zu}, \"serial\": {\"proofs_per_s\": %.3f, "
       "\"p50_ms\": %.2f, \"p99_ms\": %.2f, \"mean_ms\": %.2f, \"errors\": %zu}, \"identical_proofs\": %zu}, "
-      "\"verified\": %zu, \"verify_checked\": %zu}\n",
+      "\"verified\": %zu, \"verify_checked\": %zu, \"rate_rule\": \"%s\", \"hbm_budget_bytes\": %zu, "
+      "\"budget_waits\": %zu, \"peak_admitted_bytes\": %zu, \"device_peak_bytes\": %zu, "
+      "\"footprint_classes\": %zu, \"max_footprint_bytes\": %zu, \"footprint_measure_ms\": %.0f, "
+      "\"high_load_run\": %s}\n",
       trace.size(), jelly_n, nv_min, nv_max, rate, capacity, capacity_serial, probe, payloads.size(),
       pool_ms, slots, N,
       srs_ms, b.proofs_per_s, b.p50, b.p99,
@@ -434,7 +465,10 @@ static int run_serve(size_t count, const std::string& rate_arg, size_t nv_min, s
       brep.stats.gather_ms, brep.stats.generic_ms,
       brep.stats.kzg_ms, brep.stats.round_ms, brep.stats.eval_ms, brep.stats.waiting_ms, busy_ms(brep), s.proofs_per_s, s.p50, s.p99, s.mean, s.errors, identical,
       rate_hi, bh.proofs_per_s, bh.p50, bh.p99, bh.mean, bh.errors, sh.proofs_per_s, sh.p50, sh.p99,
-      sh.mean, sh.errors, identical_hi, verified, checked);
+      sh.mean, sh.errors, identical_hi, verified, checked,
+      rate_arg == "auto" ? "0.8 x serial capacity" : rate_arg == "cap" ? "0.8 x batched capacity" : "given",
+      brep.hbm_budget, brep.budget_waits, brep.peak_admitted, brep.device_peak, cfg.footprints.size(),
+      max_footprint, footprint_ms, opt.single ? "false" : "true");
   return identical == trace.size() && identical_hi == trace_hi.size() && verified == checked ? 0 : 1;
 }
 
@@ -474,10 +508,15 @@ int main(int argc, char** argv) {
                        std::strtoull(argv[6], nullptr, 0));
     if (mode == "batch" && argc == 4)
       return run_batch(std::atoi(argv[2]), std::strtoull(argv[3], nullptr, 0));
-    if (mode == "serve" && argc == 9)
+    if (mode == "serve" && argc >= 9 && argc <= 12) {
+      ServeOptions opt;
+      if (argc > 9 && std::string(argv[9]) != "auto") opt.budget = std::strtoull(argv[9], nullptr, 0);
+      if (argc > 10) opt.srs_nv = std::strtoul(argv[10], nullptr, 10);
+      if (argc > 11) opt.single = std::string(argv[11]) == "single";
       return run_serve(std::strtoul(argv[2], nullptr, 10), argv[3], std::strtoul(argv[4], nullptr, 10),
                        std::strtoul(argv[5], nullptr, 10), std::atof(argv[6]), std::strtoul(argv[7], nullptr, 10),
-                       std::strtoull(argv[8], nullptr, 0));
+                       std::strtoull(argv[8], nullptr, 0), opt);
+    }
     if (mode == "srs" && argc == 4) return run_srs(std::strtoul(argv[2], nullptr, 10), argv[3]);
     if (mode == "sumcheck" && argc == 6)
       return run_sumcheck(std::strtoul(argv[2], nullptr, 10), std::atoi(argv[3]), std::atoi(argv[4]),
@@ -490,7 +529,8 @@ int main(int argc, char** argv) {
                "usage: %s prove <nv> <vanilla|jellyfish> <proofs> <warmup> <seed>\n"
                "       %s sumcheck <nv> <2|5> <reps> <seed>\n"
                "       %s batch <reps> <seed>\n"
-               "       %s serve <requests> <rate|0|auto> <nv_min> <nv_max> <jelly_frac> <slots> <seed>\n"
+               "       %s serve <requests> <rate|0|auto|cap> <nv_min> <nv_max> <jelly_frac> <slots> <seed>"
+               " [<budget bytes|auto> [<srs nv> [single]]]\n"
                "       %s srs <nv> <path>\n",
                argv[0], argv[0], argv[0], argv[0], argv[0]);
   return 2;

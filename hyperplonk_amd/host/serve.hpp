@@ -12,8 +12,9 @@
 //     multilinear_kzg_batch (all MSMs of all requests in one launch sequence),
 //     sumcheck round_batch (all rounds in shared launches), evaluate_batch.
 // Requests arrive on a trace; a request is admitted when a slot is free and
-// its estimated device footprint fits the HBM budget, so new requests join
-// the batches of those already on the GPU.  A proof depends only on its own
+// its device footprint (measured per request class on the device before the
+// trace) fits the HBM budget (measured free device memory, or a set value),
+// so new requests join the batches of those already on the GPU.  A proof depends only on its own
 // request, so it is byte-identical alone and in any mix.
 #pragma once
 #include <algorithm>
@@ -400,10 +401,17 @@ inline host::MockCircuit circuit_of(const Request& r, size_t variants) {
   return r.jellyfish ? host::jellyfish_circuit(r.nv, seed) : host::vanilla_circuit(r.nv, seed);
 }
 
-// Device bytes a request may hold at its peak: its tables (witnesses,
-// selectors, permutation oracles, nums/denoms, frac/prod/factors, eq tables),
-// the sumcheck fold slots and MSM working memory, ~64 tables of 2^nv Fr.
-inline size_t footprint_bytes(const Request& r) { return size_t(64) * 32 * (size_t(1) << r.nv); }
+// Device bytes a request may hold at its peak when nothing was measured for
+// its class: its tables (witnesses, selectors, permutation oracles,
+// nums/denoms, frac/prod/factors, eq tables), the sumcheck fold slots and MSM
+// working memory, ~64 tables of 2^nv Fr.
+inline size_t estimated_footprint(const Request& r) { return size_t(64) * 32 * (size_t(1) << r.nv); }
+
+// Measured footprints by request class (jellyfish, nv): the context's peak
+// live device bytes over one serial preprocess + prove of the class, above
+// what was live before, with a 25 % margin (batched MSM work memory of a tick
+// is shared between the requests in it).
+using Footprints = std::map<std::pair<bool, size_t>, size_t>;
 
 class CircuitPool {
  public:
@@ -447,7 +455,10 @@ struct Outcome {
 
 struct ServeConfig {
   size_t slots = 16;
-  size_t hbm_budget = size_t(64) << 30;
+  // device bytes the admitted requests may hold together; 0: measured, 90 %
+  // of the device memory free when the trace starts (the SRS already resident)
+  size_t hbm_budget = 0;
+  Footprints footprints;  // per class; classes missing here use estimated_footprint
   size_t gen_threads = 8;  // threads that build the payload pool
   size_t variants = 2;     // pooled circuits per (gate, nv)
   std::chrono::microseconds window{300};
@@ -461,7 +472,36 @@ struct ServeReport {
   double span_s = 0;  // first arrival to last completion
   Scheduler::Stats stats;
   size_t max_concurrent = 0;
+  size_t hbm_budget = 0;      // the budget admission used
+  size_t budget_waits = 0;    // admissions that waited for memory with a slot free
+  size_t peak_admitted = 0;   // largest sum of admitted footprints
+  size_t device_peak = 0;     // the context's peak live bytes over the run
 };
+
+inline size_t footprint_of(const ServeConfig& cfg, const Request& r) {
+  auto it = cfg.footprints.find({r.jellyfish, r.nv});
+  return it != cfg.footprints.end() ? it->second : estimated_footprint(r);
+}
+
+inline Footprints measure_footprints(const Mi355xBackend& be, const mi355x::PreparedProverParam& pp,
+                                     const host::MultilinearVerifierParam& vk,
+                                     const std::vector<Request>& trace, const CircuitPool& payloads) {
+  Footprints out;
+  for (const Request& r : trace) {
+    const std::pair<bool, size_t> key{r.jellyfish, r.nv};
+    if (out.count(key)) continue;
+    auto c = payloads.get(r);
+    const size_t before = be.memory_info(true).live;
+    {
+      auto [pk, vkey] = host::preprocess(be, c->index, pp, vk);
+      (void)host::prove(be, pk, c->public_inputs, c->witnesses);
+    }
+    const size_t peak = be.memory_info(false).peak;
+    const size_t used = peak > before ? peak - before : 0;
+    out[key] = used + used / 4;
+  }
+  return out;
+}
 
 // Run the trace.  Batched: admission by slots and HBM budget, every admitted
 // request on its own thread through the Scheduler.  Serial: FIFO, one at a
@@ -503,6 +543,9 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
       note_done();
     }
   } else {
+    size_t budget = cfg.hbm_budget;
+    if (budget == 0) budget = be.memory_info().device_free / 10 * 9;
+    rep.hbm_budget = budget;
     Scheduler sched(be, cfg.window, cfg.kzg_hold);
     BatchingBackend bb(be, sched);
     std::mutex mu;
@@ -535,21 +578,24 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
         {
           std::lock_guard<std::mutex> g(mu);
           --running;
-          used -= std::min(footprint_bytes(r), cfg.hbm_budget);
+          used -= std::min(footprint_of(cfg, r), budget);
         }
         cv.notify_all();
       }
     };
+    (void)be.memory_info(true);  // peak from here
     std::vector<std::thread> pool;
     for (size_t i = 0; i < cfg.slots; ++i) pool.emplace_back(worker);
     for (const Request& r : trace) {
       auto c = payloads.get(r);
       wait_arrival(r);
-      const size_t need = std::min(footprint_bytes(r), cfg.hbm_budget);
+      const size_t need = std::min(footprint_of(cfg, r), budget);
       std::unique_lock<std::mutex> lk(mu);
-      cv.wait(lk, [&] { return running < cfg.slots && used + need <= cfg.hbm_budget; });
+      if (running < cfg.slots && used + need > budget) ++rep.budget_waits;
+      cv.wait(lk, [&] { return running < cfg.slots && used + need <= budget; });
       ++running;
       used += need;
+      rep.peak_admitted = std::max(rep.peak_admitted, used);
       rep.max_concurrent = std::max(rep.max_concurrent, running);
       sched.join();
       rep.outcomes[r.id].start_s = since(Clock::now());
@@ -564,6 +610,7 @@ inline ServeReport run_trace(const Mi355xBackend& be, const mi355x::PreparedProv
     cv.notify_all();
     for (auto& t : pool) t.join();
     rep.stats = sched.stats();
+    rep.device_peak = be.memory_info().peak;
   }
   double last = 0;
   for (const auto& o : rep.outcomes) last = std::max(last, o.done_s);

@@ -279,6 +279,16 @@ HpMi355xStatus hp_mi355x_v1_mle_evaluate_noalloc(HpMi355xContext *ctx,
                                                  HpMi355xDeviceAddress polynomial,
                                                  size_t num_vars, const uint8_t *point,
                                                  uint8_t *out_evaluation);
+/* Device memory as a prover service admits work by it (no counterpart in the
+ * reference): *device_free = free device memory plus the context's cached
+ * blocks, *device_total = the device's memory, *context_live = bytes of the
+ * context's live buffers (tables, prover states, parameters, work memory of
+ * calls in flight), *context_peak = their high-water mark since the last
+ * reset (reset_peak != 0 resets it to the current live bytes after
+ * reading). */
+HpMi355xStatus hp_mi355x_v1_memory_info(HpMi355xContext *ctx, size_t *device_free,
+                                        size_t *device_total, size_t *context_live,
+                                        size_t *context_peak, int32_t reset_peak);
 /* Roofline denominators measured on this device (no counterpart in the
  * reference): *out_hbm_bytes_per_s from a streaming copy of copy_bytes (read
  * plus write bytes over the kernel time), *out_mad_per_s from a u32 x u32 ->

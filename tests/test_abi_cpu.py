@@ -13,7 +13,7 @@ from hyperplonk_amd.virtual_polynomial import encode_products
 def test_library_exports_every_header_symbol():
     lib = abi.load()
     declared = abi.header_symbols()
-    assert len(declared) == 37
+    assert len(declared) == 38
     for name in declared:
         assert hasattr(lib, name), name
     # the 20 entry points of the reference bridge (bridge.h:43-62) are all present

@@ -27,3 +27,15 @@ def test_served_proofs_identical_to_serial_proofs():
     # the mix really overlapped: more than one request on the device at once
     assert out["batched"]["max_concurrent"] > 1
     assert out["batched"]["ops"] > out["batched"]["ticks"]
+
+
+def test_served_proofs_under_a_binding_budget_match_the_oracle():
+    """A budget of 2.5 of the largest measured request footprints holds
+    requests back (admission waits with slots free); served proofs still
+    equal the serial ones, and sampled served proofs equal the CPU oracle
+    backend's proofs of the same circuits (tests/cpp/serve_oracle_test.cpp)."""
+    prog = os.path.join(ROOT, "_build", "serve_oracle_test")
+    assert os.path.exists(prog), "build the test programs first (make -C tests/cpp)"
+    res = subprocess.run([prog, "24", "5", "9", "8", "2.5", "21"], capture_output=True, text=True, timeout=600)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0 and "SERVE_ORACLE PASS" in res.stdout, res.stdout + res.stderr
