@@ -67,6 +67,7 @@ SIGNATURES = {
     "hp_mi355x_v1_mle_evaluate_batch": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_vp), _u8p, _sz],
     "hp_mi355x_v1_debug_inject_failure": [_vp, ctypes.c_uint32, ctypes.c_int32],
     "hp_mi355x_v1_memory_info": [_vp, _szp, _szp, _szp, _szp, ctypes.c_int32],
+    "hp_mi355x_v1_build_info": [ctypes.c_char_p, _sz],
     "hp_mi355x_v1_multilinear_kzg_batch_submit": [_vp, _vp, _vp, _sz, _vpp],
     "hp_mi355x_v1_multilinear_kzg_batch_wait": [_vp, _vp],
     "hp_mi355x_v1_mle_evaluate_noalloc": [_vp, _vp, _sz, _u8p, _u8p],
@@ -119,3 +120,27 @@ def size_array(values):
     for i, v in enumerate(values):
         arr[i] = v
     return arr
+
+
+def source_digest() -> str:
+    """SHA-256 of the library's sources as the build computes it: every .hip and
+    .hpp file of hyperplonk_amd/csrc in name order, then include/hp_mi355x.h."""
+    import hashlib
+
+    csrc = os.path.join(HERE, "csrc")
+    names = sorted(n for n in os.listdir(csrc) if n.endswith((".hip", ".hpp")))
+    h = hashlib.sha256()
+    for n in names:
+        with open(os.path.join(csrc, n), "rb") as f:
+            h.update(f.read())
+    with open(HEADER_PATH, "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()
+
+
+def build_info() -> str:
+    buf = ctypes.create_string_buffer(128)
+    st = load().hp_mi355x_v1_build_info(buf, len(buf))
+    if st.kind != SUCCESS:
+        raise LibraryMissing("build_info failed")
+    return buf.value.decode()

@@ -289,6 +289,10 @@ HpMi355xStatus hp_mi355x_v1_mle_evaluate_noalloc(HpMi355xContext *ctx,
 HpMi355xStatus hp_mi355x_v1_memory_info(HpMi355xContext *ctx, size_t *device_free,
                                         size_t *device_total, size_t *context_live,
                                         size_t *context_peak, int32_t reset_peak);
+/* Build provenance (no counterpart in the reference): "sources-sha256=<hex>
+ * arch=gfx950" into out (capacity >= 96 bytes), the digest of the sources the
+ * library was compiled from (hyperplonk_amd/csrc/build_info.hip). */
+HpMi355xStatus hp_mi355x_v1_build_info(char *out, size_t capacity);
 /* Roofline denominators measured on this device (no counterpart in the
  * reference): *out_hbm_bytes_per_s from a streaming copy of copy_bytes (read
  * plus write bytes over the kernel time), *out_mad_per_s from a u32 x u32 ->

@@ -13,7 +13,7 @@ from hyperplonk_amd.virtual_polynomial import encode_products
 def test_library_exports_every_header_symbol():
     lib = abi.load()
     declared = abi.header_symbols()
-    assert len(declared) == 38
+    assert len(declared) == 39
     for name in declared:
         assert hasattr(lib, name), name
     # the 20 entry points of the reference bridge (bridge.h:43-62) are all present
@@ -28,6 +28,14 @@ def test_library_exports_every_header_symbol():
     for n in ref_names:
         assert "hp_mi355x_v1_" + n in declared
     assert set(declared) == set(abi.SIGNATURES)
+
+
+def test_library_was_built_from_the_sources_in_the_tree():
+    """The library reports the SHA-256 of the sources it was compiled from;
+    it must equal the digest of the files in this tree (a stale or foreign
+    build fails here)."""
+    info = abi.build_info()
+    assert info == f"sources-sha256={abi.source_digest()} arch=gfx950", info
 
 
 def test_null_and_unknown_context_are_rejected_without_a_device():

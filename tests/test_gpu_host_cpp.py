@@ -28,3 +28,11 @@ def test_static_archive_program_known_answers():
     assert os.path.exists(prog), "build the test programs first (make -C tests/cpp)"
     res = subprocess.run([prog], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0 and "STATIC_GPU PASS" in res.stdout, (res.returncode, res.stdout, res.stderr)
+
+
+def test_loaded_library_matches_the_sources_on_this_machine():
+    """On the GPU machine too, the library the tests load was compiled from the
+    sources in this tree (its embedded source digest equals theirs)."""
+    from hyperplonk_amd import abi
+
+    assert abi.build_info() == f"sources-sha256={abi.source_digest()} arch=gfx950"
