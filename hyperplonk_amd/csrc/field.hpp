@@ -99,8 +99,19 @@ HP_HD bool geq_modulus(const uint32_t* a) {
 }
 
 // Conditionally subtract p from a (a < 2p): returns a mod p.
+// Device code chains the hardware carry (v_add_co / v_addc, v_sub_co / v_subb:
+// one instruction per limb) where the portable form takes several 64-bit
+// operations per limb.
 template <class P>
 HP_HD void reduce_once(uint32_t* a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t t[P::N];
+  unsigned borrow = 0;
+#pragma unroll
+  for (int i = 0; i < P::N; ++i) t[i] = __builtin_subc(a[i], P::MOD[i], borrow, &borrow);
+#pragma unroll
+  for (int i = 0; i < P::N; ++i) a[i] = borrow ? a[i] : t[i];
+#else
   uint32_t t[P::N];
   uint32_t borrow = 0;
 #pragma unroll
@@ -112,11 +123,19 @@ HP_HD void reduce_once(uint32_t* a) {
   // borrow == 0  <=>  a >= p  -> take t
 #pragma unroll
   for (int i = 0; i < P::N; ++i) a[i] = borrow ? a[i] : t[i];
+#endif
 }
 
 template <class P>
 HP_HD Fe<P> operator+(const Fe<P>& a, const Fe<P>& b) {
   Fe<P> r;
+#if defined(__HIP_DEVICE_COMPILE__)
+  {
+    unsigned c = 0;
+#pragma unroll
+    for (int i = 0; i < P::N; ++i) r.v[i] = __builtin_addc(a.v[i], b.v[i], c, &c);
+  }
+#else
   uint32_t carry = 0;
 #pragma unroll
   for (int i = 0; i < P::N; ++i) {
@@ -124,6 +143,7 @@ HP_HD Fe<P> operator+(const Fe<P>& a, const Fe<P>& b) {
     r.v[i] = (uint32_t)s;
     carry = (uint32_t)(s >> 32);
   }
+#endif
   // both moduli are below 2^(32N - 1): the sum never carries out.
   reduce_once<P>(r.v);
   return r;
@@ -132,6 +152,17 @@ HP_HD Fe<P> operator+(const Fe<P>& a, const Fe<P>& b) {
 template <class P>
 HP_HD Fe<P> operator-(const Fe<P>& a, const Fe<P>& b) {
   Fe<P> r;
+#if defined(__HIP_DEVICE_COMPILE__)
+  {
+    unsigned bo = 0;
+#pragma unroll
+    for (int i = 0; i < P::N; ++i) r.v[i] = __builtin_subc(a.v[i], b.v[i], bo, &bo);
+    const uint32_t mask = 0u - bo;
+    unsigned c = 0;
+#pragma unroll
+    for (int i = 0; i < P::N; ++i) r.v[i] = __builtin_addc(r.v[i], P::MOD[i] & mask, c, &c);
+  }
+#else
   uint32_t borrow = 0;
 #pragma unroll
   for (int i = 0; i < P::N; ++i) {
@@ -148,6 +179,7 @@ HP_HD Fe<P> operator-(const Fe<P>& a, const Fe<P>& b) {
     r.v[i] = (uint32_t)s;
     carry = (uint32_t)(s >> 32);
   }
+#endif
   return r;
 }
 

@@ -441,6 +441,13 @@ HP_HD Fr pack_fr(const Fr29& a) {
   return r;
 }
 
+// canonical packed Fr from a value below 4 r (a reduced value)
+HP_HD Fr pack_fr_reduced(const Fr29& a) {
+  Fr r;
+  pack_t(cond_sub(cond_sub(a, FrP::P2), FrP::P), r.v);
+  return r;
+}
+
 // packed Fr of a value already below r
 HP_HD Fr pack_fr_canonical(const Fr29& a) {
   Fr r;

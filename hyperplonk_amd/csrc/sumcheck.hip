@@ -120,6 +120,12 @@ struct RoundJob {
   Fr* out;                 // degree + 1 canonical values
   Fr challenge;            // Montgomery
   uint32_t pairs, products, factors, degree, fold, items_mode, block_begin, blocks;
+  // product classes (mode 3): per class {m, output offset, member begin, member
+  // end}, then the member product indices; class coefficients (Montgomery)
+  // times 2^(5 (m - 1))
+  const uint32_t* classes;
+  const Fr* class_k;
+  uint32_t class_count, class_width;
 };
 
 constexpr int kJobsPerLaunch = 16;
@@ -320,6 +326,246 @@ __global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
   Fr out[D + 1];
   to_stored<D>(acc, out);
   finish_job<D>(job, lblock, out, pack.split != 0);
+}
+
+// ---- class-major pairs decomposition (large tables) -------------------------
+//
+// Products of equal arity m and equal coefficient form a class.  A product of
+// m lines is a polynomial of degree m in t, so a class's products are
+// evaluated without the coefficient at t = 0..m only: every factor costs m + 1
+// products instead of d + 1, and the two coefficient products per product and
+// pair disappear.  Each thread sums its pairs per (class, t), the block adds
+// the threads' sums, and k_finish_classes adds the blocks, multiplies each
+// class's m + 1 sums by its coefficient, extends them to t = 0..d by finite
+// differences (exact for a polynomial of degree m) and adds the classes.
+constexpr uint32_t kMaxClasses = 16, kMaxClassWidth = 64;
+
+// Sum over the thread's pairs of the class's products at t = 0..M, below 3r.
+// Fr29 on the stored form: a product of m stored values (a 2^256) through
+// m - 1 Fr29 products is prod(a) 2^(256 - 5 (m - 1)); the class coefficient
+// carries the missing 2^(5 (m - 1)).
+// Bounds (r the modulus): lo, hi canonical, d < 2r, lines < (2M + 1) r <= 17r,
+// a one-factor term < 3r, a product term < 5.1r (17^2 / 70.7 + 1); eight
+// terms on a reduced accumulator (< 3r) stay below 44r < 2^261.
+template <int M>
+__device__ __forceinline__ void class_pass(const RoundJob& job, const Fr* const* tbl, uint32_t mb,
+                                           uint32_t me, size_t b0, size_t stride,
+                                           Fr29 (&out)[M + 1]) {
+  Fr29 acc[M + 1];
+#pragma unroll
+  for (int t = 0; t <= M; ++t) acc[t] = f29::zero_t<f29::FrP>();
+  const uint32_t* members = job.classes + 4 * job.class_count;
+  uint32_t pending = 0;
+  for (size_t b = b0; b < job.pairs; b += stride) {
+    for (uint32_t q = mb; q < me; ++q) {
+      const uint32_t begin = job.offsets[members[q]];
+      Fr29 term[M + 1];
+      {
+        const Fr* t0 = tbl[job.slots[begin]];
+        const Fr29 lo = load29(t0, 2 * b), hi = load29(t0, 2 * b + 1);
+        const Fr29 d = f29::sub1p(hi, lo);
+        Fr29 line = lo;
+#pragma unroll
+        for (int t = 0; t <= M; ++t) {
+          term[t] = line;
+          if (t < M) line = f29::add(line, d);
+        }
+      }
+#pragma unroll 1
+      for (int e = 1; e < M; ++e) {
+        const Fr* tt = tbl[job.slots[begin + e]];
+        const Fr29 lo = load29(tt, 2 * b), hi = load29(tt, 2 * b + 1);
+        const Fr29 d = f29::sub1p(hi, lo);
+        Fr29 line = lo;
+#pragma unroll
+        for (int t = 0; t <= M; ++t) {
+          term[t] = f29::mul(term[t], line);
+          if (t < M) line = f29::add(line, d);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t <= M; ++t) acc[t] = f29::add(acc[t], term[t]);
+      if (++pending == 8) {
+#pragma unroll
+        for (int t = 0; t <= M; ++t) acc[t] = f29::reduce_est(acc[t]);
+        pending = 0;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t <= M; ++t) out[t] = f29::reduce_est(acc[t]);
+}
+
+__device__ __forceinline__ Fr29 shfl_xor29(const Fr29& v, int offset) {
+  Fr29 r;
+#pragma unroll
+  for (int i = 0; i < f29::FrP::L; ++i) r.l[i] = __shfl_xor(v.l[i], offset, 64);
+  return r;
+}
+
+// Sums of W values over the 64 lanes of a wavefront, stored canonical at
+// row[idx].  Reduce-scatter first: at each of the first log2(Wp) levels (Wp
+// the power of two >= W) a lane keeps half of its values and adds its
+// partner's copy of that half, Wp - 1 additions in all instead of W per level;
+// then the lanes that hold the same value add over the remaining levels.
+// Lane l ends with value idx(l) = sum_k bit_(5-k)(l) Wp / 2^(k+1); the lanes
+// whose low 6 - log2(Wp) bits are zero store it.  Inputs below 3r; a
+// reduction after the fourth level keeps every partial sum below 48r < 2^261.
+template <int W>
+__device__ __forceinline__ void wave_reduce_store(const Fr29 (&v)[W], Fr* row) {
+  constexpr int Wp = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : W <= 8 ? 8 : 16;
+  constexpr int LW = Wp == 1 ? 0 : Wp == 2 ? 1 : Wp == 4 ? 2 : Wp == 8 ? 3 : 4;
+  const int lane = threadIdx.x & 63;
+  Fr29 x[Wp];
+#pragma unroll
+  for (int i = 0; i < Wp; ++i) x[i] = i < W ? v[i] : f29::zero_t<f29::FrP>();
+#pragma unroll
+  for (int k = 0; k < LW; ++k) {
+    const int o = 32 >> k, half = Wp >> (k + 1);
+    const bool upper = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      Fr29 send, keep;
+#pragma unroll
+      for (int q = 0; q < f29::FrP::L; ++q) {
+        send.l[q] = upper ? x[i].l[q] : x[i + half].l[q];
+        keep.l[q] = upper ? x[i + half].l[q] : x[i].l[q];
+      }
+      x[i] = f29::add(keep, shfl_xor29(send, o));
+      if (k == 3) x[i] = f29::reduce_est(x[i]);
+    }
+  }
+#pragma unroll
+  for (int k = LW; k < 6; ++k) {
+    x[0] = f29::add(x[0], shfl_xor29(x[0], 32 >> k));
+    if (k == 3) x[0] = f29::reduce_est(x[0]);
+  }
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; k < LW; ++k)
+    if (lane & (32 >> k)) idx += Wp >> (k + 1);
+  if ((lane & ((64 >> LW) - 1)) == 0 && idx < W) row[idx] = f29::pack_fr_reduced(f29::reduce_est(x[0]));
+}
+
+template <int M>
+__device__ __forceinline__ void class_run(const RoundJob& job, const Fr* const* tbl, uint32_t c,
+                                          size_t b0, size_t stride, Fr (*lds)[kMaxClassWidth]) {
+  Fr29 v[M + 1];
+  class_pass<M>(job, tbl, job.classes[4 * c + 2], job.classes[4 * c + 3], b0, stride, v);
+  wave_reduce_store<M + 1>(v, &lds[threadIdx.x >> 6][job.classes[4 * c + 1]]);
+}
+
+// Thread per pair (grid-strided): fold every distinct factor on the pair
+// (folding rounds), then every class over the thread's pairs; the block's sums
+// per (class, t) go to job.partial[block][class offset + t].
+// At most 256 registers per lane (VGPRs and AGPRs together): two waves per
+// SIMD, which the degree-5 passes would otherwise lose by a few registers.
+template <int D>
+__global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pack) {
+  const RoundJob& job = pack.job[job_of_block(pack)];
+  const uint32_t lblock = blockIdx.x - job.block_begin;
+  const size_t stride = (size_t)job.blocks * kThreads;
+  const size_t b0 = (size_t)lblock * kThreads + threadIdx.x;
+  const Fr* const* tbl = job.fold ? job.dst : job.src;
+  if (job.fold) {
+    const Fr29 c29 = f29::prescale_fr(job.challenge);
+    for (size_t b = b0; b < job.pairs; b += stride) {
+      for (uint32_t f = 0; f < job.factors; ++f) {
+        const Fr* o = job.src[f];
+        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+        store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(fold29(c29, a0, a1)));
+        store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(fold29(c29, a2, a3)));
+      }
+    }
+    // each thread reads back only the folded pairs it wrote itself
+  }
+  __shared__ Fr lds[kThreads / 64][kMaxClassWidth];
+  for (uint32_t c = 0; c < job.class_count; ++c) {
+    switch (job.classes[4 * c]) {
+      case 1: class_run<1>(job, tbl, c, b0, stride, lds); break;
+      case 2: if constexpr (D >= 2) class_run<2>(job, tbl, c, b0, stride, lds); break;
+      case 3: if constexpr (D >= 3) class_run<3>(job, tbl, c, b0, stride, lds); break;
+      case 4: if constexpr (D >= 4) class_run<4>(job, tbl, c, b0, stride, lds); break;
+      case 5: if constexpr (D >= 5) class_run<5>(job, tbl, c, b0, stride, lds); break;
+      case 6: if constexpr (D >= 6) class_run<6>(job, tbl, c, b0, stride, lds); break;
+      case 7: if constexpr (D >= 7) class_run<7>(job, tbl, c, b0, stride, lds); break;
+      case 8: if constexpr (D >= 8) class_run<8>(job, tbl, c, b0, stride, lds); break;
+      default: break;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < job.class_width) {
+    Fr s = Fr::zero();
+#pragma unroll
+    for (int w = 0; w < (int)(kThreads / 64); ++w) s = s + lds[w][threadIdx.x];
+    store_fr(job.partial, (size_t)lblock * job.class_width + threadIdx.x, s);
+  }
+}
+
+// Second phase of a class launch: block j adds the block partials of job j
+// per (class, t), multiplies by the class coefficients, extends every class
+// from t = 0..m to t = 0..d and writes the canonical message.  Extension by
+// the trailing difference diagonal e_j = nabla^j f(m): nabla^m f is constant,
+// and f(t + 1) follows from e_j += e_(j+1) for j = m - 1 .. 0.
+__global__ void __launch_bounds__(kThreads) k_finish_classes(const JobPack pack) {
+  const RoundJob& job = pack.job[blockIdx.x];
+  const uint32_t width = job.class_width, groups = kThreads / width;
+  const uint32_t i = threadIdx.x % width, g = threadIdx.x / width;
+  Fr s = Fr::zero();
+  if (g < groups) {
+    // eight independent loads in flight per step, added as a tree
+    constexpr uint32_t kU = 8;
+    for (uint32_t blk = g; blk < job.blocks; blk += kU * groups) {
+      Fr v[kU];
+#pragma unroll
+      for (uint32_t u = 0; u < kU; ++u) {
+        const uint32_t b = blk + u * groups;
+        v[u] = b < job.blocks ? load_fr(job.partial, (size_t)b * width + i) : Fr::zero();
+      }
+#pragma unroll
+      for (uint32_t h = kU / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (uint32_t u = 0; u < h; ++u) v[u] = v[u] + v[u + h];
+      s = s + v[0];
+    }
+  }
+  __shared__ Fr red[kThreads];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t hop = 1; hop < groups; hop <<= 1) {
+    if (g < groups && (g & (2 * hop - 1)) == 0 && g + hop < groups)
+      red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + hop * width];
+    __syncthreads();
+  }
+  constexpr uint32_t kW = 9;  // degree <= 8
+  __shared__ Fr val[kMaxClasses][kW], dif[kMaxClasses][kW], diag[kMaxClasses][kW];
+  const uint32_t deg = job.degree;
+  if (threadIdx.x < job.class_count) {
+    const uint32_t c = threadIdx.x;
+    const uint32_t m = job.classes[4 * c], off = job.classes[4 * c + 1];
+    const Fr k = load_fr(job.class_k, c);
+    for (uint32_t j = 0; j <= m; ++j) {
+      val[c][j] = red[off + j] * k;
+      dif[c][j] = val[c][j];
+    }
+    diag[c][0] = val[c][m];
+    for (uint32_t j = 1; j <= m; ++j) {
+      for (uint32_t q = 0; q + j <= m; ++q) dif[c][q] = dif[c][q + 1] - dif[c][q];
+      diag[c][j] = dif[c][m - j];
+    }
+    for (uint32_t t = m + 1; t <= deg; ++t) {
+      for (uint32_t j = m; j-- > 0;) diag[c][j] = diag[c][j] + diag[c][j + 1];
+      val[c][t] = diag[c][0];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x <= deg) {
+    Fr sum = Fr::zero();
+    for (uint32_t c = 0; c < job.class_count; ++c) sum = sum + val[c][threadIdx.x];
+    store_fr(job.out, threadIdx.x, from_mont(sum));
+    __threadfence_system();  // `out` may be pinned host memory read after the sync
+  }
 }
 
 // Thread per (product p, pair b), product-major, one item per thread: the
@@ -707,6 +953,10 @@ struct State {
   const uint8_t* owner = nullptr;
   const Fr** table_ptrs[3] = {nullptr, nullptr, nullptr};  // inputs, slot A, slot B
   size_t distinct = 0, product_count = 0, index_count = 0;
+  // product classes of the class-major decomposition (class_count 0: not used)
+  const uint32_t* classes = nullptr;
+  const Fr* class_k = nullptr;
+  uint32_t class_count = 0, class_width = 0;
   rt::BufferRef partial, result, ticket;
   Fr* slot_base[2] = {nullptr, nullptr};
   // persistent round loop: this state's mailbox (pinned host memory, from the
@@ -729,6 +979,9 @@ using namespace hp;
 namespace {
 
 constexpr unsigned kMaxJobDegree = 8;
+// class-major launches up to this degree (above it the class passes' terms
+// and sums exceed the register file)
+constexpr unsigned kMaxClassDegree = 6;
 
 unsigned message_blocks(size_t pairs) {
   size_t b = (pairs + sc::kThreads - 1) / sc::kThreads;
@@ -756,10 +1009,35 @@ size_t items_below() {
 
 // grid blocks and decomposition (0 pairs, 1 items, 2 points) of round k of a
 // state (same rule at create time and at run time)
+// The class-major decomposition of large tables (mode 3) for states whose
+// products form few enough classes; HP_MI355X_SC_CLASSES=0 keeps the
+// per-product one (mode 0).
+bool classes_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("HP_MI355X_SC_CLASSES");
+    return e == nullptr || std::atoi(e) != 0;
+  }();
+  return on;
+}
+
+// Pairs per thread of a class launch, HP_MI355X_SC_CLASS_PAIRS (default 1:
+// at n = 20, 2 and 4 pairs per thread measured 4 % and 20 % slower, the
+// smaller grids leaving SIMDs idle, against the amortised class reductions).
+size_t class_pairs_per_thread() {
+  static const size_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_CLASS_PAIRS");
+    const size_t x = e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(1);
+    return std::max<size_t>(1, x);
+  }();
+  return v;
+}
+
 unsigned job_blocks(const sc::State& s, size_t k, uint32_t* mode) {
   const size_t pairs = size_t(1) << (s.num_vars - k - 1);
   if (s.product_count != 0 && pairs >= items_below()) {
-    *mode = 0;
+    *mode = s.class_count != 0 && classes_enabled() ? 3 : 0;
+    if (*mode == 3)
+      return message_blocks((pairs + class_pairs_per_thread() - 1) / class_pairs_per_thread());
     return message_blocks(pairs);
   }
   const size_t items = pairs * s.product_count;
@@ -793,7 +1071,9 @@ void round_tables(const sc::State& s, size_t k, const Fr* const** src, Fr* const
   *dst = k == 0 ? nullptr : const_cast<Fr* const*>(s.table_ptrs[1 + ((k - 1) & 1)]);
 }
 
-sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
+// graph: the job runs in a replayed round graph, whose launches finish jobs
+// with tickets (no class-major decomposition there)
+sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out, bool graph = false) {
   const size_t k = s.round;
   sc::RoundJob j{};
   round_tables(s, k, &j.src, &j.dst);
@@ -813,7 +1093,12 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
   j.fold = (k > 0 && s.product_count > 0) ? 1 : 0;
   uint32_t mode = 0;
   j.blocks = job_blocks(s, k, &mode);
+  if (mode == 3 && graph) mode = 0;
   j.items_mode = mode;
+  j.classes = s.classes;
+  j.class_k = s.class_k;
+  j.class_count = s.class_count;
+  j.class_width = s.class_width;
   return j;
 }
 
@@ -854,7 +1139,10 @@ std::vector<PackLaunch> plan_packs(const std::vector<sc::RoundJob>& jobs) {
 
 template <int D>
 const void* kernel_for(uint32_t mode) {
-  return mode == 2   ? reinterpret_cast<const void*>(&sc::k_round_points<D>)
+  if constexpr (D <= kMaxClassDegree) {
+    if (mode == 3) return reinterpret_cast<const void*>(&sc::k_round_classes<D>);
+  }
+  return mode == 2 ? reinterpret_cast<const void*>(&sc::k_round_points<D>)
          : mode == 1 ? reinterpret_cast<const void*>(&sc::k_round_items<D>)
                      : reinterpret_cast<const void*>(&sc::k_round_pairs<D>);
 }
@@ -883,6 +1171,12 @@ void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
     void* args[] = {&l.pack};
     rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.mode), dim3(l.grid), dim3(sc::kThreads), args,
                                   0, c.stream));
+    if (l.mode == 3) {
+      // class launches always finish in a second kernel
+      rt::hip_check(hipLaunchKernel(reinterpret_cast<const void*>(&sc::k_finish_classes),
+                                    dim3(l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
+      continue;
+    }
     bool multi = false;
     for (uint32_t j = 0; j < l.pack.count; ++j) multi = multi || l.pack.job[j].blocks > 1;
     if (multi) {
@@ -1400,11 +1694,61 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     const size_t idx_bytes = std::max<size_t>(index_count, 1) * sizeof(uint32_t);
     const size_t own_bytes = std::max<size_t>(index_count, 1);
     const size_t cb = 2 * coeff_bytes;
-    const size_t meta_bytes = cb + ptr_bytes + off_bytes + idx_bytes + own_bytes;
+    // product classes (equal arity, equal coefficient) in order of first
+    // appearance, for the class-major decomposition of large tables; none
+    // past kMaxClasses classes or kMaxClassWidth values (m + 1 per class)
+    std::vector<uint32_t> cls_m, cls_first;
+    std::vector<std::vector<uint32_t>> cls_members;
+    bool use_classes = product_count > 0 && max_degree <= kMaxClassDegree;
+    uint32_t cls_width = 0;
+    for (size_t p = 0; p < product_count && use_classes; ++p) {
+      const uint32_t m = static_cast<uint32_t>(product_offsets[p + 1] - product_offsets[p]);
+      size_t k = 0;
+      while (k < cls_m.size() &&
+             !(cls_m[k] == m && std::memcmp(coefficients + rt::kFrBytes * cls_first[k],
+                                            coefficients + rt::kFrBytes * p, rt::kFrBytes) == 0))
+        ++k;
+      if (k == cls_m.size()) {
+        if (k == sc::kMaxClasses || cls_width + m + 1 > sc::kMaxClassWidth) {
+          use_classes = false;
+          break;
+        }
+        cls_m.push_back(m);
+        cls_first.push_back(static_cast<uint32_t>(p));
+        cls_members.emplace_back();
+        cls_width += m + 1;
+      }
+      cls_members[k].push_back(static_cast<uint32_t>(p));
+    }
+    const size_t C = use_classes ? cls_m.size() : 0;
+    const size_t base_bytes = cb + ptr_bytes + off_bytes + idx_bytes + own_bytes;
+    const size_t cls_base = (base_bytes + 31) / 32 * 32;  // class coefficients | table | members
+    const size_t meta_bytes =
+        C ? cls_base + C * rt::kFrBytes + (4 * C + product_count) * sizeof(uint32_t) : base_bytes;
     s->meta = c.allocate(meta_bytes);
     std::vector<uint8_t> host(meta_bytes, 0);
     Fr k5;
     std::memcpy(k5.v, f29::FrConsts::K5, sizeof k5.v);
+    if (C) {
+      uint32_t* tab = reinterpret_cast<uint32_t*>(host.data() + cls_base + C * rt::kFrBytes);
+      uint32_t* mem = tab + 4 * C;
+      uint32_t off = 0, at = 0;
+      for (size_t k = 0; k < C; ++k) {
+        Fr v;
+        std::memcpy(v.v, coefficients + rt::kFrBytes * cls_first[k], rt::kFrBytes);
+        v = to_mont(v);
+        // times 2^(5 (m - 1)): a product of m stored values over m - 1 Fr29
+        // products is prod(a) 2^(256 - 5 (m - 1))
+        for (uint32_t e = 1; e < cls_m[k]; ++e) v = v * k5;
+        std::memcpy(host.data() + cls_base + rt::kFrBytes * k, v.v, rt::kFrBytes);
+        tab[4 * k] = cls_m[k];
+        tab[4 * k + 1] = off;
+        tab[4 * k + 2] = at;
+        for (uint32_t p : cls_members[k]) mem[at++] = p;
+        tab[4 * k + 3] = at;
+        off += cls_m[k] + 1;
+      }
+    }
     for (size_t p = 0; p < product_count; ++p) {
       Fr v;
       std::memcpy(v.v, coefficients + rt::kFrBytes * p, rt::kFrBytes);
@@ -1450,13 +1794,19 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     s->table_ptrs[1] = dptrs + F;
     s->table_ptrs[2] = dptrs + 2 * F;
     s->owner = dev + cb + ptr_bytes + off_bytes + idx_bytes;
+    if (C) {
+      s->class_k = reinterpret_cast<const Fr*>(dev + cls_base);
+      s->classes = reinterpret_cast<const uint32_t*>(dev + cls_base + C * rt::kFrBytes);
+      s->class_count = static_cast<uint32_t>(C);
+      s->class_width = cls_width;
+    }
     const size_t width = max_degree + 1;
     unsigned max_blocks = message_blocks(size_t(1) << (num_vars - 1));
     for (size_t k = 0; k < num_vars; ++k) {
       uint32_t mode = 0;
       max_blocks = std::max(max_blocks, job_blocks(*s, k, &mode));
     }
-    s->partial = c.allocate(size_t(max_blocks) * width * rt::kFrBytes);
+    s->partial = c.allocate(size_t(max_blocks) * std::max<size_t>(width, s->class_width) * rt::kFrBytes);
     s->result = c.allocate(width * rt::kFrBytes);
     s->ticket = c.allocate(sizeof(unsigned));
     hipLaunchKernelGGL(sc::k_zero_fr, dim3(1), dim3(64), 0, c.stream,
@@ -1485,7 +1835,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
       if (tail_ok(s, false) && tail_rounds(c, {&s}, {ch}, {out_evaluations})) return rt::ok();
       quiesce_tails(c);
       if (s.max_degree <= kMaxJobDegree && round_graphs_enabled()) {
-        std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr)};
+        std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr, true)};
         std::vector<uint8_t> out(width * rt::kFrBytes);
         run_jobs_graph(c, jobs, {0}, out.size(), out.data());
         std::memcpy(out_evaluations, out.data(), out.size());
@@ -1564,7 +1914,7 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
         for (size_t i = 0; i < count; ++i) {
           sc::State& s = *ss[i];
           const Fr ch = challenge_of(s, challenges_or_null ? challenges_or_null[i] : nullptr);
-          jobs.push_back(make_job(s, ch, nullptr));
+          jobs.push_back(make_job(s, ch, nullptr, true));
           offs.push_back(offset[i]);
         }
         std::vector<uint8_t> host(total * rt::kFrBytes);

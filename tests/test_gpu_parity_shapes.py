@@ -1,8 +1,10 @@
 """Every decomposition of the sumcheck round kernels and every shape of the
 batched MSM against the oracle, byte for byte.
 
-* k_round_pairs<D> for D = 1..8 (a process with HP_MI355X_SC_ITEMS_BELOW=0, so
-  every round runs on it; tests/sumcheck_pairs_check.py), and the 14-table
+* the pair-per-thread kernels for D = 1..8 (processes with
+  HP_MI355X_SC_ITEMS_BELOW=0, so every round runs on them;
+  tests/sumcheck_pairs_check.py): k_round_classes<D> by default and
+  k_round_pairs<D> with HP_MI355X_SC_CLASSES=0, and the 14-table
   degree-5 jellyfish-style gate of the n = 20 workload at num_vars 16 on the
   default thresholds (round 0 on the pairs kernel, then the item and point
   kernels), each round against oracle.SumCheckState
@@ -29,8 +31,9 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_pairs_kernel_every_degree_matches_oracle():
-    env = dict(os.environ, HP_MI355X_SC_ITEMS_BELOW="0")
+@pytest.mark.parametrize("classes", ["1", "0"])
+def test_pairs_kernel_every_degree_matches_oracle(classes):
+    env = dict(os.environ, HP_MI355X_SC_ITEMS_BELOW="0", HP_MI355X_SC_CLASSES=classes)
     res = subprocess.run([sys.executable, "-u", os.path.join(ROOT, "tests", "sumcheck_pairs_check.py")],
                          capture_output=True, text=True, timeout=300, env=env, cwd=ROOT)
     print(res.stdout[-3000:])
