@@ -126,6 +126,7 @@ struct RoundJob {
   const uint32_t* classes;
   const Fr* class_k;
   uint32_t class_count, class_width;
+  uint32_t entries;  // flattened factor entries (offsets[products])
 };
 
 constexpr int kJobsPerLaunch = 16;
@@ -568,13 +569,137 @@ __global__ void __launch_bounds__(kThreads) k_finish_classes(const JobPack pack)
   }
 }
 
+// ---- latency-bound decompositions (small tables) ----------------------------
+//
+// A small round is a chain of dependent steps per thread; the metadata
+// lookups (offsets -> slots -> table pointer -> table) used to add three
+// dependent global loads per factor.  The block first copies its job's
+// offsets, coefficients and per-entry table pointers into LDS (one round of
+// global loads for all of them), and each thread loads the next factor's table
+// values while it computes on the current one.
+constexpr uint32_t kMetaProducts = 64, kMetaEntries = 256;
+
+struct SmallMeta {
+  uint32_t off[kMetaProducts + 1];
+  const Fr* src[kMetaEntries];
+  Fr* dst[kMetaEntries];
+  uint8_t own[kMetaEntries];
+};
+
+// Copies the job's metadata into `m` when it fits (uniform per block; every
+// thread reaches the barrier).  Returns whether it did.
+__device__ __forceinline__ bool load_meta(const RoundJob& job, SmallMeta& m) {
+  const bool fits = job.products <= kMetaProducts && job.entries <= kMetaEntries;
+  if (fits) {
+    for (uint32_t i = threadIdx.x; i <= job.products; i += kThreads) m.off[i] = job.offsets[i];
+    for (uint32_t e = threadIdx.x; e < job.entries; e += kThreads) {
+      const uint32_t f = job.slots[e];
+      m.src[e] = job.src[f];
+      if (job.fold) {
+        m.dst[e] = job.dst[f];
+        m.own[e] = job.owner[e];
+      }
+    }
+  }
+  __syncthreads();
+  return fits;
+}
+
+// The table values one entry contributes on pair b: the four old values of the
+// folding rounds, the two current ones otherwise.
+template <bool FOLD>
+struct EntryVals {
+  Fr v[FOLD ? 4 : 2];
+};
+
+template <bool FOLD>
+__device__ __forceinline__ void load_entry(const Fr* t, size_t b, EntryVals<FOLD>& x) {
+#pragma unroll
+  for (int i = 0; i < (FOLD ? 4 : 2); ++i) x.v[i] = load_fr(t, (FOLD ? 4 : 2) * b + i);
+}
+
+// lo, hi of the entry on pair b (folded with c29 in folding rounds, the fold
+// stored to dst when dst is not null).
+template <bool FOLD>
+__device__ __forceinline__ void entry_line(const EntryVals<FOLD>& x, const Fr29& c29, Fr* dst,
+                                           size_t b, Fr29& lo, Fr29& hi) {
+  if constexpr (FOLD) {
+    lo = fold29(c29, f29::unpack_fr(x.v[0]), f29::unpack_fr(x.v[1]));
+    hi = fold29(c29, f29::unpack_fr(x.v[2]), f29::unpack_fr(x.v[3]));
+    if (dst != nullptr) {
+      store_fr(dst, 2 * b, f29::pack_fr_canonical(lo));
+      store_fr(dst, 2 * b + 1, f29::pack_fr_canonical(hi));
+    }
+  } else {
+    lo = f29::unpack_fr(x.v[0]);
+    hi = f29::unpack_fr(x.v[1]);
+  }
+}
+
+// Metadata accessors: the LDS copy when the job's metadata fits, else global.
+struct MetaView {
+  const RoundJob& job;
+  const SmallMeta& meta;
+  bool cached;
+  __device__ __forceinline__ uint32_t off(uint32_t p) const {
+    return cached ? meta.off[p] : job.offsets[p];
+  }
+  __device__ __forceinline__ const Fr* src(uint32_t e) const {
+    return cached ? meta.src[e] : job.src[job.slots[e]];
+  }
+  __device__ __forceinline__ Fr* dst(uint32_t e) const {
+    return cached ? meta.dst[e] : job.dst[job.slots[e]];
+  }
+  __device__ __forceinline__ bool own(uint32_t e) const {
+    return cached ? meta.own[e] != 0 : job.owner[e] != 0;
+  }
+};
+
+// Product p on pair b at every t (items) or at t only (points, ALL = false),
+// the factors' table values loaded one factor ahead of the arithmetic.
+template <int D, bool FOLD, bool ALL>
+__device__ __forceinline__ void eval_product(const MetaView& mv, uint32_t p, size_t b, uint32_t t,
+                                             bool store, ProductTerms<D>& pt, Fr29& term) {
+  const RoundJob& job = mv.job;
+  const Fr29 c29 = FOLD ? f29::prescale_fr(job.challenge) : f29::zero_t<f29::FrP>();
+  const uint32_t begin = mv.off(p);
+  const uint32_t m = mv.off(p + 1) - begin;
+  EntryVals<FOLD> buf[2];
+  load_entry<FOLD>(mv.src(begin), b, buf[0]);
+  const Fr29 coeff = load29(job.coeffs29, p);
+  if constexpr (!ALL) term = coeff;
+  // fully unrolled: buf is indexed by compile-time constants only
+#pragma clang loop unroll(full)
+  for (int i = 0; i < D; ++i) {
+    if ((uint32_t)i < m) {
+      const uint32_t e = begin + i;
+      if ((uint32_t)i + 1 < m) load_entry<FOLD>(mv.src(e + 1), b, buf[(i + 1) & 1]);
+      Fr* dst = FOLD && store && mv.own(e) ? mv.dst(e) : nullptr;
+      Fr29 lo, hi;
+      entry_line<FOLD>(buf[i & 1], c29, dst, b, lo, hi);
+      if constexpr (ALL) {
+        if (i == 0)
+          pt.first(coeff, lo, hi);
+        else
+          pt.times(lo, hi);
+      } else {
+        // lo + t (hi - lo) < 17 r
+        const Fr29 d = f29::sub1p(hi, lo);
+        Fr29 line = lo;
+        for (uint32_t k = 0; k < t; ++k) line = f29::add(line, d);
+        term = f29::mul(term, line);
+      }
+    }
+  }
+}
+
 // Thread per (product p, pair b), product-major, one item per thread: the
 // factors of p are folded on the fly (the first occurrence of a factor writes
-// its folded pair) and p is evaluated; short independent chains for the small
-// late rounds where latency dominates.
-template <int D>
-__global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
-  const RoundJob& job = pack.job[job_of_block(pack)];
+// its folded pair) and p is evaluated at every t; short independent chains for
+// the mid-size rounds where latency dominates.
+template <int D, bool FOLD>
+__device__ __forceinline__ void items_body(const JobPack& pack, const RoundJob& job,
+                                           const MetaView& mv) {
   const uint32_t lblock = blockIdx.x - job.block_begin;
   const size_t items = (size_t)job.pairs * job.products;
   const size_t it = (size_t)lblock * kThreads + threadIdx.x;
@@ -582,91 +707,188 @@ __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
 #pragma unroll
   for (int t = 0; t <= D; ++t) pt.term[t] = f29::zero_t<f29::FrP>();
   if (it < items) {
-    const Fr29 c29 = f29::prescale_fr(job.challenge);
     const uint32_t p = static_cast<uint32_t>(it / job.pairs);
     const size_t b = it - (size_t)p * job.pairs;
-    const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
-    for (uint32_t e = begin; e < end; ++e) {
-      const uint32_t f = job.slots[e];
-      Fr29 lo, hi;
-      if (job.fold) {
-        const Fr* o = job.src[f];
-        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
-        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
-        lo = fold29(c29, a0, a1);
-        hi = fold29(c29, a2, a3);
-        if (job.owner[e]) {
-          store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(lo));
-          store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(hi));
-        }
-      } else {
-        const Fr* t = job.src[f];
-        lo = load29(t, 2 * b);
-        hi = load29(t, 2 * b + 1);
-      }
-      if (e == begin)
-        pt.first(load29(job.coeffs29, p), lo, hi);
-      else
-        pt.times(lo, hi);
-    }
+    Fr29 unused;
+    eval_product<D, FOLD, true>(mv, p, b, 0, FOLD, pt, unused);
   }
   Fr out[D + 1];
   to_stored<D>(pt.term, out);
   finish_job<D>(job, lblock, out, pack.split != 0);
 }
 
-// Thread per (product p, pair b, point t), t fastest, for the smallest
-// rounds: each thread folds p's factors on the pair (the t = 0 thread of a
-// factor's first occurrence stores the fold) and evaluates p at its one point,
-// a chain of 3 m products for m factors instead of (deg + 1)(m - 1) + 2 + 2 m.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
+  const RoundJob& job = pack.job[job_of_block(pack)];
+  __shared__ SmallMeta meta;
+  const MetaView mv{job, meta, load_meta(job, meta)};
+  if (job.fold)
+    items_body<D, true>(pack, job, mv);
+  else
+    items_body<D, false>(pack, job, mv);
+}
+
+// ---- the smallest rounds: (pair, product, point) items in three phases ------
+//
+// A block takes whole pairs (or, when a pair has more than a block's items,
+// a range of its products) and runs
+//   1. fold: thread per (distinct factor, pair) folds the factor on the pair
+//      (the first block of the pair stores the fold for the next round) and
+//      leaves lo, hi in LDS;
+//   2. evaluate: thread per (pair, product, t) multiplies the coefficient and
+//      the product's lines at t as a tree (depth ceil(log2(m + 1)) for m
+//      factors instead of a chain of 3 m products);
+//   3. reduce: per t, a tree over the block's values in LDS.
+// Threads are t-major within the block (thread = t G + g for G = threads /
+// (d + 1) slots g), so every t's values are contiguous.  Needs the job's
+// metadata in LDS (at most kMetaProducts products and kMetaEntries entries,
+// hence at most kMetaEntries distinct factors).
+struct PointsMeta {
+  uint32_t off[kMetaProducts + 1];
+  uint8_t slot[kMetaEntries];
+  const Fr* src[kMetaEntries];
+  Fr* dst[kMetaEntries];
+};
+
+// Block shape of a points job: G slots per t, pairs per block (0 when one
+// pair's products span blocks) and blocks per pair.
+struct PointsShape {
+  uint32_t slots, pairs_per_block, blocks_per_pair;
+};
+
+__host__ __device__ inline PointsShape points_shape(uint32_t products, uint32_t degree) {
+  PointsShape sh;
+  sh.slots = kThreads / (degree + 1);
+  if (products <= sh.slots) {
+    sh.pairs_per_block = sh.slots / products;
+    sh.blocks_per_pair = 1;
+  } else {
+    sh.pairs_per_block = 0;
+    sh.blocks_per_pair = (products + sh.slots - 1) / sh.slots;
+  }
+  return sh;
+}
+
 template <int D>
 __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
   constexpr uint32_t width = D + 1;
-  const size_t items = (size_t)job.pairs * job.products * width;
-  const size_t it = (size_t)lblock * kThreads + threadIdx.x;
+  const PointsShape sh = points_shape(job.products, D);
+  const uint32_t G = sh.slots;
+  const uint32_t bpp = sh.blocks_per_pair, sub = lblock % bpp;
+  const size_t b0 = sh.pairs_per_block ? (size_t)lblock * sh.pairs_per_block : (size_t)(lblock / bpp);
+  const uint32_t npairs = sh.pairs_per_block
+                              ? static_cast<uint32_t>(min((size_t)sh.pairs_per_block, job.pairs - b0))
+                              : 1u;
+  const uint32_t F = job.factors;
+
+  __shared__ PointsMeta meta;
+  __shared__ Fr29 lo_s[kMetaEntries], hi_s[kMetaEntries];
+  __shared__ Fr29 red[kThreads];
+  for (uint32_t i = threadIdx.x; i <= job.products; i += kThreads) meta.off[i] = job.offsets[i];
+  for (uint32_t e = threadIdx.x; e < job.entries; e += kThreads) meta.slot[e] = (uint8_t)job.slots[e];
+  for (uint32_t f = threadIdx.x; f < F; f += kThreads) {
+    meta.src[f] = job.src[f];
+    if (job.fold) meta.dst[f] = job.dst[f];
+  }
+  __syncthreads();
+
+  // 1. fold (or load) every distinct factor on the block's pairs
+  for (uint32_t u = threadIdx.x; u < F * npairs; u += kThreads) {
+    const uint32_t f = u / npairs, q = u - f * npairs;
+    const size_t b = b0 + q;
+    const Fr* o = meta.src[f];
+    Fr29 lo, hi;
+    if (job.fold) {
+      const Fr a0 = load_fr(o, 4 * b), a1 = load_fr(o, 4 * b + 1);
+      const Fr a2 = load_fr(o, 4 * b + 2), a3 = load_fr(o, 4 * b + 3);
+      const Fr29 c29 = f29::prescale_fr(job.challenge);
+      lo = fold29(c29, f29::unpack_fr(a0), f29::unpack_fr(a1));
+      hi = fold29(c29, f29::unpack_fr(a2), f29::unpack_fr(a3));
+      if (sub == 0) {
+        store_fr(meta.dst[f], 2 * b, f29::pack_fr_canonical(lo));
+        store_fr(meta.dst[f], 2 * b + 1, f29::pack_fr_canonical(hi));
+      }
+    } else {
+      lo = load29(o, 2 * b);
+      hi = load29(o, 2 * b + 1);
+    }
+    lo_s[u] = lo;
+    hi_s[u] = hi;
+  }
+  __syncthreads();
+
+  // 2. evaluate: thread = t G + g
+  const uint32_t t = threadIdx.x / G, g = threadIdx.x - t * G;
+  Fr29 val = f29::zero_t<f29::FrP>();
+  uint32_t q = 0, p = 0;
+  bool active = t < width;
+  if (sh.pairs_per_block) {
+    q = g / job.products;
+    p = g - q * job.products;
+    active = active && q < npairs;
+  } else {
+    p = sub * G + g;
+    active = active && p < job.products;
+  }
+  if (active) {
+    const uint32_t begin = meta.off[p], m = meta.off[p + 1] - begin;
+    Fr29 v[D + 1];
+    v[0] = load29(job.coeffs29, p);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      if ((uint32_t)i < m) {
+        const uint32_t u = (uint32_t)meta.slot[begin + i] * npairs + q;
+        const Fr29 lo = lo_s[u];
+        // lo + t (hi - lo) < 17 r
+        const Fr29 d = f29::sub1p(hi_s[u], lo);
+        Fr29 line = lo;
+        for (uint32_t k = 0; k < t; ++k) line = f29::add(line, d);
+        v[i + 1] = line;
+      }
+    }
+    // tree over the m + 1 leaves (products stay below 5.1 r)
+#pragma unroll
+    for (int h = 1; h <= D; h <<= 1)
+#pragma unroll
+      for (int i = 0; i + h <= D; i += 2 * h)
+        if ((uint32_t)(i + h) <= m) v[i] = f29::mul(v[i], v[i + h]);
+    val = f29::reduce_est(v[0]);  // < 3 r
+  }
+  red[threadIdx.x] = val;
+  __syncthreads();
+  // 3. per t, a tree over the G slots: each level adds two values below 3 r
+  // and reduces again
+  for (uint32_t h = 1; h < G; h <<= 1) {
+    if (t < width && (g & (2 * h - 1)) == 0 && g + h < G)
+      red[threadIdx.x] = f29::reduce_est(f29::add(red[threadIdx.x], red[threadIdx.x + h]));
+    __syncthreads();
+  }
+  // red[k G] is the block's value at t = k
+  if (job.blocks == 1 || pack.split) {
+    if (threadIdx.x < width) {
+      const Fr v = f29::pack_fr_reduced(red[threadIdx.x * G]);
+      if (job.blocks == 1) {
+        store_fr(job.out, threadIdx.x, from_mont(v));
+        __threadfence_system();  // `out` may be pinned host memory read after the sync
+      } else {
+        store_fr(job.partial, (size_t)lblock * width + threadIdx.x, v);
+      }
+    }
+    return;
+  }
+  // ticket finish (replayed round graphs): thread k carries the value at t = k
   Fr out[D + 1];
 #pragma unroll
-  for (int t = 0; t <= D; ++t) out[t] = Fr::zero();
-  if (it < items) {
-    const uint32_t t = static_cast<uint32_t>(it % width);
-    const size_t pb = it / width;
-    const uint32_t p = static_cast<uint32_t>(pb / job.pairs);
-    const size_t b = pb - (size_t)p * job.pairs;
-    const Fr29 c29 = f29::prescale_fr(job.challenge);
-    const uint32_t begin = job.offsets[p], end = job.offsets[p + 1];
-    Fr29 term = load29(job.coeffs29, p);
-    for (uint32_t e = begin; e < end; ++e) {
-      const uint32_t f = job.slots[e];
-      Fr29 lo, hi;
-      if (job.fold) {
-        const Fr* o = job.src[f];
-        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
-        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
-        lo = fold29(c29, a0, a1);
-        hi = fold29(c29, a2, a3);
-        if (t == 0 && job.owner[e]) {
-          store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(lo));
-          store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(hi));
-        }
-      } else {
-        const Fr* tb = job.src[f];
-        lo = load29(tb, 2 * b);
-        hi = load29(tb, 2 * b + 1);
-      }
-      // lo + t (hi - lo) < 17 r
-      const Fr29 d = f29::sub1p(hi, lo);
-      Fr29 line = lo;
-      for (uint32_t k = 0; k < t; ++k) line = f29::add(line, d);
-      term = f29::mul(term, line);
-    }
-    const Fr v = f29::pack_fr(term);
+  for (int k = 0; k <= D; ++k) out[k] = Fr::zero();
+  if (threadIdx.x < width) {
+    const Fr v = f29::pack_fr_reduced(red[threadIdx.x * G]);
 #pragma unroll
     for (int k = 0; k <= D; ++k)
-      if ((uint32_t)k == t) out[k] = v;
+      if ((uint32_t)k == threadIdx.x) out[k] = v;
   }
-  finish_job<D>(job, lblock, out, pack.split != 0);
+  finish_job<D>(job, lblock, out, false);
 }
 
 // ---- persistent round loops for the last (small) rounds -------------------
@@ -1041,10 +1263,23 @@ unsigned job_blocks(const sc::State& s, size_t k, uint32_t* mode) {
     return message_blocks(pairs);
   }
   const size_t items = pairs * s.product_count;
-  const size_t width = s.max_degree + 1;
-  *mode = s.product_count != 0 && items <= points_below() && s.max_degree <= kMaxJobDegree ? 2 : 1;
-  const size_t threads = *mode == 2 ? items * width : items;
-  return static_cast<unsigned>(std::max<size_t>(1, (threads + sc::kThreads - 1) / sc::kThreads));
+  *mode = s.product_count != 0 && items <= points_below() && s.max_degree <= kMaxJobDegree &&
+                  s.product_count <= sc::kMetaProducts && s.index_count <= sc::kMetaEntries
+              ? 2
+              : 1;
+  if (*mode == 2) {
+    const sc::PointsShape sh = sc::points_shape(static_cast<uint32_t>(s.product_count),
+                                                static_cast<uint32_t>(s.max_degree));
+    // the fold phase keeps (distinct factor, pair) lines of the block in LDS
+    if (s.distinct * std::max<uint32_t>(1, sh.pairs_per_block) > sc::kMetaEntries) {
+      *mode = 1;
+      return static_cast<unsigned>(std::max<size_t>(1, (items + sc::kThreads - 1) / sc::kThreads));
+    }
+    const size_t blocks = sh.pairs_per_block ? (pairs + sh.pairs_per_block - 1) / sh.pairs_per_block
+                                             : pairs * sh.blocks_per_pair;
+    return static_cast<unsigned>(blocks);
+  }
+  return static_cast<unsigned>(std::max<size_t>(1, (items + sc::kThreads - 1) / sc::kThreads));
 }
 
 // Generic path (degree > kMaxJobDegree): message for t in chunks of 8 values.
@@ -1099,6 +1334,7 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out, bool graph = fa
   j.class_k = s.class_k;
   j.class_count = s.class_count;
   j.class_width = s.class_width;
+  j.entries = static_cast<uint32_t>(s.index_count);
   return j;
 }
 

@@ -163,6 +163,52 @@ def test_degree2_and_degree5_match_oracle(backend, oracle, rng_factory, nv):
     run_vp(backend, oracle, nv, 5, tabs, jellyfish_products(pf.SplitMix64(5), list(range(14))), 12)
 
 
+@pytest.mark.parametrize("degree,nprod,nmle", [(5, 50, 20), (5, 64, 12), (2, 64, 30), (8, 40, 9),
+                                               (1, 3, 3)])
+def test_many_products_match_oracle(backend, oracle, rng_factory, degree, nprod, nmle):
+    """Block shapes of the smallest-round kernel: 50 and 64 degree-5 products
+    span two blocks per pair (42 slots per point), 64 degree-2 products leave
+    one pair per block, degree 8 leaves 28 slots, degree 1 with 3 products
+    packs 42 pairs per block.  Random arities 1..degree, repeated factors,
+    shared and negated coefficients."""
+    nv = 7
+    rng = rng_factory(500 + degree * 100 + nprod)
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(nmle)]
+    crng = pf.SplitMix64(900 + nprod)
+    prods = []
+    for i in range(nprod):
+        m = degree if i == 0 else 1 + crng.next() % degree
+        ids = [crng.next() % nmle for _ in range(m)]
+        c = (1, R - 1, crng.nonboolean())[crng.next() % 3]
+        prods.append((c, ids))
+    run_vp(backend, oracle, nv, degree, tabs, prods, 17)
+
+
+def test_unreferenced_factors_match_oracle(backend, oracle, rng_factory):
+    """40 distinct factor handles of which two products use three: the
+    smallest-round kernel would need 40 x 85 (factor, pair) lines per block in
+    LDS, so those rounds take the per-(product, pair) kernel; messages equal
+    the oracle's over the same tables."""
+    nv = 8
+    rng = rng_factory(777)
+    tabs = [rng.fr_bytes(1 << nv) for _ in range(40)]
+    mles = [backend.mle_from_evaluations(nv, t) for t in tabs]
+    products = [(3, [5, 17]), (R - 1, [39])]
+    st, state = raw_state(backend, nv, 2, [m.address for m in mles], products)
+    assert st.kind == 0
+    ost = oracle.SumCheckState(nv, 2, [fr_from_bytes(t.tobytes()) for t in tabs], products)
+    crng = pf.SplitMix64(778)
+    c = None
+    try:
+        for _ in range(nv):
+            st, raw = round_raw(backend, state, c, 3)
+            assert st.kind == 0
+            assert fr_from_bytes(raw) == ost.round(c)
+            c = crng.nonboolean()
+    finally:
+        backend.lib.hp_mi355x_v1_sumcheck_prover_state_destroy(backend.context, state)
+
+
 @pytest.mark.parametrize("degree", [7, 9, 34])
 def test_high_degree_products(backend, oracle, rng_factory, degree):
     # degrees beyond the register-resident kernel (cf. sum_check/mod.rs:663-914 degrees {2,3,7,34})
