@@ -70,6 +70,9 @@ BufferRef upload_fr_table(Context& c, const uint8_t* host, size_t count, size_t 
 void upload_fr_values(Context& c, Fr* dst, const uint8_t* host, size_t count);
 // Copy `count` Montgomery Fr values back to host as canonical bytes (synchronises).
 void download_fr(Context& c, const Fr* src, size_t count, uint8_t* host);
+// Enqueues the canonical form of src[0..count) into dst (device or pinned
+// host memory) on the context stream; no synchronisation.
+void enqueue_from_mont(Context& c, const Fr* src, size_t count, Fr* dst);
 // Copy raw device bytes to host (synchronises).
 void download_bytes(Context& c, const void* src, size_t bytes, uint8_t* host);
 

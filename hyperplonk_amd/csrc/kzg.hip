@@ -91,14 +91,24 @@ kzg::Params* bind(HpMi355xContext* ctx, rt::Context& c, const HpMi355xKzgParams*
   return params->impl.get();
 }
 
-// Run a batch of MSMs and write one 97-byte packet per MSM.
-void run_batch(rt::Context& c, const msm::Batch& batch, uint8_t* out97) {
+static_assert(sizeof(G1Xyzz) % 32 == 0, "Fr values after the result slots stay 32-byte aligned");
+
+// Run a batch of MSMs and write one 97-byte packet per MSM; `extra` Fr
+// values on the device come back canonical into extra_out under the same
+// synchronisation.
+void run_batch(rt::Context& c, const msm::Batch& batch, uint8_t* out97, const Fr* extra = nullptr,
+               size_t extra_count = 0, uint8_t* extra_out = nullptr) {
   const size_t slots = batch.result_slots();
   // the level-1 reduction writes its points straight into pinned host memory
-  G1Xyzz* host = reinterpret_cast<G1Xyzz*>(c.results(std::max<size_t>(slots, 1) * sizeof(G1Xyzz)));
+  // (G1Xyzz is a multiple of 32 bytes: the extra values stay aligned)
+  const size_t slot_bytes = std::max<size_t>(slots, 1) * sizeof(G1Xyzz);
+  uint8_t* base = c.results(slot_bytes + extra_count * rt::kFrBytes);
+  G1Xyzz* host = reinterpret_cast<G1Xyzz*>(base);
+  if (extra_count) rt::enqueue_from_mont(c, extra, extra_count, reinterpret_cast<Fr*>(base + slot_bytes));
   batch.enqueue(c, host);
   c.sync();
   batch.finish_packets(host, out97);
+  if (extra_count) std::memcpy(extra_out, base + slot_bytes, extra_count * rt::kFrBytes);
 }
 
 }  // namespace
@@ -360,8 +370,10 @@ extern "C" HpMi355xStatus hp_mi355x_v1_multilinear_kzg_open(
     std::vector<rt::BufferRef> keep;
     const Fr* value = nullptr;
     enqueue_open(c, *p, static_cast<const Fr*>(f->ptr), num_vars, point, batch, keep, &value);
-    if (batch.size()) run_batch(c, batch, out_quotients);
-    rt::download_fr(c, value, 1, out_evaluation);
+    if (batch.size())
+      run_batch(c, batch, out_quotients, value, 1, out_evaluation);
+    else
+      rt::download_fr(c, value, 1, out_evaluation);
     return rt::ok();
   });
 }

@@ -148,7 +148,17 @@ BufferRef Context::lookup(void* handle, size_t nv) const {
 }
 
 uint8_t* Context::staging(size_t bytes) {
-  if (pinned_free) hip_check(hipEventSynchronize(pinned_free));
+  if (pinned_free) {
+    // the previous copy out of the staging buffer has usually completed (its
+    // call synchronised): wait only when it has not
+    const hipError_t q = hipEventQuery(pinned_free);
+    if (q == hipErrorNotReady) {
+      (void)hipGetLastError();
+      hip_check(hipEventSynchronize(pinned_free));
+    } else {
+      hip_check(q);
+    }
+  }
   if (bytes > pinned_bytes) {
     if (pinned) hip_check(hipHostFree(pinned));
     pinned = nullptr;
@@ -244,6 +254,12 @@ void download_bytes(Context& c, const void* src, size_t bytes, uint8_t* host) {
     std::memcpy(host + done, stage, n);
     done += n;
   }
+}
+
+void enqueue_from_mont(Context& c, const Fr* src, size_t count, Fr* dst) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_fr_from_mont, dim3(grid_for(count)), dim3(256), 0, c.stream, src, dst, count);
+  check_launch();
 }
 
 void download_fr(Context& c, const Fr* src, size_t count, uint8_t* host) {

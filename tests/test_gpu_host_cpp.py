@@ -36,3 +36,18 @@ def test_loaded_library_matches_the_sources_on_this_machine():
     from hyperplonk_amd import abi
 
     assert abi.build_info() == f"sources-sha256={abi.source_digest()} arch=gfx950"
+
+
+def test_runtime_audit_of_online_calls():
+    """Every HIP allocation, copy, memset, synchronisation and launch the
+    archive makes is counted per entry-point call (--wrap shims,
+    tests/cpp/runtime_audit_test.c, as the reference audits its bridge:
+    backend/src/ascend910b3/native/tests/runtime_audit.h:10-38).  After
+    warm-up, evaluate, commit, open and every sumcheck round allocate nothing,
+    keep their copies within byte budgets and synchronise at most once;
+    buffer_retire neither copies, synchronises nor launches."""
+    prog = os.path.join(ROOT, "_build", "runtime_audit_test")
+    assert os.path.exists(prog), "build the test programs first (make -C tests/cpp)"
+    res = subprocess.run([prog], capture_output=True, text=True, timeout=180)
+    print(res.stdout)
+    assert res.returncode == 0 and "RUNTIME_AUDIT PASS" in res.stdout, (res.returncode, res.stdout, res.stderr)
