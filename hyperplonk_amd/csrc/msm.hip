@@ -101,9 +101,25 @@ __device__ __forceinline__ uint32_t window_digit(const uint32_t* s, uint32_t w, 
 //   2. one block per partition counts its records per bucket in LDS and writes
 //      the base references in bucket order inside the partition's span, with
 //      the bucket starts, counts and owners.
+// A prepared base: packed Fq29 x, y (96 bytes) padded to 128 so that every
+// base is one aligned 128-byte line; a random base read then fetches one line
+// instead of straddling two.
+struct alignas(128) PreparedBase {
+  G1Affine p;
+  uint32_t pad[8];
+};
+static_assert(sizeof(PreparedBase) == 128, "one 128-byte line per base");
+
+__device__ __forceinline__ G1Affine load_base(const PreparedBase* b, size_t i) {
+  return load_affine(&b[i].p, 0);
+}
+__device__ __forceinline__ void store_base(PreparedBase* b, size_t i, const G1Affine& a) {
+  store_affine(&b[i].p, 0, a);
+}
+
 struct Job {
   const Fr* scalars;
-  const G1Affine* bases;  // W window-shifted copies of the layer, window-major
+  const PreparedBase* bases;  // W window-shifted copies of the layer, window-major
   uint32_t n, c, windows, buckets;
   uint32_t bucket_base, chunks, cc_base, fold_base;  // fold_base: first folded scalar
   // bucket sums: 2^lanes_log2 lanes per bucket, threads [sum_base, sum_base +
@@ -446,11 +462,11 @@ __global__ void __launch_bounds__(256) k_accumulate(const Task* tasks, const uin
   const uint32_t task = blockIdx.x * blockDim.x + threadIdx.x;
   if (task >= *ntasks) return;
   const Task t = tasks[task];
-  const G1Affine* bases = jobs[t.job].bases;
+  const PreparedBase* bases = jobs[t.job].bases;
   Xyzz29 acc = Xyzz29::identity();
   for (uint32_t e = 0; e < t.len; ++e) {
     const uint32_t v = sorted[t.start + e];
-    const G1Affine raw = load_affine(bases, v & 0x7fffffffu);
+    const G1Affine raw = load_base(bases, v & 0x7fffffffu);
     if (raw.is_identity()) continue;
     f29::Aff29 p{f29::unpack(raw.x.v), f29::unpack(raw.y.v)};
     if (v >> 31) p.y = f29::neg_canonical(p.y);
@@ -731,22 +747,24 @@ __global__ void __launch_bounds__(G) k_reduce_buckets_two(const Xyzz29* sums, ui
   }
 }
 
-// Prepared bases: 32-bit-limb Montgomery -> packed Fq29 (in place) and back.
-__global__ void k_bases_to29(G1Affine* pts, size_t n) {
+// Prepared bases: 32-bit-limb Montgomery points -> padded packed Fq29 bases,
+// and back.
+__global__ void k_bases_to29(const G1Affine* pts, PreparedBase* out, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     G1Affine p = load_affine(pts, i);
-    if (p.is_identity()) continue;
-    p.x = f29::to29_packed(p.x);
-    p.y = f29::to29_packed(p.y);
-    store_affine(pts, i, p);
+    if (!p.is_identity()) {
+      p.x = f29::to29_packed(p.x);
+      p.y = f29::to29_packed(p.y);
+    }
+    store_base(out, i, p);
   }
 }
 
-__global__ void k_bases_from29(const G1Affine* pts, G1Affine* out, size_t n) {
+__global__ void k_bases_from29(const PreparedBase* pts, G1Affine* out, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    G1Affine p = load_affine(pts, i);
+    G1Affine p = load_base(pts, i);
     if (!p.is_identity()) {
       p.x = f29::from29_packed(p.x);
       p.y = f29::from29_packed(p.y);
@@ -759,14 +777,14 @@ __global__ void k_bases_from29(const G1Affine* pts, G1Affine* out, size_t n) {
 // next[i] = 2^c prev[i] on packed Fq29 coordinates: c XYZZ doublings on
 // 14 x 29-bit limbs, normalised with one inversion per thread (Montgomery
 // batch inversion over the thread's strided points).
-__global__ void k_shift_window29(const G1Affine* prev, G1Affine* next, size_t n, uint32_t c,
+__global__ void k_shift_window29(const PreparedBase* prev, PreparedBase* next, size_t n, uint32_t c,
                                  Xyzz29* scratch, Fq29* prefix) {
   const size_t threads = (size_t)gridDim.x * blockDim.x;
   const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (tid >= n) return;
   Fq29 acc = f29::one();
   for (size_t i = tid; i < n; i += threads) {
-    const G1Affine raw = load_affine(prev, i);
+    const G1Affine raw = load_base(prev, i);
     Xyzz29 q = Xyzz29::identity();
     if (!raw.is_identity()) {
       q = f29::dbl_affine(f29::Aff29{f29::unpack(raw.x.v), f29::unpack(raw.y.v)});
@@ -781,14 +799,14 @@ __global__ void k_shift_window29(const G1Affine* prev, G1Affine* next, size_t n,
   for (size_t i = last;; i -= threads) {
     const Xyzz29 q = load_xyzz29(scratch, i);
     if (q.is_identity()) {
-      store_affine(next, i, G1Affine::identity());
+      store_base(next, i, G1Affine::identity());
     } else {
       const Fq29 d_inv = f29::mul(inv, prefix[i]);  // 1 / (ZZ ZZZ)
       inv = f29::mul(inv, f29::mul(q.ZZ, q.ZZZ));
       G1Affine out;
       f29::pack(f29::canonical(f29::mul(q.X, f29::mul(d_inv, q.ZZZ))), out.x.v);
       f29::pack(f29::canonical(f29::mul(q.Y, f29::mul(d_inv, q.ZZ))), out.y.v);
-      store_affine(next, i, out);
+      store_base(next, i, out);
     }
     if (i < threads) break;
   }
@@ -1077,13 +1095,11 @@ void prepare_layer_from_device(rt::Context& c, Layer& layer, const G1Affine* poi
   layer.n = n;
   layer.plan = plan_for(c, n, override_c);
   const uint32_t W = layer.plan.windows;
-  layer.bases = c.allocate(W * n * sizeof(G1Affine));
-  G1Affine* bases = static_cast<G1Affine*>(layer.bases->ptr);
-  rt::hip_check(hipMemcpyAsync(bases, points, n * sizeof(G1Affine), hipMemcpyDeviceToDevice,
-                               c.stream));
+  layer.bases = c.allocate(W * n * sizeof(PreparedBase));
+  PreparedBase* bases = static_cast<PreparedBase*>(layer.bases->ptr);
   // the bucket stages read packed Fq29 coordinates: window 0 is converted,
   // the shifted copies are built from it in that form
-  hipLaunchKernelGGL(k_bases_to29, dim3(grid_for(n)), dim3(256), 0, c.stream, bases, n);
+  hipLaunchKernelGGL(k_bases_to29, dim3(grid_for(n)), dim3(256), 0, c.stream, points, bases, n);
   rt::check_launch();
   if (W > 1) {
     const size_t threads = std::max<size_t>(1, std::min<size_t>(n, 65536));
@@ -1104,7 +1120,7 @@ rt::BufferRef layer_points(rt::Context& c, const Layer& layer) {
   const size_t n = layer.n;
   rt::BufferRef out = c.allocate(std::max<size_t>(n, 1) * sizeof(G1Affine));
   hipLaunchKernelGGL(k_bases_from29, dim3(grid_for(n)), dim3(256), 0, c.stream,
-                     static_cast<const G1Affine*>(layer.bases->ptr),
+                     static_cast<const PreparedBase*>(layer.bases->ptr),
                      static_cast<G1Affine*>(out->ptr), n);
   rt::check_launch();
   return out;
@@ -1202,7 +1218,7 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev,
   for (uint32_t j = 0; j < njobs; ++j) {
     const Entry& e = entries_[j];
     const Plan& p = e.layer->plan;
-    jobs[j] = {e.scalars, static_cast<const G1Affine*>(e.layer->bases->ptr), (uint32_t)e.layer->n,
+    jobs[j] = {e.scalars, static_cast<const PreparedBase*>(e.layer->bases->ptr), (uint32_t)e.layer->n,
                p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base,
                (uint32_t)fold_total, 0u, 0u, 0u, 0u};
     fold_total += e.layer->n;

@@ -30,7 +30,8 @@ void autotune(rt::Context& c, size_t log_n, uint32_t* best_c, uint32_t* best_cap
 struct Layer {
   size_t n = 0;               // points per window copy
   Plan plan;
-  rt::BufferRef bases;        // W * n affine points, window-major, packed Fq29 coordinates
+  rt::BufferRef bases;        // W * n affine points, window-major, packed Fq29 coordinates,
+                              // 128 bytes per point (96 used)
 };
 
 // Validate 97-byte packets on the host (tags, canonical coordinates) and
