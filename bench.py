@@ -51,11 +51,15 @@ LIMB_PRODUCTS_PER_FQM = 300
 # convention credits the kernel with 10 % more work than it does).
 FQM_PER_MIXED_ADD = 11
 # HBM bytes of one 2^20 k_accumulate launch by (window bits, task cap), from
-# the FETCH_SIZE and WRITE_SIZE passes calibrated on the in-place conversion
-# of 2^20 points (96 MiB read and written; profiles/k_accumulate_msm_2_20_pmc.md);
-# rocprofv3 counters cannot be read from inside this process.  The autotuner
-# picks c = 17 with cap 16 or 32 (within 1 % of each other) on these boxes.
-ACCUMULATE_TRAFFIC_2_20 = {(17, 32): 3.878e9, (17, 16): 3.622e9}
+# the FETCH_SIZE and WRITE_SIZE passes calibrated on bench.py's own 1 GiB copy
+# kernel (profiles/k_accumulate_msm_2_20_pmc.md); rocprofv3 counters cannot be
+# read from inside this process.  The autotuner picks c = 17, cap 32 here.
+ACCUMULATE_TRAFFIC_2_20 = {(17, 32): 2.560e9}
+# VALU lane instructions k_accumulate issues per non-zero digit: SQ_INSTS_VALU
+# x 64 / digits from the SQ pass of the same profile (1 276 594 302 x 64 /
+# 15 728 519); with the measured multiply-add peak it gives the fraction of
+# the VALU issue rate the kernel runs at.
+ACCUMULATE_VALU_INSTS_PER_DIGIT = 5194
 # task cap of the built-in plan at 2^20
 DEFAULT_CAP_2_20 = 32
 
@@ -326,7 +330,12 @@ def main():
             "peak_measured_kernel": "k_mad_u64: 16 independent v_mad_u64_u32 chains per lane on every CU",
             "hbm_copy_bytes_per_s_measured": round(peaks["hbm_bytes_per_s"], 1),
             "traffic": (ACCUMULATE_TRAFFIC_2_20.get((plan["c"], cap_used)) if log_n == 20 else None),
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/k_accumulate_msm_2_20_pmc.md",
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at (c, cap) = (17, 32), "
+            "profiles/k_accumulate_msm_2_20_pmc.md",
+            "valu_issue_frac": round(plan["entries"] * ACCUMULATE_VALU_INSTS_PER_DIGIT
+                                     / (acc_ms * 1e-3) / peaks["mad_per_s"], 4),
+            "valu_issue_note": f"{ACCUMULATE_VALU_INSTS_PER_DIGIT} VALU lane instructions per digit "
+            "(SQ_INSTS_VALU pass) over the measured multiply-add issue rate",
             "kernel_ms": round(acc_ms, 4),
             "stage_ms": {k: round(v, 4) for k, v in stages.items()},
             "whole_msm_conventional_T_per_s": round(whole_msm_rate, 3),

@@ -5,7 +5,10 @@
  * backend/src/ascend910b3/ffi.rs:34-173): the same 20 entry points with the
  * same argument meaning, ownership, validation order and status kinds, so the
  * Rust trait glue of that backend (MleBackend, MultilinearKzgBackend,
- * SumCheckProver) binds to these symbols unchanged apart from the prefix.
+ * SumCheckProver) binds to these symbols unchanged apart from the prefix and
+ * the archive's name: both link a static archive (libhp_mi355x.a here, plus
+ * -lamdhip64 and the C++ runtime; the shared libhp_mi355x.so is built from the
+ * same objects for ctypes and the C++ host programs).
  *
  * Host exchange formats (not device layouts): Fr = canonical 32-byte
  * little-endian integer; G1 = 97 bytes: tag (0 finite, 1 identity with zero
