@@ -143,6 +143,58 @@ HP_HD Xyzz29 add_pair(const Xyzz29& p, const Xyzz29& q, uint32_t half, Ex ex) {
   return {X3, Y3, h ? o6 : m6, h ? m6 : o6};
 }
 
+// Per-lane choice among four values by the lane's index q in its quad.
+HP_HD Fq29 pick4(uint32_t q, const Fq29& a0, const Fq29& a1, const Fq29& a2, const Fq29& a3) {
+  Fq29 r;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const uint32_t lo = (q & 1u) ? a1.l[i] : a0.l[i];
+    const uint32_t hi = (q & 1u) ? a3.l[i] : a2.l[i];
+    r.l[i] = (q & 2u) ? hi : lo;
+  }
+  return r;
+}
+
+// P + Q by the four lanes q = 0..3 of a quad that all hold P and Q, one
+// product per lane and stage; `bc.template get<K>(v)` returns lane K's v
+// (every broadcast is reached by the four lanes together: P and Q, hence
+// every branch, are the same in all of them).  Per lane: four products
+// against add()'s 12 products and 2 squarings, so for latency-bound
+// reductions a dependent addition costs about a third of add()'s chain.
+//   stage 1: U1 = X1 ZZ2 | U2 = X2 ZZ1 | S1 = Y1 ZZZ2 | S2 = Y2 ZZZ1
+//   stage 2: PP = P P    | RR = R R    | ZZ12 = ZZ1 ZZ2 | ZZZ12 = ZZZ1 ZZZ2
+//   stage 3: PPP = P PP  | Q = U1 PP   | ZZ3 = ZZ12 PP  | (PPP again)
+//   stage 4: R (Q + 16p - X3) | ZZZ3 = ZZZ12 PPP | (2p - S1) PPP | (lane 0's)
+// X3, ZZ3, ZZZ3 equal add()'s limb for limb (the squarings as products give
+// the same integers); Y3 is the sum of two reduced products instead of
+// add()'s single reduction of their sum: the same field element, below
+// 2 p (1 + 2^-15) < 4 p, so the bounds of every formula hold.
+template <class Bc>
+HP_HD Xyzz29 add_quad(const Xyzz29& p, const Xyzz29& q, uint32_t lane, Bc& bc) {
+  if (q.is_identity()) return p;
+  if (p.is_identity()) return q;
+  const Fq29 m1 = mul(pick4(lane, p.X, q.X, p.Y, q.Y), pick4(lane, q.ZZ, p.ZZ, q.ZZZ, p.ZZZ));
+  const Fq29 U1 = bc.template get<0>(m1), U2 = bc.template get<1>(m1);
+  const Fq29 S1 = bc.template get<2>(m1), S2 = bc.template get<3>(m1);
+  const Fq29 P = sub2p(U2, U1);
+  const Fq29 R = sub2p(S2, S1);
+  if (is_zero_mod(P)) {
+    if (is_zero_mod(R)) return dbl(p);
+    return Xyzz29::identity();
+  }
+  const Fq29 m2 = mul(pick4(lane, P, R, p.ZZ, p.ZZZ), pick4(lane, P, R, q.ZZ, q.ZZZ));
+  const Fq29 PP = bc.template get<0>(m2), RR = bc.template get<1>(m2);
+  const Fq29 ZZ12 = bc.template get<2>(m2), ZZZ12 = bc.template get<3>(m2);
+  const Fq29 m3 = mul(pick4(lane, P, U1, ZZ12, P), PP);
+  const Fq29 PPP = bc.template get<0>(m3), Qv = bc.template get<1>(m3);
+  const Fq29 ZZ3 = bc.template get<2>(m3);
+  const Fq29 X3 = sub8p(RR, add_twice(PPP, Qv));
+  const Fq29 m4 = mul(pick4(lane, R, ZZZ12, sub2p(zero(), S1), R),
+                      pick4(lane, sub16p(Qv, X3), PPP, PPP, sub16p(Qv, X3)));
+  const Fq29 Y3 = add(bc.template get<0>(m4), bc.template get<2>(m4));
+  return {X3, Y3, ZZ3, bc.template get<1>(m4)};
+}
+
 // a^(p - 2) (zero maps to zero); the one inversion per thread of the batch
 // normalisations
 HP_HD Fq29 inverse(const Fq29& a) {

@@ -25,6 +25,7 @@
 // bridge uses unsigned windows with c = ell - ell/3 + 2
 // (backend/src/ascend910b3/native/msm_plan.cpp:119-123).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <cmath>
 #include <vector>
@@ -491,10 +492,11 @@ __device__ __forceinline__ Xyzz29 shfl_xor_xyzz(const Xyzz29& v, unsigned mask) 
 // job: larger MSMs have more partials per bucket): each lane adds every
 // group-th partial, then a butterfly over the group.  Buckets with more than
 // seq_max partials per lane (outliers, several times the job's mean) are
-// appended to `heavy` instead and summed by the heavy kernels.  kPair: every
-// lane of the group is a pair of threads adding with f29::add_pair (small
-// batches, whose wavefronts leave SIMDs idle; groups of <= 32 pairs, so a
-// group stays inside one wavefront); total_threads counts lanes.
+// appended to `heavy` instead and summed by the heavy kernels.  W > 1: every
+// lane of the group is W threads adding with f29::add_pair (W = 2) or
+// f29::add_quad (W = 4), for small batches whose wavefronts leave SIMDs idle
+// (a group of 64 / W lanes at most, so it stays inside one wavefront);
+// total_threads counts lanes.
 constexpr uint32_t kSeqMax = 8;
 __device__ __forceinline__ Fq29 shfl_pair(const Fq29& v) {
   Fq29 r;
@@ -503,7 +505,35 @@ __device__ __forceinline__ Fq29 shfl_pair(const Fq29& v) {
   return r;
 }
 
-template <bool kPair>
+// Lane K's value to every lane of each quad (DPP quad_perm [K, K, K, K], one
+// VALU move per limb): the exchanges of f29::add_quad.
+struct QuadBroadcast {
+  template <int K>
+  __device__ __forceinline__ Fq29 get(const Fq29& v) const {
+    Fq29 r;
+#pragma unroll
+    for (int i = 0; i < f29::L; ++i)
+      r.l[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.l[i], K * 0x55, 0xf, 0xf, false);
+    return r;
+  }
+};
+
+// One XYZZ addition by W = 1, 2 or 4 threads (h: the thread's index among
+// them; all W hold both operands and get the sum).
+template <int W>
+__device__ __forceinline__ Xyzz29 lane_add(const Xyzz29& a, const Xyzz29& b, uint32_t h) {
+  if constexpr (W == 4) {
+    QuadBroadcast bc;
+    return f29::add_quad(a, b, h, bc);
+  } else if constexpr (W == 2) {
+    auto ex = [](const Fq29& v) { return shfl_pair(v); };
+    return f29::add_pair(a, b, h, ex);
+  } else {
+    return f29::add(a, b);
+  }
+}
+
+template <int W>
 __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
                                                      const uint32_t* ntask,
                                                      const Xyzz29* partials, const Job* jobs,
@@ -511,8 +541,7 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
                                                      Xyzz29* sums, uint32_t* heavy,
                                                      uint32_t* heavy_count) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t gid = kPair ? tid >> 1 : tid, h = kPair ? tid & 1u : 0u;
-  auto ex = [](const Fq29& v) { return shfl_pair(v); };
+  const uint32_t gid = tid / W, h = tid % W;
   // job owning this lane (last with sum_base <= gid); wave-uniform since
   // every job's lane range is a multiple of 64 lanes
   uint32_t lo = 0, hi = njobs;
@@ -536,13 +565,13 @@ __global__ void __launch_bounds__(256) k_bucket_sums(const uint32_t* task_off,
     } else {
       for (uint32_t k = lane; k < cnt; k += group) {
         const Xyzz29 x = load_xyzz29(partials, first + k);
-        s = kPair ? f29::add_pair(s, x, h, ex) : f29::add(s, x);
+        s = lane_add<W>(s, x, h);
       }
     }
   }
   for (uint32_t m = 1; m < group; m <<= 1) {
-    const Xyzz29 o = shfl_xor_xyzz(s, kPair ? 2 * m : m);
-    s = kPair ? f29::add_pair(s, o, h, ex) : f29::add(s, o);
+    const Xyzz29 o = shfl_xor_xyzz(s, W * m);
+    s = lane_add<W>(s, o, h);
   }
   if (live && lane == 0 && h == 0) store_xyzz29(sums, b, s);
 }
@@ -663,22 +692,22 @@ __global__ void __launch_bounds__(G) k_reduce_buckets(const Xyzz29* sums, uint32
   }
 }
 
-// The same level-1 reduction with two lanes per bucket (f29::add_pair): for
-// small batches, whose wavefronts leave most SIMDs idle, each of the
-// 2 log2 G dependent additions takes the per-lane share of its products.
-template <unsigned G>
-__global__ void __launch_bounds__(2 * G) k_reduce_buckets_pair(const Xyzz29* sums,
+// The same level-1 reduction with W = 2 or 4 threads per bucket
+// (f29::add_pair, f29::add_quad): for small batches, whose wavefronts leave
+// most SIMDs idle, each of the 2 log2 G dependent additions takes the
+// per-thread share of its products.
+template <unsigned G, int W>
+__global__ void __launch_bounds__(W * G) k_reduce_buckets_pair(const Xyzz29* sums,
                                                                uint32_t buckets, G1Xyzz* pair_a,
                                                                G1Xyzz* pair_r) {
   __shared__ Xyzz29 u[G];
-  const uint32_t t = threadIdx.x >> 1, h = threadIdx.x & 1u;  // bucket in the group, lane of the pair
+  const uint32_t t = threadIdx.x / W, h = threadIdx.x % W;  // bucket in the group, thread of its lane
   const uint32_t b = blockIdx.x * G + t;
-  auto ex = [](const Fq29& v) { return shfl_pair(v); };
   Xyzz29 v = b < buckets ? load_xyzz29(sums, b) : Xyzz29::identity();
   if (h == 0) u[t] = v;
   __syncthreads();
   for (unsigned d = 1; d < G; d <<= 1) {
-    if (t + d < G) v = f29::add_pair(v, u[t + d], h, ex);  // uniform within a pair
+    if (t + d < G) v = lane_add<W>(v, u[t + d], h);  // uniform within a lane
     __syncthreads();
     if (h == 0) u[t] = v;
     __syncthreads();
@@ -687,7 +716,7 @@ __global__ void __launch_bounds__(2 * G) k_reduce_buckets_pair(const Xyzz29* sum
   // tree sum of U
   for (unsigned d = G / 2; d > 0; d >>= 1) {
     if (t < d) {
-      v = f29::add_pair(v, u[t + d], h, ex);
+      v = lane_add<W>(v, u[t + d], h);
       if (h == 0) u[t] = v;
     }
     __syncthreads();
@@ -698,29 +727,29 @@ __global__ void __launch_bounds__(2 * G) k_reduce_buckets_pair(const Xyzz29* sum
   }
 }
 
-// The same level-1 reduction for large batches, two buckets per lane pair:
-// pair t holds buckets 2t and 2t+1 of the group, R_t = S_2t + S_2t+1 and
-// A_t = S_2t + 2 S_2t+1.  With U_t the suffix scan of the R_t,
+// The same level-1 reduction for large batches, two buckets per lane of W
+// threads: lane t holds buckets 2t and 2t+1 of the group, R_t = S_2t + S_2t+1
+// and A_t = S_2t + 2 S_2t+1.  With U_t the suffix scan of the R_t,
 // sum_j (j + 1) S_j = sum_t (A_t + 2 U_t+1), so the scan and the tree run
 // over G / 2 values: 18 dependent additions instead of 16, on half the
-// wavefronts, where the one-bucket-per-pair launch is bound by the issue
+// wavefronts, where the one-bucket-per-lane launch is bound by the issue
 // rate of two wavefronts per SIMD.
-template <unsigned G>
-__global__ void __launch_bounds__(G) k_reduce_buckets_two(const Xyzz29* sums, uint32_t buckets,
-                                                          G1Xyzz* pair_a, G1Xyzz* pair_r) {
+template <unsigned G, int W>
+__global__ void __launch_bounds__(G / 2 * W) k_reduce_buckets_two(const Xyzz29* sums,
+                                                                  uint32_t buckets, G1Xyzz* pair_a,
+                                                                  G1Xyzz* pair_r) {
   constexpr unsigned U = G / 2;
   __shared__ Xyzz29 u[U];
-  const uint32_t t = threadIdx.x >> 1, h = threadIdx.x & 1u;  // bucket pair, lane of the lane pair
+  const uint32_t t = threadIdx.x / W, h = threadIdx.x % W;  // bucket pair, thread of its lane
   const uint32_t b = blockIdx.x * G + 2 * t;
-  auto ex = [](const Fq29& v) { return shfl_pair(v); };
   const Xyzz29 s0 = b < buckets ? load_xyzz29(sums, b) : Xyzz29::identity();
   const Xyzz29 s1 = b + 1 < buckets ? load_xyzz29(sums, b + 1) : Xyzz29::identity();
-  Xyzz29 v = f29::add_pair(s0, s1, h, ex);  // R_t
-  Xyzz29 a = f29::add_pair(v, s1, h, ex);   // A_t
+  Xyzz29 v = lane_add<W>(s0, s1, h);  // R_t
+  Xyzz29 a = lane_add<W>(v, s1, h);   // A_t
   if (h == 0) u[t] = v;
   __syncthreads();
   for (unsigned d = 1; d < U; d <<= 1) {
-    if (t + d < U) v = f29::add_pair(v, u[t + d], h, ex);  // uniform within a pair
+    if (t + d < U) v = lane_add<W>(v, u[t + d], h);  // uniform within a lane
     __syncthreads();
     if (h == 0) u[t] = v;
     __syncthreads();
@@ -728,7 +757,7 @@ __global__ void __launch_bounds__(G) k_reduce_buckets_two(const Xyzz29* sums, ui
   if (threadIdx.x == 0) store_xyzz(pair_r, blockIdx.x, f29::to_xyzz(v));
   if (t + 1 < U) {
     const Xyzz29 w = u[t + 1];
-    a = f29::add_pair(f29::add_pair(a, w, h, ex), w, h, ex);
+    a = lane_add<W>(lane_add<W>(a, w, h), w, h);
   }
   __syncthreads();  // every U_t+1 read before the slots take the A_t + 2 U_t+1
   if (h == 0) u[t] = a;
@@ -736,7 +765,7 @@ __global__ void __launch_bounds__(G) k_reduce_buckets_two(const Xyzz29* sums, ui
   v = a;
   for (unsigned d = U / 2; d > 0; d >>= 1) {
     if (t < d) {
-      v = f29::add_pair(v, u[t + d], h, ex);
+      v = lane_add<W>(v, u[t + d], h);
       if (h == 0) u[t] = v;
     }
     __syncthreads();
@@ -1066,6 +1095,16 @@ Plan choose_plan(size_t n, uint32_t override_c) {
 // wavefronts per SIMD); beyond it the one-lane launch issues as fast.
 constexpr size_t kPairLanes = size_t(1) << 17;
 
+// Four threads per bucket lane (f29::add_quad) in the latency-bound bucket sums
+// and level-1 reductions; HP_MI355X_MSM_QUAD=0 keeps two (f29::add_pair).
+bool quad_lanes() {
+  static const bool on = [] {
+    const char* e = std::getenv("HP_MI355X_MSM_QUAD");
+    return e == nullptr || std::atoi(e) != 0;
+  }();
+  return on;
+}
+
 // Entries per lane task by the largest MSM of a batch (autotuner sweep; for
 // batches of 2^12-point MSMs, 4 beats 2 and 8 in whole proofs: the shorter
 // bucket-sum chains outweigh the longer lane tasks).
@@ -1343,14 +1382,19 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev,
   // wavefront as pairs
   bool pair = 2 * sum_threads_total <= kPairLanes;
   for (uint32_t j = 0; j < njobs; ++j) pair = pair && jobs[j].lanes_log2 <= 5;
-  if (pair)
-    hipLaunchKernelGGL(k_bucket_sums<true>, dim3((unsigned)((2 * sum_threads_total + 255) / 256)),
+  bool quad = pair && quad_lanes() && 4 * sum_threads_total <= kPairLanes;
+  for (uint32_t j = 0; j < njobs; ++j) quad = quad && jobs[j].lanes_log2 <= 4;
+  auto launch_sums = [&](auto kernel, size_t threads_per_lane) {
+    hipLaunchKernelGGL(kernel, dim3((unsigned)((threads_per_lane * sum_threads_total + 255) / 256)),
                        dim3(256), 0, c.stream, task_off, ntask, partials, djobs, njobs,
                        (uint32_t)sum_threads_total, sums, heavy, heavy_count);
+  };
+  if (quad)
+    launch_sums(k_bucket_sums<4>, 4);
+  else if (pair)
+    launch_sums(k_bucket_sums<2>, 2);
   else
-    hipLaunchKernelGGL(k_bucket_sums<false>, dim3((unsigned)((sum_threads_total + 255) / 256)),
-                       dim3(256), 0, c.stream, task_off, ntask, partials, djobs, njobs,
-                       (uint32_t)sum_threads_total, sums, heavy, heavy_count);
+    launch_sums(k_bucket_sums<1>, 1);
   rt::check_launch();
   rt::BufferRef heavy_parts = c.allocate((size_t)kHeavySplitSlots * kHeavySplit * sizeof(Xyzz29));
   Xyzz29* block_sums = static_cast<Xyzz29*>(heavy_parts->ptr);
@@ -1370,11 +1414,17 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev,
   mark(5);
   // two lanes per bucket while the doubled launch stays within two wavefronts
   // per SIMD (the one-lane launch leaves SIMDs idle or is latency-bound)
-  if (rg == kRedSmall)
-    hipLaunchKernelGGL(k_reduce_buckets_pair<kRedSmall>, dim3((unsigned)groups),
+  if (rg == kRedSmall && quad_lanes())
+    hipLaunchKernelGGL((k_reduce_buckets_pair<kRedSmall, 4>), dim3((unsigned)groups),
+                       dim3(4 * kRedSmall), 0, c.stream, sums, G, result, result + groups);
+  else if (rg == kRedSmall)
+    hipLaunchKernelGGL((k_reduce_buckets_pair<kRedSmall, 2>), dim3((unsigned)groups),
                        dim3(2 * kRedSmall), 0, c.stream, sums, G, result, result + groups);
+  else if (G <= kPairReduceMax && quad_lanes())
+    hipLaunchKernelGGL((k_reduce_buckets_two<kRed1, 4>), dim3((unsigned)groups), dim3(2 * kRed1), 0,
+                       c.stream, sums, G, result, result + groups);
   else if (G <= kPairReduceMax)
-    hipLaunchKernelGGL(k_reduce_buckets_two<kRed1>, dim3((unsigned)groups), dim3(kRed1), 0,
+    hipLaunchKernelGGL((k_reduce_buckets_two<kRed1, 2>), dim3((unsigned)groups), dim3(kRed1), 0,
                        c.stream, sums, G, result, result + groups);
   else
     hipLaunchKernelGGL(k_reduce_buckets<kRed1>, dim3((unsigned)groups), dim3(kRed1), 0, c.stream,

@@ -15,7 +15,10 @@
 //   * the host finish's Fq inverse (host_g1.hpp, binary extended Euclid)
 //     against the 32-bit-limb inverse, including 0, 1 and -1;
 //   * the two-lane addition add_pair, its lanes played by two host threads,
-//     limb for limb against add() (general, P + P, P + (-P), identities).
+//     limb for limb against add() (general, P + P, P + (-P), identities);
+//   * the four-lane addition add_quad, its lanes played by four host threads,
+//     against add() (X, ZZ, ZZZ limb for limb, Y modulo p and below 4 p),
+//     also fed its own outputs.
 // Prints "FIELD29 PASS" or the failing checks.
 #include <cstdio>
 #include <cstdint>
@@ -122,6 +125,82 @@ static bool same_limbs(const Xyzz29& a, const Xyzz29& b) {
         a.ZZZ.l[i] != b.ZZZ.l[i])
       return false;
   return true;
+}
+
+// Four host threads stand for the four lanes of a quad in add_quad: get<K>
+// publishes every lane's value, waits, reads lane K's and waits again.
+struct QuadExchange {
+  std::mutex mu;
+  std::condition_variable cv;
+  Fq29 slot[4];
+  unsigned arrived = 0, generation = 0;
+  void barrier() {
+    std::unique_lock<std::mutex> lk(mu);
+    const unsigned gen = generation;
+    if (++arrived == 4) {
+      arrived = 0;
+      ++generation;
+      cv.notify_all();
+    } else {
+      cv.wait(lk, [&] { return generation != gen; });
+    }
+  }
+};
+struct QuadLane {
+  QuadExchange* ex;
+  unsigned lane;
+  template <int K>
+  Fq29 get(const Fq29& v) {
+    {
+      std::lock_guard<std::mutex> g(ex->mu);
+      ex->slot[lane] = v;
+    }
+    ex->barrier();
+    Fq29 r;
+    {
+      std::lock_guard<std::mutex> g(ex->mu);
+      r = ex->slot[K];
+    }
+    ex->barrier();
+    return r;
+  }
+};
+
+// a < the normalised constant k (limbs compared from the top)
+static bool below(const Fq29& a, const uint32_t* k) {
+  for (int i = L - 1; i >= 0; --i)
+    if (a.l[i] != k[i]) return a.l[i] < k[i];
+  return false;
+}
+
+static bool same_fq(const Fq29& a, const Fq29& b) {
+  for (int i = 0; i < L; ++i)
+    if (a.l[i] != b.l[i]) return false;
+  return true;
+}
+
+// add_quad on four threads: every lane returns add(p, q)'s X, ZZ and ZZZ limb
+// for limb and the same Y modulo p (the sum of two reduced products below 4 p).
+static bool quad_add_matches(const Xyzz29& p, const Xyzz29& q) {
+  QuadExchange ex;
+  Xyzz29 out[4];
+  std::thread th[3];
+  for (unsigned l = 1; l < 4; ++l)
+    th[l - 1] = std::thread([&, l] {
+      QuadLane bc{&ex, l};
+      out[l] = add_quad(p, q, l, bc);
+    });
+  QuadLane bc0{&ex, 0};
+  out[0] = add_quad(p, q, 0u, bc0);
+  for (auto& t : th) t.join();
+  const Xyzz29 want = f29::add(p, q);
+  bool ok = true;
+  for (int l = 0; l < 4; ++l) {
+    ok = ok && same_fq(out[l].X, want.X) && same_fq(out[l].ZZ, want.ZZ) && same_fq(out[l].ZZZ, want.ZZZ);
+    ok = ok && same_fq(canonical(out[l].Y), canonical(want.Y));
+    ok = ok && below(out[l].Y, FqP::P4);  // the bound every formula assumes of Y
+  }
+  return ok;
 }
 
 // add_pair on two threads: both lanes must return add(p, q) limb for limb.
@@ -298,6 +377,27 @@ int main() {
       CHECK(pair_add_matches(acc, nacc));
       CHECK(pair_add_matches(acc, Xyzz29::identity()));
       CHECK(pair_add_matches(Xyzz29::identity(), b29));
+      // the four-lane addition, also on its own outputs (Y as a sum of two
+      // reduced products) as the reductions chain it
+      CHECK(quad_add_matches(acc, b29));
+      CHECK(quad_add_matches(acc, acc));
+      CHECK(quad_add_matches(acc, nacc));
+      CHECK(quad_add_matches(acc, Xyzz29::identity()));
+      CHECK(quad_add_matches(Xyzz29::identity(), b29));
+      QuadExchange qx;
+      Xyzz29 r[4];
+      std::thread qt[3];
+      for (unsigned l = 1; l < 4; ++l)
+        qt[l - 1] = std::thread([&, l] {
+          QuadLane bc{&qx, l};
+          r[l] = add_quad(acc, b29, l, bc);
+        });
+      QuadLane bq{&qx, 0};
+      r[0] = add_quad(acc, b29, 0u, bq);
+      for (auto& t : qt) t.join();
+      CHECK(quad_add_matches(r[0], acc));
+      CHECK(quad_add_matches(r[0], r[0]));
+      CHECK(quad_add_matches(b29, r[0]));
     }
   }
   // ---- Fr29: prescaled operands keep products in the stored form

@@ -12,6 +12,9 @@ batched MSM against the oracle, byte for byte.
 * multilinear_kzg_batch and multi_commit of mixed sizes (num_vars 0..16, with
   repeats) entry by entry against the oracle's Pippenger and KZG open
   (backend/src/cpu/multilinear_kzg.rs:40-137);
+* the bucket sums and level-1 reductions on four and on two threads per
+  addition (processes with HP_MI355X_MSM_QUAD=1 / 0; tests/msm_lanes_check.py)
+  against the oracle's Pippenger;
 * the 2^18 MSM against the oracle's Pippenger and the 2^22 MSM of the
   standalone MSM workload against the closed form g * f(t) (layer 0 of an SRS
   with trapdoor t holds g * eq(t, x), so the commitment of f is g * f(t);
@@ -38,6 +41,15 @@ def test_pairs_kernel_every_degree_matches_oracle(classes):
                          capture_output=True, text=True, timeout=300, env=env, cwd=ROOT)
     print(res.stdout[-3000:])
     assert res.returncode == 0 and "PAIRS PASS" in res.stdout, res.stdout[-3000:] + res.stderr[-3000:]
+
+
+@pytest.mark.parametrize("quad", ["1", "0"])
+def test_msm_bucket_lanes_match_oracle(quad):
+    env = dict(os.environ, HP_MI355X_MSM_QUAD=quad)
+    res = subprocess.run([sys.executable, "-u", os.path.join(ROOT, "tests", "msm_lanes_check.py")],
+                         capture_output=True, text=True, timeout=300, env=env, cwd=ROOT)
+    print(res.stdout[-3000:])
+    assert res.returncode == 0 and "LANES PASS" in res.stdout, res.stdout[-3000:] + res.stderr[-3000:]
 
 
 def jellyfish_products(rng, ids):
