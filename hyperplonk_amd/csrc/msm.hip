@@ -1282,7 +1282,13 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev,
       t += (((size_t)jobs[j].padded_buckets << std::min(jobs[j].lanes_log2, max_l)) + 63) / 64 * 64;
     return t;
   };
-  const uint32_t max_l = 2 * sum_threads(6) > kPairLanes && 2 * sum_threads(5) <= kPairLanes ? 5 : 6;
+  // At most 16 lanes per bucket when that admits the four-thread lanes within
+  // two wavefronts per SIMD (a group of 16 lanes of four threads fills a
+  // wavefront): one more dependent addition, each at about 60 % of the
+  // two-thread latency.
+  const uint32_t max_l = quad_lanes() && 4 * sum_threads(4) <= kPairLanes ? 4
+                         : 2 * sum_threads(6) > kPairLanes && 2 * sum_threads(5) <= kPairLanes ? 5
+                                                                                                : 6;
   for (uint32_t j = 0; j < njobs; ++j) {
     const uint32_t l = std::min(jobs[j].lanes_log2, max_l);
     jobs[j].lanes_log2 = l;

@@ -902,11 +902,13 @@ __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
 // wait (a wall-clock limit and an iteration cap) and an abort flag, so every
 // wave always reaches the exit: all rounds done, abort from the host, or the
 // wait limit (the host then relaunches from the round it is at).
-constexpr unsigned kTailThreads = 512;
+constexpr unsigned kTailThreads = 1024;
 constexpr uint32_t kTailRunning = 0, kTailDone = 1, kTailTimedOut = 2, kTailAborted = 3;
 constexpr uint32_t kTailMaxWidth = 9;  // degree <= 8
 // metadata limits of a state that runs its rounds in the loop (kept in LDS)
 constexpr uint32_t kTailMaxProducts = 32, kTailMaxEntries = 128, kTailMaxFactors = 64;
+// (distinct factor, pair) lines of one round kept in LDS (2 x 32 bytes each)
+constexpr uint32_t kTailMaxLines = 768;
 
 struct TailJob {
   const Fr* const* tables[3];  // inputs, fold slot A, fold slot B (device pointer arrays)
@@ -916,7 +918,7 @@ struct TailJob {
   const uint8_t* owner;
   uint32_t products, degree, num_vars, first_round;
   uint32_t entries, factors, pad0, pad1;
-  uint32_t challenge[8];  // stored (Montgomery) form of the first round's challenge
+  uint32_t challenge29[f29::FrP::L];  // the first round's challenge c as c 2^261 in Fr29 limbs
 };
 
 struct TailMailbox {
@@ -924,7 +926,7 @@ struct TailMailbox {
   // host -> device
   alignas(64) uint32_t challenge_round;  // r: `challenge` holds round r's challenge
   uint32_t abort;                        // 1: stop at the next wait
-  alignas(64) uint32_t challenge[8];
+  alignas(64) uint32_t challenge29[f29::FrP::L];  // c 2^261 in Fr29 limbs (prescaled by the host)
   // device -> host
   alignas(64) uint32_t message_round;  // r + 1: `message` holds round r's message
   uint32_t status;                     // kTailRunning / Done / TimedOut / Aborted
@@ -946,117 +948,117 @@ __device__ __forceinline__ void sys_release(uint32_t* p, uint32_t v) {
 }
 
 // Block k of the launch runs the state of boxes[k] from job.first_round to
-// its last round (or until it stops).  Each round: thread per (product p,
-// pair b, point t) items as in k_round_points, strided over the block; the
-// fold writes of round k are read by other threads in round k + 1 after the
-// block barrier (one workgroup, one CU).
+// its last round (or until it stops).  Each round runs in three phases (one
+// workgroup, one CU):
+//   1. thread per (distinct factor f, pair b) folds f on b with the previous
+//      challenge (or loads it in round 0), stores the fold for the next round
+//      and leaves the line's ends lo, hi in LDS: every factor is folded once
+//      per pair, however many products and points use it;
+//   2. the block's 16 wavefronts split over the points, 16 / (d + 1) per
+//      point (at least one; wavefront w takes point w / ws, w >= ws (d + 1)
+//      idle): the lanes of the ws wavefronts of t take its (product, pair)
+//      items, multiply the coefficient by the product's lines at t from LDS,
+//      and sum; pairs is a power of two, so an item splits by shifts;
+//   3. wavefront sums, then one thread per point adds its ws of them.
+// The fold writes of round k are read by other threads in round k + 1 after
+// the block barriers.
 __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const* boxes,
                                                              uint64_t wait_ticks,
                                                              uint32_t spin_cap) {
   TailMailbox* box = boxes[blockIdx.x];
   __shared__ TailJob job;
-  __shared__ uint32_t s_ch[8];
+  __shared__ Fr29 s_c29;
   __shared__ uint32_t s_stop;
-  __shared__ Fr lds[kTailThreads / 64][kTailMaxWidth];
   __shared__ Fr msg[kTailMaxWidth];
-  // the state's metadata, kept in LDS for all its rounds (one global load
-  // fewer per factor in every item)
+  __shared__ Fr part[kTailThreads / 64];  // wavefront sums
+  // the state's metadata, kept in LDS for all its rounds
   __shared__ uint32_t s_off[kTailMaxProducts + 1];
   __shared__ uint32_t s_slot[kTailMaxEntries];
-  __shared__ uint8_t s_owner[kTailMaxEntries];
   __shared__ Fr s_coeff[kTailMaxProducts];
   __shared__ const Fr* s_tab[3][kTailMaxFactors];
+  // line ends of every (distinct factor, pair) of the round, stored form
+  __shared__ Fr s_lo[kTailMaxLines], s_hi[kTailMaxLines];
   if (threadIdx.x < sizeof(TailJob) / 4)
     reinterpret_cast<uint32_t*>(&job)[threadIdx.x] =
         reinterpret_cast<const uint32_t*>(&box->job)[threadIdx.x];
   __syncthreads();
-  if (threadIdx.x < 8) s_ch[threadIdx.x] = job.challenge[threadIdx.x];
+  if (threadIdx.x < f29::FrP::L) s_c29.l[threadIdx.x] = job.challenge29[threadIdx.x];
   if (threadIdx.x <= job.products) s_off[threadIdx.x] = job.offsets[threadIdx.x];
   if (threadIdx.x < job.products) s_coeff[threadIdx.x] = load_fr(job.coeffs29, threadIdx.x);
-  if (threadIdx.x < job.entries) {
-    s_slot[threadIdx.x] = job.slots[threadIdx.x];
-    s_owner[threadIdx.x] = job.owner[threadIdx.x];
-  }
+  if (threadIdx.x < job.entries) s_slot[threadIdx.x] = job.slots[threadIdx.x];
   if (threadIdx.x < job.factors)
     for (int t = 0; t < 3; ++t) s_tab[t][threadIdx.x] = job.tables[t][threadIdx.x];
   __syncthreads();
-  const uint32_t n = job.num_vars, width = job.degree + 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __shared__ Fr29 s_c29;
+  const uint32_t n = job.num_vars, width = job.degree + 1, F = job.factors;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t k = job.first_round;; ++k) {
-    if (threadIdx.x == 0) {
-      Fr ch;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) ch.v[i] = s_ch[i];
-      s_c29 = f29::prescale_fr(ch);  // once per round, for every thread
-    }
-    __syncthreads();
     const Fr29 c29 = s_c29;
-    const size_t pairs = size_t(1) << (n - k - 1);
+    const uint32_t lp = n - k - 1;  // log2 of the round's pairs
+    const uint32_t pairs = 1u << lp;
     const bool fold = k > 0;
     const Fr* const* src = k <= 1 ? s_tab[0] : s_tab[1 + ((k - 2) & 1)];
     const Fr* const* dst = k == 0 ? nullptr : s_tab[1 + ((k - 1) & 1)];
-    // lazy Fr29 sums (each term < 3.3 r; reduced every 16 terms, below 56 r)
-    Fr29 acc29[kTailMaxWidth];
-#pragma unroll
-    for (uint32_t t = 0; t < kTailMaxWidth; ++t) acc29[t] = f29::zero_t<f29::FrP>();
-    uint32_t terms = 0;
-    const size_t items = pairs * job.products * width;
-    for (size_t it = threadIdx.x; it < items; it += kTailThreads) {
-      const uint32_t t = static_cast<uint32_t>(it % width);
-      const size_t pb = it / width;
-      const uint32_t p = static_cast<uint32_t>(pb / pairs);
-      const size_t b = pb - (size_t)p * pairs;
-      const uint32_t begin = s_off[p], end = s_off[p + 1];
-      Fr29 term = f29::unpack_fr(s_coeff[p]);
-      for (uint32_t e = begin; e < end; ++e) {
-        const uint32_t f = s_slot[e];
-        Fr29 lo, hi;
-        if (fold) {
-          const Fr* o = src[f];
-          const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
-          const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
-          lo = fold29(c29, a0, a1);
-          hi = fold29(c29, a2, a3);
-          if (t == 0 && s_owner[e]) {
-            store_fr(const_cast<Fr*>(dst[f]), 2 * b, f29::pack_fr_canonical(lo));
-            store_fr(const_cast<Fr*>(dst[f]), 2 * b + 1, f29::pack_fr_canonical(hi));
-          }
-        } else {
-          const Fr* tb = src[f];
-          lo = load29(tb, 2 * b);
-          hi = load29(tb, 2 * b + 1);
-        }
-        const Fr29 d = f29::sub1p(hi, lo);
-        Fr29 line = lo;
-        for (uint32_t q = 0; q < t; ++q) line = f29::add(line, d);
-        term = f29::mul(term, line);
-      }
-#pragma unroll
-      for (uint32_t q = 0; q < kTailMaxWidth; ++q)
-        if (q == t) acc29[q] = f29::add(acc29[q], term);
-      if ((++terms & 15u) == 0) {
-#pragma unroll
-        for (uint32_t q = 0; q < kTailMaxWidth; ++q) acc29[q] = f29::reduce_est(acc29[q]);
-      }
-    }
-    Fr acc[kTailMaxWidth];
-#pragma unroll
-    for (uint32_t t = 0; t < kTailMaxWidth; ++t)
-      acc[t] = t < width ? f29::pack_fr(f29::reduce_est(acc29[t])) : Fr::zero();
-    // message: wavefront sums, then one thread per point t
-#pragma unroll
-    for (uint32_t t = 0; t < kTailMaxWidth; ++t) {
-      if (t < width) {
-        const Fr w = wave_sum(acc[t]);
-        if (lane == 0) lds[wave][t] = w;
+    // 1. lines of every distinct factor on every pair
+    for (uint32_t u = threadIdx.x; u < (F << lp); u += kTailThreads) {
+      const uint32_t f = u >> lp, b = u & (pairs - 1);
+      const Fr* o = src[f];
+      if (fold) {
+        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+        const Fr lo = f29::pack_fr_canonical(fold29(c29, a0, a1));
+        const Fr hi = f29::pack_fr_canonical(fold29(c29, a2, a3));
+        store_fr(const_cast<Fr*>(dst[f]), 2 * b, lo);
+        store_fr(const_cast<Fr*>(dst[f]), 2 * b + 1, hi);
+        s_lo[u] = lo;
+        s_hi[u] = hi;
+      } else {
+        s_lo[u] = load_fr(o, 2 * b);
+        s_hi[u] = load_fr(o, 2 * b + 1);
       }
     }
     __syncthreads();  // also orders this round's fold writes before the next round's reads
-    if (threadIdx.x < width) {
-      Fr sum = Fr::zero();
+    // 2, 3. ws wavefronts per point: lazy Fr29 sums (each term < 1.3 r;
+    // reduced every 16 terms, below 24 r), then a butterfly over the lanes
+    // that hold items (Fr additions on the packed values)
+    const uint32_t items = job.products << lp;
+    constexpr uint32_t kWaves = kTailThreads / 64;
+    const uint32_t ws_fit = width >= kWaves ? 1u : kWaves / width, ws_need = (items + 63) / 64;
+    const uint32_t ws = ws_fit < ws_need ? ws_fit : ws_need;  // >= 1: items >= 1
+    const uint32_t held = items < 64 ? items : 64;  // lanes with items in a wavefront
+    for (uint32_t w = wave; w < ws * width; w += kWaves) {
+      const uint32_t t = w / ws, j = w - t * ws;
+      Fr29 acc = f29::zero_t<f29::FrP>();
+      uint32_t terms = 0;
+      for (uint32_t it = j * 64 + lane; it < items; it += ws * 64) {
+        const uint32_t p = it >> lp, b = it & (pairs - 1);
+        Fr29 term = f29::unpack_fr(s_coeff[p]);
+        for (uint32_t e = s_off[p]; e < s_off[p + 1]; ++e) {
+          const uint32_t u = (s_slot[e] << lp) | b;
+          const Fr29 lo = f29::unpack_fr(s_lo[u]);
+          const Fr29 d = f29::sub1p(f29::unpack_fr(s_hi[u]), lo);
+          Fr29 line = lo;  // lo + t (hi - lo) < 17 r
+          for (uint32_t q = 0; q < t; ++q) line = f29::add(line, d);
+          term = f29::mul(term, line);
+        }
+        acc = f29::add(acc, term);
+        if ((++terms & 15u) == 0) acc = f29::reduce_est(acc);
+      }
+      Fr v = f29::pack_fr_reduced(f29::reduce_est(acc));
 #pragma unroll
-      for (int w = 0; w < (int)(kTailThreads / 64); ++w) sum = sum + lds[w][threadIdx.x];
+      for (uint32_t off = 32; off > 0; off >>= 1) {
+        if (off < held) {  // uniform: lanes >= held hold zero
+          Fr o;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) o.v[i] = __shfl_xor(v.v[i], off, 64);
+          v = v + o;
+        }
+      }
+      if (lane == 0) part[w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < width) {
+      Fr sum = part[threadIdx.x * ws];
+      for (uint32_t j = 1; j < ws; ++j) sum = sum + part[threadIdx.x * ws + j];
       msg[threadIdx.x] = sum;  // stored form: the host converts to canonical
     }
     __syncthreads();
@@ -1073,27 +1075,34 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const*
       }
     }
     if (last) return;
-    // the challenge of round k + 1 (bounded wait; abort from the host)
-    if (threadIdx.x == 0) {
+    // the challenge of round k + 1 (bounded wait; abort from the host):
+    // thread 0 polls the round word, then lanes 0..8 of its wavefront read
+    // the nine words of the prescaled challenge with one load instruction
+    // (one round trip to host memory instead of nine in a row)
+    if (threadIdx.x < 64) {
       uint32_t stop = kTailRunning;
-      const uint64_t start = static_cast<uint64_t>(wall_clock64());
-      for (uint32_t spin = 0;; ++spin) {
-        if (sys_load_relaxed(&box->abort) != 0) {
-          stop = kTailAborted;
-          break;
+      if (threadIdx.x == 0) {
+        const uint64_t start = static_cast<uint64_t>(wall_clock64());
+        for (uint32_t spin = 0;; ++spin) {
+          if (sys_load_relaxed(&box->abort) != 0) {
+            stop = kTailAborted;
+            break;
+          }
+          if (sys_load_relaxed(&box->challenge_round) == k + 1) break;
+          if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
+            stop = kTailTimedOut;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
         }
-        if (sys_load_relaxed(&box->challenge_round) == k + 1) {
-          for (int i = 0; i < 8; ++i) s_ch[i] = sys_load_relaxed(&box->challenge[i]);
-          break;
-        }
-        if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
-          stop = kTailTimedOut;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
       }
-      if (stop != kTailRunning) sys_release(&box->status, stop);
-      s_stop = stop;
+      stop = __shfl(stop, 0, 64);
+      if (stop == kTailRunning && threadIdx.x < f29::FrP::L)
+        s_c29.l[threadIdx.x] = sys_load_relaxed(&box->challenge29[threadIdx.x]);
+      if (threadIdx.x == 0) {
+        if (stop != kTailRunning) sys_release(&box->status, stop);
+        s_stop = stop;
+      }
     }
     __syncthreads();
     if (s_stop != kTailRunning) return;
@@ -1450,12 +1459,13 @@ uint64_t tail_wait_us() {
 }
 
 // A state runs its rounds in the loop once a round has at most this many
-// (product, pair, point) items: HP_MI355X_SC_TAIL_ITEMS for rounds of several
-// states at once (round_batch; default 4096: measured ~5 % faster on the
-// 64-sumcheck batch), HP_MI355X_SC_TAIL_SINGLE_ITEMS for one state's rounds
-// (default 0, off: one block per round computes no faster than the launch
-// path's several, and the handshake saves about what the launch costs, so
-// measured no gain alone); 0 turns the loop off.
+// (product, pair, point) items (and its (distinct factor, pair) lines fit the
+// block's LDS): HP_MI355X_SC_TAIL_ITEMS for rounds of several states at once
+// (round_batch; default 4096: measured ~5 % faster on the 64-sumcheck batch),
+// HP_MI355X_SC_TAIL_SINGLE_ITEMS for one state's rounds (default 16384: at
+// n = 20, degree 2, rounds 12-19 run in the loop at ~9 us each against
+// 21-29 us per launched round, the whole sumcheck 0.97 -> 0.88 ms); 0 turns
+// the loop off.
 size_t tail_items(bool batch) {
   static const size_t batch_v = [] {
     const char* e = std::getenv("HP_MI355X_SC_TAIL_ITEMS");
@@ -1463,7 +1473,7 @@ size_t tail_items(bool batch) {
   }();
   static const size_t single_v = [] {
     const char* e = std::getenv("HP_MI355X_SC_TAIL_SINGLE_ITEMS");
-    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(0);
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(16384);
   }();
   return batch ? batch_v : single_v;
 }
@@ -1476,6 +1486,7 @@ bool tail_ok(const sc::State& s, bool batch) {
       s.distinct > sc::kTailMaxFactors)
     return false;
   const size_t pairs = size_t(1) << (s.num_vars - s.round - 1);
+  if (s.distinct * pairs > sc::kTailMaxLines) return false;
   return pairs * s.product_count * (s.max_degree + 1) <= tail_items(batch);
 }
 
@@ -1572,7 +1583,8 @@ void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>&
     j.first_round = static_cast<uint32_t>(s.round);
     j.entries = static_cast<uint32_t>(s.index_count);
     j.factors = static_cast<uint32_t>(s.distinct);
-    std::memcpy(j.challenge, challenges[i].v, sizeof j.challenge);
+    const f29::Fr29 c29 = f29::prescale_fr(challenges[i]);
+    std::memcpy(j.challenge29, c29.l, sizeof j.challenge29);
     s.box->challenge_round = 0;
     s.box->abort = 0;
     s.box->message_round = 0;
@@ -1591,7 +1603,8 @@ void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>&
 }
 
 void tail_post(sc::State& s, const Fr& challenge) {
-  std::memcpy(s.box->challenge, challenge.v, sizeof s.box->challenge);
+  const f29::Fr29 c29 = f29::prescale_fr(challenge);
+  std::memcpy(s.box->challenge29, c29.l, sizeof s.box->challenge29);
   __atomic_store_n(&s.box->challenge_round, static_cast<uint32_t>(s.round), __ATOMIC_RELEASE);
 }
 
@@ -1652,9 +1665,14 @@ bool tail_rounds(rt::Context& c, const std::vector<sc::State*>& states, const st
       launch_tails(c, pool, states, chs);
     } else {
       for (size_t i = 0; i < states.size(); ++i) tail_post(*states[i], chs[i]);
+      // the runtime's error state, checked where a launched round checks its
+      // launch (a loop round makes no other HIP call)
+      rt::check_launch();
     }
     bool all = true;
     for (size_t i = 0; i < states.size() && all; ++i) all = tail_wait(c, *states[i], outs[i]);
+    // and where a launched round checks its synchronisation
+    if (all) rt::check_launch();
     if (all) {
       for (sc::State* s : states) tail_advance(pool, *s);
       return true;

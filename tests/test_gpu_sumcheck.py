@@ -348,10 +348,11 @@ def test_round_graph_replay_gives_same_messages(degree):
 @pytest.mark.parametrize("degree", ["2", "5"])
 def test_round_loops_give_the_same_messages(degree):
     """The persistent round loops change no message: whole n = 12 sumchecks
-    of one state with the loops off (default for one state), on for rounds of
-    at most 4096 items, taking every round from the first (a large item
-    limit), and with a zero device wait limit (every block stops right after
-    its message, so each round relaunches the loop from the state's round)."""
+    of one state with the loops on for rounds of at most 16384 items (the
+    default), off, on for rounds of at most 4096 items, taking every round
+    whose lines fit the block (a large item limit), and with a zero device
+    wait limit (every block stops right after its message, so each round
+    relaunches the loop from the state's round)."""
     import json
     import os
     import subprocess
@@ -359,7 +360,7 @@ def test_round_loops_give_the_same_messages(degree):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     prog = os.path.join(root, "_build", "prove_bench")
     digests = []
-    for env_extra in ({}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096"},
+    for env_extra in ({}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "0"}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096"},
                       {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "100000000"},
                       {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096", "HP_MI355X_SC_TAIL_WAIT_US": "0"}):
         env = dict(os.environ, **env_extra)
