@@ -4,8 +4,9 @@
  * backend/src/cpu/multilinear_kzg.rs:55,131): sum_i s_i * P_i, result as the
  * unique affine point.  Two independent routes:
  *   - oracle_msm_naive: per-point double-and-add, summed (small inputs);
- *   - oracle_msm_pippenger: unsigned c-bit windows, Jacobian buckets, running
- *     sums, threads over disjoint slices of the points (the CPU baseline).
+ *   - oracle_msm_pippenger: unsigned c-bit windows, Jacobian buckets filled by
+ *     mixed (Jacobian + affine) additions, running sums, threads over disjoint
+ *     slices of the points (the CPU baseline).
  * Both are checked against Python-integer fixtures of the closed form
  * (sum a_i s_i) * G for bases a_i * G. */
 #include <pthread.h>
