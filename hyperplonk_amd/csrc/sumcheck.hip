@@ -348,19 +348,6 @@ constexpr uint32_t kMaxClasses = 16, kMaxClassWidth = 64;
 // Bounds (r the modulus): lo, hi canonical, d < 2r, lines < (2M + 1) r <= 17r,
 // a one-factor term < 3r, a product term < 5.1r (17^2 / 70.7 + 1); eight
 // terms on a reduced accumulator (< 3r) stay below 44r < 2^261.
-// x^M by squarings (x < 17 r: every intermediate stays below 5.1 r, the
-// bound of a product term), M - 1 products' worth of 2^261 divisions
-template <int M>
-__device__ __forceinline__ Fr29 pow_lines(const Fr29& x) {
-  if constexpr (M == 1) {
-    return x;
-  } else if constexpr (M % 2 == 0) {
-    return f29::sqr(pow_lines<M / 2>(x));
-  } else {
-    return f29::mul(f29::sqr(pow_lines<M / 2>(x)), x);
-  }
-}
-
 template <int M>
 __device__ __forceinline__ void class_pass(const RoundJob& job, const Fr* const* tbl, uint32_t mb,
                                            uint32_t me, size_t b0, size_t stride,
@@ -374,45 +361,27 @@ __device__ __forceinline__ void class_pass(const RoundJob& job, const Fr* const*
     for (uint32_t q = mb; q < me; ++q) {
       const uint32_t begin = job.offsets[members[q]];
       Fr29 term[M + 1];
-      // a power of one factor (every entry the same table, e.g. w^5 in a
-      // custom gate): its line at t raised to M by squarings, the same
-      // value and scale as M - 1 products (each product divides by 2^261)
-      bool power = M >= 2;
-#pragma unroll
-      for (int e = 1; e < M; ++e) power = power && job.slots[begin + e] == job.slots[begin];
-      if (power) {
+      {
         const Fr* t0 = tbl[job.slots[begin]];
         const Fr29 lo = load29(t0, 2 * b), hi = load29(t0, 2 * b + 1);
         const Fr29 d = f29::sub1p(hi, lo);
         Fr29 line = lo;
 #pragma unroll
         for (int t = 0; t <= M; ++t) {
-          term[t] = pow_lines<M>(line);
+          term[t] = line;
           if (t < M) line = f29::add(line, d);
         }
-      } else {
-        {
-          const Fr* t0 = tbl[job.slots[begin]];
-          const Fr29 lo = load29(t0, 2 * b), hi = load29(t0, 2 * b + 1);
-          const Fr29 d = f29::sub1p(hi, lo);
-          Fr29 line = lo;
-#pragma unroll
-          for (int t = 0; t <= M; ++t) {
-            term[t] = line;
-            if (t < M) line = f29::add(line, d);
-          }
-        }
+      }
 #pragma unroll 1
-        for (int e = 1; e < M; ++e) {
-          const Fr* tt = tbl[job.slots[begin + e]];
-          const Fr29 lo = load29(tt, 2 * b), hi = load29(tt, 2 * b + 1);
-          const Fr29 d = f29::sub1p(hi, lo);
-          Fr29 line = lo;
+      for (int e = 1; e < M; ++e) {
+        const Fr* tt = tbl[job.slots[begin + e]];
+        const Fr29 lo = load29(tt, 2 * b), hi = load29(tt, 2 * b + 1);
+        const Fr29 d = f29::sub1p(hi, lo);
+        Fr29 line = lo;
 #pragma unroll
-          for (int t = 0; t <= M; ++t) {
-            term[t] = f29::mul(term[t], line);
-            if (t < M) line = f29::add(line, d);
-          }
+        for (int t = 0; t <= M; ++t) {
+          term[t] = f29::mul(term[t], line);
+          if (t < M) line = f29::add(line, d);
         }
       }
 #pragma unroll
@@ -939,7 +908,7 @@ constexpr uint32_t kTailMaxWidth = 9;  // degree <= 8
 // metadata limits of a state that runs its rounds in the loop (kept in LDS)
 constexpr uint32_t kTailMaxProducts = 32, kTailMaxEntries = 128, kTailMaxFactors = 64;
 // (distinct factor, pair) lines of one round kept in LDS (2 x 32 bytes each)
-constexpr uint32_t kTailMaxLines = 1024;
+constexpr uint32_t kTailMaxLines = 768;
 
 struct TailJob {
   const Fr* const* tables[3];  // inputs, fold slot A, fold slot B (device pointer arrays)
