@@ -10,7 +10,9 @@ degrees 1..8 at num_vars 8, 10 and 12 with two product lists:
   * products sharing coefficients so that classes hold several products:
     (1, [f0, f1]), (1, [f2, f0]), (r - 1, [f1, f2]), (1, [f0..f_{D-1}]),
     (c, [f1]), (c, [f2]), (c, [f0]) (arity-1 and arity-2 classes of three,
-    a negated coefficient in a class of its own).
+    a negated coefficient in a class of its own), (c', f1^D) and (1, f2^D)
+    (powers of one table: alone in a class and in the class of
+    (1, [f0..f_{D-1}])).
 Prints PAIRS PASS and exits 0 when every message matches
 (cf. backend/src/cpu/sum_check.rs:74-180)."""
 import os
@@ -47,6 +49,9 @@ def main() -> int:
                 prods = [(1, list(range(degree))), (c, [1]), (c, [2]), (c, [0])]
                 if degree >= 2:
                     prods += [(1, [0, 1]), (1, [2, 0]), (pf.R - 1, [1, 2])]
+                    # powers of one table: alone in a class, and beside a
+                    # product of distinct tables in the degree-D class
+                    prods += [(crng.nonboolean(), [1] * degree), (1, [2] * degree)]
             mles = [be.mle_from_evaluations(nv, t) for t in tabs]
             vp = VirtualPolynomial(nv)
             for c, idx in prods:
