@@ -1717,18 +1717,30 @@ void autotune(rt::Context& c, size_t log_n, uint32_t* best_c, uint32_t* best_cap
     if (extra + 3 >= rule && extra <= rule + 3 &&
         std::find(cands.begin(), cands.end(), extra) == cands.end())
       cands.push_back(extra);
+  // A candidate replaces the built-in choice only when it is faster by more
+  // than kMargin: near ties (within the run-to-run spread) keep the rule, so
+  // the plan does not flip between runs on timing noise.
+  constexpr float kMargin = 0.985f;
   float best = 1e30f;
   uint32_t bc = rule;
   Layer best_layer;
+  float rule_ms = 1e30f;
   for (uint32_t cc : cands) {
     Layer layer;
     prepare_layer_from_device(c, layer, static_cast<const G1Affine*>(pts->ptr), n, cc);
     const float ms = time_msm(c, layer, scalars, default_cap(n), 3);
+    if (cc == rule) rule_ms = ms;
     if (ms < best) {
       best = ms;
       bc = cc;
       best_layer = std::move(layer);
     }
+  }
+  if (bc != rule && best > rule_ms * kMargin) {
+    bc = rule;
+    best = rule_ms;
+    best_layer = Layer();
+    prepare_layer_from_device(c, best_layer, static_cast<const G1Affine*>(pts->ptr), n, rule);
   }
   // caps: two passes (the second in reverse order), each cap's better median
   const uint32_t caps[] = {2u, 4u, 8u, 16u, 32u, 64u};
@@ -1738,12 +1750,19 @@ void autotune(rt::Context& c, size_t log_n, uint32_t* best_c, uint32_t* best_cap
   for (int i = kCaps - 1; i >= 0; --i)
     cap_ms[i] = std::min(cap_ms[i], time_msm(c, best_layer, scalars, caps[i], 5));
   uint32_t bcap = 0;
-  float best2 = 1e30f;
-  for (int i = 0; i < kCaps; ++i)
+  float best2 = 1e30f, def_ms = 1e30f;
+  const uint32_t def_cap = default_cap(n);
+  for (int i = 0; i < kCaps; ++i) {
+    if (caps[i] == def_cap) def_ms = cap_ms[i];
     if (cap_ms[i] < best2) {
       best2 = cap_ms[i];
       bcap = caps[i];
     }
+  }
+  if (bcap != def_cap && def_ms < 1e29f && best2 > def_ms * kMargin) {
+    bcap = def_cap;
+    best2 = def_ms;
+  }
   c.tuned_window[log_n] = static_cast<uint8_t>(bc);
   c.tuned_cap[log_n] = static_cast<uint8_t>(bcap);
   *best_c = bc;
