@@ -221,12 +221,21 @@ struct Fp {
     *out = from_canonical(c);
     return true;
   }
-  // little-endian integer of any length, reduced mod p (Horner over bytes
-  // from the most significant end)
+  // little-endian integer of any length, reduced mod p: Horner over chunks of
+  // N limbs from the most significant end, base 2^(64N) (= S::R1 mod p), so a
+  // 64-byte Fr challenge costs three Montgomery products instead of 128
   static Fp from_le_bytes_mod_order(const uint8_t* in, size_t len) {
-    const Fp base = from_u64(256);
+    constexpr size_t kChunk = 8 * N;
+    const Fp base = from_canonical(S::R1);
     Fp acc = zero();
-    for (size_t i = len; i-- > 0;) acc = acc * base + from_u64(in[i]);
+    size_t end = len;
+    while (end > 0) {
+      const size_t begin = (end - 1) / kChunk * kChunk;  // the top chunk may be partial
+      Limbs<N> c{};
+      for (size_t i = begin; i < end; ++i) c[(i - begin) / 8] |= static_cast<u64>(in[i]) << (8 * ((i - begin) % 8));
+      acc = acc * base + from_limbs_reduce(c);
+      end = begin;
+    }
     return acc;
   }
   // arkworks' "y > -y" on canonical integers

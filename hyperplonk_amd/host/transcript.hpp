@@ -22,23 +22,63 @@
 namespace hyperplonk {
 namespace host {
 
+// FIPS 202 algorithm 5
+constexpr int keccak_rc_bit(int t) {
+  if (t % 255 == 0) return 1;
+  unsigned r = 1;
+  for (int i = 1; i <= t % 255; ++i) {
+    r <<= 1;
+    if (r & 0x100) r ^= 0x171;
+  }
+  return r & 1;
+}
+
+struct KeccakTables {
+  uint64_t rc[24];
+  int rho[25];
+};
+
+// evaluated by the compiler, so the permutation's rotations are constants
+constexpr KeccakTables make_keccak_tables() {
+  KeccakTables t{};
+  for (int ir = 0; ir < 24; ++ir) {
+    uint64_t rc = 0;
+    for (int j = 0; j <= 6; ++j)
+      if (keccak_rc_bit(j + 7 * ir)) rc |= uint64_t(1) << ((1 << j) - 1);
+    t.rc[ir] = rc;
+  }
+  t.rho[0] = 0;
+  int x = 1, y = 0;
+  for (int k = 0; k < 24; ++k) {
+    t.rho[x + 5 * y] = ((k + 1) * (k + 2) / 2) % 64;
+    const int nx = y, ny = (2 * x + 3 * y) % 5;
+    x = nx;
+    y = ny;
+  }
+  return t;
+}
+
 class Keccak {
  public:
   static void permute(uint8_t state[200]) {
-    static const Tables t = make_tables();
+    static constexpr KeccakTables t = make_keccak_tables();
     uint64_t a[25];
     std::memcpy(a, state, 200);  // lanes are little-endian 64-bit words
     for (int round = 0; round < 24; ++round) {
-      uint64_t c[5], d[5];
+      uint64_t c[5], d[5], b[25];
+#pragma GCC unroll 5
       for (int x = 0; x < 5; ++x) c[x] = a[x] ^ a[x + 5] ^ a[x + 10] ^ a[x + 15] ^ a[x + 20];
+#pragma GCC unroll 5
       for (int x = 0; x < 5; ++x) d[x] = c[(x + 4) % 5] ^ rotl(c[(x + 1) % 5], 1);
-      for (int i = 0; i < 25; ++i) a[i] ^= d[i % 5];
-      uint64_t b[25];
+#pragma GCC unroll 5
       for (int x = 0; x < 5; ++x)
+#pragma GCC unroll 5
         for (int y = 0; y < 5; ++y)
-          b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl(a[x + 5 * y], t.rho[x + 5 * y]);
-      for (int x = 0; x < 5; ++x)
-        for (int y = 0; y < 5; ++y)
+          b[y + 5 * ((2 * x + 3 * y) % 5)] = rotl(a[x + 5 * y] ^ d[x], t.rho[x + 5 * y]);
+#pragma GCC unroll 5
+      for (int y = 0; y < 5; ++y)
+#pragma GCC unroll 5
+        for (int x = 0; x < 5; ++x)
           a[x + 5 * y] = b[x + 5 * y] ^ (~b[(x + 1) % 5 + 5 * y] & b[(x + 2) % 5 + 5 * y]);
       a[0] ^= t.rc[round];
     }
@@ -46,39 +86,7 @@ class Keccak {
   }
 
  private:
-  struct Tables {
-    uint64_t rc[24];
-    int rho[25];
-  };
   static uint64_t rotl(uint64_t v, int n) { return n ? (v << n) | (v >> (64 - n)) : v; }
-  // FIPS 202 algorithm 5
-  static int rc_bit(int t) {
-    if (t % 255 == 0) return 1;
-    unsigned r = 1;
-    for (int i = 1; i <= t % 255; ++i) {
-      r <<= 1;
-      if (r & 0x100) r ^= 0x171;
-    }
-    return r & 1;
-  }
-  static Tables make_tables() {
-    Tables t{};
-    for (int ir = 0; ir < 24; ++ir) {
-      uint64_t rc = 0;
-      for (int j = 0; j <= 6; ++j)
-        if (rc_bit(j + 7 * ir)) rc |= uint64_t(1) << ((1 << j) - 1);
-      t.rc[ir] = rc;
-    }
-    t.rho[0] = 0;
-    int x = 1, y = 0;
-    for (int k = 0; k < 24; ++k) {
-      t.rho[x + 5 * y] = ((k + 1) * (k + 2) / 2) % 64;
-      const int nx = y, ny = (2 * x + 3 * y) % 5;
-      x = nx;
-      y = ny;
-    }
-    return t;
-  }
 };
 
 // SHA3-256 (FIPS 202: rate 136, domain bits 01 then pad10*1); used by the

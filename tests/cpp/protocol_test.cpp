@@ -89,6 +89,24 @@ static void kat() {
     std::printf("modorder %s\n", Fr::from_le_bytes_mod_order(b, 64).hex().c_str());
     std::memset(b, 0xff, 64);
     std::printf("modorder %s\n", Fr::from_le_bytes_mod_order(b, 64).hex().c_str());
+    // random strings of lengths around the chunk sizes (32 bytes for Fr, 48
+    // for Fq): "modorderx <field> <input hex, byte order as given> <result>"
+    SplitMix64 brng(11);
+    const size_t lens[] = {0, 1, 7, 31, 32, 33, 47, 48, 49, 64, 65, 96, 97, 130};
+    for (size_t len : lens) {
+      uint8_t in[130];
+      for (size_t i = 0; i < len; ++i) in[i] = static_cast<uint8_t>(brng.next() >> 56);
+      if (len == 96) std::memset(in, 0xff, len);
+      std::string hx;
+      for (size_t i = 0; i < len; ++i) {
+        char h[3];
+        std::snprintf(h, 3, "%02x", in[i]);
+        hx += h;
+      }
+      if (hx.empty()) hx = "-";
+      std::printf("modorderx r %s %s\n", hx.c_str(), Fr::from_le_bytes_mod_order(in, len).hex().c_str());
+      std::printf("modorderx q %s %s\n", hx.c_str(), Fq::from_le_bytes_mod_order(in, len).hex().c_str());
+    }
   }
   // curves
   const G1Aff g1 = g1_generator();

@@ -20,6 +20,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "_build", "protocol_test")
 R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+Q = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
 MERLIN_SIMPLE = "d5a21972d0d5fe320c0d263fac7fffb8145aa640af6e9bca177c03c7efcf0615"
 G2_GEN_COMPRESSED = (
     "93e02b6052719f607dacd3a088274f65596bd0d09920b61ab5da61bbdc7f5049334cf11213945d57e5ac7d055d042b7e"
@@ -59,6 +60,14 @@ def test_merlin_known_answer(kat_lines):
 def test_mod_order_reduction(kat_lines):
     got = [int(l[1], 16) for l in kat_lines if l[0] == "modorder"]
     assert got == [2**256 % R, (2**512 - 1) % R]
+
+
+def test_mod_order_reduction_lengths(kat_lines):
+    rows = [l for l in kat_lines if l[0] == "modorderx"]
+    assert len(rows) == 28
+    for _, field, hx, got in rows:
+        data = b"" if hx == "-" else bytes.fromhex(hx)
+        assert int(got, 16) == int.from_bytes(data, "little") % (R if field == "r" else Q), (field, len(data))
 
 
 def test_compressed_g1(kat_lines):
