@@ -895,20 +895,45 @@ __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
 //
 // Once a state's table is small, a round's arithmetic takes a few
 // microseconds and a launch per round (launch, end-of-kernel write-back, the
-// host waking on the stream) costs several times that.  One block per state
-// then runs every remaining round in a loop and talks to the host through a
-// mailbox in pinned host memory: it publishes each message with a
-// system-scope release and waits for the host's next challenge, with a bounded
-// wait (a wall-clock limit and an iteration cap) and an abort flag, so every
-// wave always reaches the exit: all rounds done, abort from the host, or the
-// wait limit (the host then relaunches from the round it is at).
+// host waking on the stream) costs several times that.  Blocks then run the
+// remaining rounds in a loop and talk to the host through a mailbox in pinned
+// host memory: each message is published there and the blocks wait for the
+// host's next challenge, with a bounded wait (a wall-clock limit and an
+// iteration cap) and an abort flag, so every wave always reaches the exit:
+// all rounds done, abort from the host, or the wait limit (the host then
+// relaunches from the round it is at).
+//
+// One state may run on G = 2^b blocks (a wide loop, one state only): block g
+// owns pairs [g P, (g + 1) P) of every round (P = pairs / G).  Pair i of a
+// round folds entries 4i .. 4i + 3 of the previous round's table into entries
+// 2i, 2i + 1, so a block only ever reads what it wrote itself and the blocks
+// exchange nothing but their partial messages.  A wide loop runs in up to
+// three stages of G, then fewer, then one block (every block keeps at least
+// one pair); at a stage change the blocks beyond the new count exit and the
+// others take over their pairs, reading what those blocks wrote with
+// agent-scope loads.  A wide loop stores its folds through to memory
+// (agent-scope stores): its later rounds rewrite entries that other blocks,
+// on other XCDs, wrote in earlier rounds, and with plain stores two L2s could
+// hold such a line dirty, the stale copy written back last.
+//
+// Nothing here relies on the order in which two stores become visible: every
+// word that crosses from one block to another, or between the device and the
+// host, carries the round it belongs to.  The host's challenge words carry a
+// round tag in their spare top bits; the message words and the wide loop's
+// partial words are 64-bit atomic stores of (stamp << 32 | word), and the
+// reader waits until every word it needs has its round's stamp.  So no fence
+// is needed (a device-scope fence writes back the L2, several microseconds on
+// this part), and an earlier value of a word can never pass for a later one.
 constexpr unsigned kTailThreads = 1024;
 constexpr uint32_t kTailRunning = 0, kTailDone = 1, kTailTimedOut = 2, kTailAborted = 3;
 constexpr uint32_t kTailMaxWidth = 9;  // degree <= 8
 // metadata limits of a state that runs its rounds in the loop (kept in LDS)
 constexpr uint32_t kTailMaxProducts = 32, kTailMaxEntries = 128, kTailMaxFactors = 64;
-// (distinct factor, pair) lines of one round kept in LDS (2 x 32 bytes each)
+// (distinct factor, pair) lines kept in LDS at once (2 x 32 bytes each); a
+// block takes its pairs in chunks of at most this many lines
 constexpr uint32_t kTailMaxLines = 768;
+// blocks of one wide loop (at most one per CU, so all are resident)
+constexpr uint32_t kWideMaxBlocks = 256;
 
 struct TailJob {
   const Fr* const* tables[3];  // inputs, fold slot A, fold slot B (device pointer arrays)
@@ -921,35 +946,145 @@ struct TailJob {
   uint32_t challenge29[f29::FrP::L];  // the first round's challenge c as c 2^261 in Fr29 limbs
 };
 
+// The stages of a wide launch (one state): stage i runs rounds
+// [end[i - 1], end[i]) on 2^blocks_log2[i] blocks (end[2] is the state's
+// round count).  Words exchanged between blocks in round k carry the stamp
+// stamp_base + k + 1.  All zero: a one-block launch of several states.
+struct TailStage {
+  uint32_t end[3];
+  uint32_t blocks_log2[3];
+  uint32_t stamp_base, wide;
+};
+
+// The host's challenge for round r is the prescaled challenge's nine limb
+// words, each carrying tail_tag(r) in its top three bits (limbs are below
+// 2^29), so one load instruction per poll reads a whole challenge and knows
+// it is whole: a word is either the previous post's or the new one's, and
+// consecutive rounds have different tags.  Tag 0 marks cleared words.
+__host__ __device__ constexpr uint32_t tail_tag(uint32_t round) { return round % 7 + 1; }
+constexpr uint32_t kTailAbortWord = f29::FrP::L;
+
 struct TailMailbox {
   alignas(64) TailJob job;  // written by the host before the launch
-  // host -> device
-  alignas(64) uint32_t challenge_round;  // r: `challenge` holds round r's challenge
-  uint32_t abort;                        // 1: stop at the next wait
-  alignas(64) uint32_t challenge29[f29::FrP::L];  // c 2^261 in Fr29 limbs (prescaled by the host)
+  // host -> device, one 64-byte line: [0, 9) challenge limbs | tag << 29,
+  // [9] abort (1: stop at the next wait)
+  alignas(64) uint32_t post[16];
   // device -> host
-  alignas(64) uint32_t message_round;  // r + 1: `message` holds round r's message
-  uint32_t status;                     // kTailRunning / Done / TimedOut / Aborted
-  alignas(64) uint32_t message[kTailMaxWidth * 8];  // stored (Montgomery) form
+  alignas(64) uint32_t status;  // kTailRunning / Done / TimedOut / Aborted
+  // round r's message, stored (Montgomery) form: word i as (r + 1) << 32 | w
+  alignas(64) uint64_t message[kTailMaxWidth * 8];
+};
+
+// Device memory of a wide loop: the blocks' partial messages, word i of
+// block g's point t at (g kTailMaxWidth + t) 8 + i, and the challenge block 0
+// passes on to the other blocks (limb words 0..8, the stop code in word 9),
+// each word as stamp << 32 | w.
+struct WideArea {
+  uint64_t* part;   // kWideMaxBlocks x kTailMaxWidth x 8 words
+  uint64_t* bcast;  // 16 words
 };
 
 // Host-written words are read with relaxed system-scope loads (they bypass the
 // caches): an acquire would also invalidate the L2, and every table load of
-// the next round would then go to HBM.  The words after a flag are loaded
-// only once the flag's load has returned.
+// the next round would then go to HBM.
 __device__ __forceinline__ uint32_t sys_load_relaxed(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ void sys_store(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-__device__ __forceinline__ void sys_release(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+__device__ __forceinline__ void sys_store64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// Agent-scope relaxed atomics are coherent across the XCDs' L2 caches
+// without the L2 write-back or invalidation of a device-scope fence.
+__device__ __forceinline__ void agent_store64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t agent_load64(uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Entry i of a table that other blocks stored through to memory, read with
+// agent-scope loads (this L2 may hold an older copy of its line).
+__device__ __forceinline__ Fr29 load29_agent(const Fr* base, size_t i) {
+  uint64_t* p = reinterpret_cast<uint64_t*>(const_cast<Fr*>(base) + i);
+  Fr v;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint64_t x = agent_load64(p + q);
+    v.v[2 * q] = static_cast<uint32_t>(x);
+    v.v[2 * q + 1] = static_cast<uint32_t>(x >> 32);
+  }
+  return f29::unpack_fr(v);
 }
 
-// Block k of the launch runs the state of boxes[k] from job.first_round to
-// its last round (or until it stops).  Each round runs in three phases (one
-// workgroup, one CU):
+// A fold output of a wide loop, stored through to memory (agent-scope
+// stores leave no dirty line in this XCD's L2).  A wide loop's later rounds
+// rewrite entries that other blocks, on other XCDs, wrote in earlier rounds;
+// with plain stores both L2s could hold the line dirty, and the stale copy,
+// written back last, would overwrite the new one.
+__device__ __forceinline__ void store_fr_through(Fr* base, size_t i, const Fr& v) {
+  uint64_t* p = reinterpret_cast<uint64_t*>(base + i);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    agent_store64(p + q, static_cast<uint64_t>(v.v[2 * q]) | static_cast<uint64_t>(v.v[2 * q + 1]) << 32);
+}
+
+// Wavefront 0 of a block waits for round `round`'s challenge (bounded;
+// abort from the host); returns the stop code (kTailRunning: `c29` holds the
+// challenge's limbs).  Lanes 0..8 load the limb words and lane 9 the abort
+// word, one load instruction per poll.
+__device__ uint32_t tail_wait_challenge(TailMailbox* box, uint32_t round, uint64_t wait_ticks,
+                                        uint32_t spin_cap, uint32_t* c29) {
+  const uint32_t lane = threadIdx.x & 63, want = tail_tag(round);
+  const uint64_t start = static_cast<uint64_t>(wall_clock64());
+  for (uint32_t spin = 0;; ++spin) {
+    const uint32_t w = lane <= kTailAbortWord ? sys_load_relaxed(&box->post[lane]) : 0;
+    const bool ready = lane >= kTailAbortWord || (w >> 29) == want;
+    const uint32_t abort_word = __shfl(w, kTailAbortWord, 64);
+    if (abort_word != 0) return kTailAborted;
+    if (__ballot(ready) == ~0ull) {
+      if (lane < kTailAbortWord) c29[lane] = w & 0x1fffffffu;
+      return kTailRunning;
+    }
+    if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks)
+      return kTailTimedOut;
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// Wavefront 0 of every block of a loop: round r's challenge into c29, or the
+// stop code.  Only block 0 polls the host's mailbox (host memory read by
+// hundreds of blocks at once slowed every round to hundreds of
+// microseconds); in a wide loop it passes the challenge, or its stop code,
+// on through device memory, stamped with `stamp`, and the other blocks poll
+// that (bounded).
+__device__ uint32_t loop_wait(TailMailbox* box, uint32_t r, uint32_t g, uint32_t G, WideArea wide,
+                              uint32_t stamp, uint64_t wait_ticks, uint32_t spin_cap, uint32_t* c29) {
+  const uint32_t lane = threadIdx.x & 63;
+  if (g == 0) {
+    const uint32_t stop = tail_wait_challenge(box, r, wait_ticks, spin_cap, c29);
+    if (G > 1 && lane <= kTailAbortWord)
+      agent_store64(&wide.bcast[lane], static_cast<uint64_t>(stamp) << 32 |
+                                           (lane < kTailAbortWord ? c29[lane] : stop));
+    return stop;
+  }
+  const uint64_t start = static_cast<uint64_t>(wall_clock64());
+  for (uint32_t spin = 0;; ++spin) {
+    const uint64_t x = lane <= kTailAbortWord ? agent_load64(&wide.bcast[lane]) : 0;
+    const bool ready = lane > kTailAbortWord || (x >> 32) == stamp;
+    if (__ballot(ready) == ~0ull) {
+      if (lane < kTailAbortWord) c29[lane] = static_cast<uint32_t>(x);
+      return __shfl(static_cast<uint32_t>(x), kTailAbortWord, 64);
+    }
+    if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) return kTailTimedOut;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Block k of the launch (one state per block), or every block of a wide
+// launch (blocks_log2 = b > 0, one state), runs its rounds.  Each round runs,
+// chunk by chunk over the block's pairs, in three phases:
 //   1. thread per (distinct factor f, pair b) folds f on b with the previous
 //      challenge (or loads it in round 0), stores the fold for the next round
 //      and leaves the line's ends lo, hi in LDS: every factor is folded once
@@ -958,16 +1093,23 @@ __device__ __forceinline__ void sys_release(uint32_t* p, uint32_t v) {
 //      point (at least one; wavefront w takes point w / ws, w >= ws (d + 1)
 //      idle): the lanes of the ws wavefronts of t take its (product, pair)
 //      items, multiply the coefficient by the product's lines at t from LDS,
-//      and sum; pairs is a power of two, so an item splits by shifts;
-//   3. wavefront sums, then one thread per point adds its ws of them.
-// The fold writes of round k are read by other threads in round k + 1 after
-// the block barriers.
+//      and add to their running sums, kept across chunks; pairs is a power
+//      of two, so an item splits by shifts;
+//   3. after the last chunk, wavefront sums, then one thread per point adds
+//      its ws of them: the block's message (one block) or partial (wide).
+// A wide loop's blocks store their stamped partials; block 0 waits for all of
+// them (bounded), adds them and publishes the message.
+// The fold writes of round k are read by other threads of the same block in
+// round k + 1 after the block barriers.
 __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const* boxes,
                                                              uint64_t wait_ticks,
-                                                             uint32_t spin_cap) {
-  TailMailbox* box = boxes[blockIdx.x];
+                                                             uint32_t spin_cap,
+                                                             TailStage stage, WideArea wide) {
+  const bool staged = stage.wide != 0;
+  const uint32_t g = staged ? blockIdx.x : 0;
+  TailMailbox* box = boxes[staged ? 0 : blockIdx.x];
   __shared__ TailJob job;
-  __shared__ Fr29 s_c29;
+  __shared__ uint32_t s_c29[f29::FrP::L];
   __shared__ uint32_t s_stop;
   __shared__ Fr msg[kTailMaxWidth];
   __shared__ Fr part[kTailThreads / 64];  // wavefront sums
@@ -976,13 +1118,14 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const*
   __shared__ uint32_t s_slot[kTailMaxEntries];
   __shared__ Fr s_coeff[kTailMaxProducts];
   __shared__ const Fr* s_tab[3][kTailMaxFactors];
-  // line ends of every (distinct factor, pair) of the round, stored form
+  // line ends of every (distinct factor, pair) of a chunk, stored form
   __shared__ Fr s_lo[kTailMaxLines], s_hi[kTailMaxLines];
   if (threadIdx.x < sizeof(TailJob) / 4)
     reinterpret_cast<uint32_t*>(&job)[threadIdx.x] =
         reinterpret_cast<const uint32_t*>(&box->job)[threadIdx.x];
   __syncthreads();
-  if (threadIdx.x < f29::FrP::L) s_c29.l[threadIdx.x] = job.challenge29[threadIdx.x];
+  if (threadIdx.x < f29::FrP::L) s_c29[threadIdx.x] = job.challenge29[threadIdx.x];
+  if (threadIdx.x == 0) s_stop = kTailRunning;
   if (threadIdx.x <= job.products) s_off[threadIdx.x] = job.offsets[threadIdx.x];
   if (threadIdx.x < job.products) s_coeff[threadIdx.x] = load_fr(job.coeffs29, threadIdx.x);
   if (threadIdx.x < job.entries) s_slot[threadIdx.x] = job.slots[threadIdx.x];
@@ -991,58 +1134,89 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const*
   __syncthreads();
   const uint32_t n = job.num_vars, width = job.degree + 1, F = job.factors;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // pairs per chunk: the largest power of two whose F lines fit the LDS
+  const uint32_t chunk_cap = 31u - __clz(kTailMaxLines / F);
+  constexpr uint32_t kWaves = kTailThreads / 64;
   for (uint32_t k = job.first_round;; ++k) {
-    const Fr29 c29 = s_c29;
-    const uint32_t lp = n - k - 1;  // log2 of the round's pairs
-    const uint32_t pairs = 1u << lp;
+    Fr29 c29;
+#pragma unroll
+    for (int i = 0; i < f29::FrP::L; ++i) c29.l[i] = s_c29[i];
+    // this round's stage: blocks beyond its count are done
+    uint32_t si = 0;
+    if (staged)
+      while (k >= stage.end[si]) ++si;
+    const uint32_t blocks_log2 = stage.blocks_log2[si], G = 1u << blocks_log2;
+    if (g >= G) return;
+    // the first round of a later stage reads folds other blocks stored
+    const bool handed = si > 0 && k == stage.end[si - 1];
+    const uint32_t lb = n - k - 1 - blocks_log2;  // log2 of the block's pairs (the host keeps it >= 0)
+    const uint32_t lc = lb < chunk_cap ? lb : chunk_cap;  // log2 of a chunk's pairs
+    const uint32_t cpairs = 1u << lc, base = g << lb;
     const bool fold = k > 0;
     const Fr* const* src = k <= 1 ? s_tab[0] : s_tab[1 + ((k - 2) & 1)];
     const Fr* const* dst = k == 0 ? nullptr : s_tab[1 + ((k - 1) & 1)];
-    // 1. lines of every distinct factor on every pair
-    for (uint32_t u = threadIdx.x; u < (F << lp); u += kTailThreads) {
-      const uint32_t f = u >> lp, b = u & (pairs - 1);
-      const Fr* o = src[f];
-      if (fold) {
-        const Fr29 a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
-        const Fr29 a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
-        const Fr lo = f29::pack_fr_canonical(fold29(c29, a0, a1));
-        const Fr hi = f29::pack_fr_canonical(fold29(c29, a2, a3));
-        store_fr(const_cast<Fr*>(dst[f]), 2 * b, lo);
-        store_fr(const_cast<Fr*>(dst[f]), 2 * b + 1, hi);
-        s_lo[u] = lo;
-        s_hi[u] = hi;
-      } else {
-        s_lo[u] = load_fr(o, 2 * b);
-        s_hi[u] = load_fr(o, 2 * b + 1);
-      }
-    }
-    __syncthreads();  // also orders this round's fold writes before the next round's reads
-    // 2, 3. ws wavefronts per point: lazy Fr29 sums (each term < 1.3 r;
-    // reduced every 16 terms, below 24 r), then a butterfly over the lanes
-    // that hold items (Fr additions on the packed values)
-    const uint32_t items = job.products << lp;
-    constexpr uint32_t kWaves = kTailThreads / 64;
+    // phase 2's split: ws wavefronts per point over a chunk's items
+    const uint32_t items = job.products << lc;
     const uint32_t ws_fit = width >= kWaves ? 1u : kWaves / width, ws_need = (items + 63) / 64;
     const uint32_t ws = ws_fit < ws_need ? ws_fit : ws_need;  // >= 1: items >= 1
     const uint32_t held = items < 64 ? items : 64;  // lanes with items in a wavefront
-    for (uint32_t w = wave; w < ws * width; w += kWaves) {
-      const uint32_t t = w / ws, j = w - t * ws;
-      Fr29 acc = f29::zero_t<f29::FrP>();
-      uint32_t terms = 0;
-      for (uint32_t it = j * 64 + lane; it < items; it += ws * 64) {
-        const uint32_t p = it >> lp, b = it & (pairs - 1);
-        Fr29 term = f29::unpack_fr(s_coeff[p]);
-        for (uint32_t e = s_off[p]; e < s_off[p + 1]; ++e) {
-          const uint32_t u = (s_slot[e] << lp) | b;
-          const Fr29 lo = f29::unpack_fr(s_lo[u]);
-          const Fr29 d = f29::sub1p(f29::unpack_fr(s_hi[u]), lo);
-          Fr29 line = lo;  // lo + t (hi - lo) < 17 r
-          for (uint32_t q = 0; q < t; ++q) line = f29::add(line, d);
-          term = f29::mul(term, line);
+    const bool active = wave < ws * width;           // at most one (point, share) per wavefront
+    const uint32_t t = active ? wave / ws : 0, j = active ? wave - t * ws : 0;
+    // lazy Fr29 sums (each term < 1.3 r; reduced every 16 terms, below 24 r)
+    Fr29 acc = f29::zero_t<f29::FrP>();
+    uint32_t terms = 0;
+    for (uint32_t cb = base; cb < base + (1u << lb); cb += cpairs) {
+      // 1. lines of every distinct factor on every pair of the chunk
+      for (uint32_t u = threadIdx.x; u < (F << lc); u += kTailThreads) {
+        const uint32_t f = u >> lc, b = cb + (u & (cpairs - 1));
+        const Fr* o = src[f];
+        if (fold) {
+          Fr29 a0, a1, a2, a3;
+          if (handed) {
+            a0 = load29_agent(o, 4 * b), a1 = load29_agent(o, 4 * b + 1);
+            a2 = load29_agent(o, 4 * b + 2), a3 = load29_agent(o, 4 * b + 3);
+          } else {
+            a0 = load29(o, 4 * b), a1 = load29(o, 4 * b + 1);
+            a2 = load29(o, 4 * b + 2), a3 = load29(o, 4 * b + 3);
+          }
+          const Fr lo = f29::pack_fr_canonical(fold29(c29, a0, a1));
+          const Fr hi = f29::pack_fr_canonical(fold29(c29, a2, a3));
+          if (G > 1) {
+            store_fr_through(const_cast<Fr*>(dst[f]), 2 * b, lo);
+            store_fr_through(const_cast<Fr*>(dst[f]), 2 * b + 1, hi);
+          } else {
+            store_fr(const_cast<Fr*>(dst[f]), 2 * b, lo);
+            store_fr(const_cast<Fr*>(dst[f]), 2 * b + 1, hi);
+          }
+          s_lo[u] = lo;
+          s_hi[u] = hi;
+        } else {
+          s_lo[u] = load_fr(o, 2 * b);
+          s_hi[u] = load_fr(o, 2 * b + 1);
         }
-        acc = f29::add(acc, term);
-        if ((++terms & 15u) == 0) acc = f29::reduce_est(acc);
       }
+      __syncthreads();  // also orders this round's fold writes before the next round's reads
+      // 2. the chunk's items of this wavefront's point
+      if (active)
+        for (uint32_t it = j * 64 + lane; it < items; it += ws * 64) {
+          const uint32_t p = it >> lc, b = it & (cpairs - 1);
+          Fr29 term = f29::unpack_fr(s_coeff[p]);
+          for (uint32_t e = s_off[p]; e < s_off[p + 1]; ++e) {
+            const uint32_t u = (s_slot[e] << lc) | b;
+            const Fr29 lo = f29::unpack_fr(s_lo[u]);
+            const Fr29 d = f29::sub1p(f29::unpack_fr(s_hi[u]), lo);
+            Fr29 line = lo;  // lo + t (hi - lo) < 17 r
+            for (uint32_t q = 0; q < t; ++q) line = f29::add(line, d);
+            term = f29::mul(term, line);
+          }
+          acc = f29::add(acc, term);
+          if ((++terms & 15u) == 0) acc = f29::reduce_est(acc);
+        }
+      __syncthreads();  // the next chunk overwrites the lines
+    }
+    // 3. a butterfly over the lanes that hold items (Fr additions on the
+    // packed values), then one thread per point
+    if (active) {
       Fr v = f29::pack_fr_reduced(f29::reduce_est(acc));
 #pragma unroll
       for (uint32_t off = 32; off > 0; off >>= 1) {
@@ -1053,54 +1227,102 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const*
           v = v + o;
         }
       }
-      if (lane == 0) part[w] = v;
+      if (lane == 0) part[wave] = v;
     }
     __syncthreads();
     if (threadIdx.x < width) {
       Fr sum = part[threadIdx.x * ws];
-      for (uint32_t j = 1; j < ws; ++j) sum = sum + part[threadIdx.x * ws + j];
+      for (uint32_t q = 1; q < ws; ++q) sum = sum + part[threadIdx.x * ws + q];
       msg[threadIdx.x] = sum;  // stored form: the host converts to canonical
     }
     __syncthreads();
-    const bool last = k + 1 == n;
-    // wave 0 writes the message words (one store instruction per 64 words),
-    // then its lane 0 publishes the round: the release waits for the wave's
-    // own stores
-    if (threadIdx.x < 64) {
-      for (uint32_t i = threadIdx.x; i < width * 8; i += 64)
-        sys_store(&box->message[i], msg[i / 8].v[i % 8]);
-      if (threadIdx.x == 0) {
-        if (last) sys_store(&box->status, kTailDone);
-        sys_release(&box->message_round, k + 1);
-      }
-    }
-    if (last) return;
-    // the challenge of round k + 1 (bounded wait; abort from the host):
-    // thread 0 polls the round word, then lanes 0..8 of its wavefront read
-    // the nine words of the prescaled challenge with one load instruction
-    // (one round trip to host memory instead of nine in a row)
-    if (threadIdx.x < 64) {
-      uint32_t stop = kTailRunning;
-      if (threadIdx.x == 0) {
+    if (G > 1) {
+      // the block's partial, stamped; block 0 adds the G partials of each
+      // point: thread q waits for block q's words (bounded), a butterfly per
+      // wavefront, then one thread per point adds the wavefront sums (kept in
+      // s_lo, free after the chunks)
+      const uint64_t stamp = static_cast<uint64_t>(stage.stamp_base + k + 1) << 32;
+      if (threadIdx.x < 64)
+        for (uint32_t i = threadIdx.x; i < width * 8; i += 64)
+          agent_store64(&wide.part[(g * kTailMaxWidth + i / 8) * 8 + i % 8], stamp | msg[i / 8].v[i % 8]);
+      if (g == 0) {
+        // item j = t G + q (block q's partial at point t), 1024 items a pass:
+        // each thread loads its item's eight words at once (waiting only for
+        // words not yet stamped), then a butterfly over segments of
+        // min(G, 64) lanes (one point each) and one LDS slot per segment
+        bool late = false;
         const uint64_t start = static_cast<uint64_t>(wall_clock64());
-        for (uint32_t spin = 0;; ++spin) {
-          if (sys_load_relaxed(&box->abort) != 0) {
-            stop = kTailAborted;
-            break;
+        const uint32_t seg = G < 64 ? G : 64, nseg = G < 64 ? 1u : G / 64;
+        for (uint32_t j0 = 0; j0 < width * G; j0 += kTailThreads) {
+          const uint32_t j = j0 + threadIdx.x;
+          const uint32_t tt = j >> blocks_log2, q = j & (G - 1);
+          Fr v = Fr::zero();
+          if (j < width * G) {
+            uint64_t* w = &wide.part[(q * kTailMaxWidth + tt) * 8];
+            uint64_t x[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) x[i] = agent_load64(w + i);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+              for (uint32_t spin = 0; (x[i] & ~0xffffffffull) != stamp; ++spin) {
+                if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
+                  late = true;
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                x[i] = agent_load64(w + i);
+              }
+              v.v[i] = static_cast<uint32_t>(x[i]);
+            }
           }
-          if (sys_load_relaxed(&box->challenge_round) == k + 1) break;
-          if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
-            stop = kTailTimedOut;
-            break;
+#pragma unroll
+          for (uint32_t off = 32; off > 0; off >>= 1) {
+            if (off < seg) {  // uniform
+              Fr o;
+#pragma unroll
+              for (int i = 0; i < 8; ++i) o.v[i] = __shfl_xor(v.v[i], off, 64);
+              v = v + o;
+            }
           }
-          __builtin_amdgcn_s_sleep(2);
+          if (j < width * G && (q & (seg - 1)) == 0) s_lo[tt * nseg + q / 64] = v;
+        }
+        if (late) s_stop = kTailTimedOut;  // a block never delivered
+        __syncthreads();
+        if (threadIdx.x < width) {
+          Fr sum = s_lo[threadIdx.x * nseg];
+          for (uint32_t q = 1; q < nseg; ++q) sum = sum + s_lo[threadIdx.x * nseg + q];
+          msg[threadIdx.x] = sum;
+        }
+        __syncthreads();
+        if (s_stop != kTailRunning) {
+          if (threadIdx.x == 0) sys_store(&box->status, kTailTimedOut);
+          return;
         }
       }
-      stop = __shfl(stop, 0, 64);
-      if (stop == kTailRunning && threadIdx.x < f29::FrP::L)
-        s_c29.l[threadIdx.x] = sys_load_relaxed(&box->challenge29[threadIdx.x]);
+    }
+    const bool last_round = k + 1 == n;
+    // the block holding the message: wavefront 0 writes its stamped words
+    // (64-bit atomic stores); the host takes them once all carry round k's
+    // stamp
+    if (g == 0 && threadIdx.x < 64) {
+      const uint64_t stamp = static_cast<uint64_t>(k + 1) << 32;
+      if (last_round && threadIdx.x == 0) sys_store(&box->status, kTailDone);
+      for (uint32_t i = threadIdx.x; i < width * 8; i += 64)
+        sys_store64(&box->message[i], stamp | msg[i / 8].v[i % 8]);
+    }
+    if (last_round) return;
+    // blocks without pairs in round k + 1 are done
+    uint32_t next_si = si;
+    if (staged)
+      while (k + 1 >= stage.end[next_si]) ++next_si;
+    const uint32_t next_G = 1u << stage.blocks_log2[next_si];
+    if (g >= next_G) return;
+    // the challenge of round k + 1
+    if (threadIdx.x < 64) {
+      const uint32_t stop = loop_wait(box, k + 1, g, next_G, wide, stage.stamp_base + k + 2, wait_ticks,
+                                      spin_cap, s_c29);
       if (threadIdx.x == 0) {
-        if (stop != kTailRunning) sys_release(&box->status, stop);
+        if (stop != kTailRunning) sys_store(&box->status, stop);
         s_stop = stop;
       }
     }
@@ -1445,6 +1667,13 @@ struct TailPool {
   sc::TailMailbox** dptrs = nullptr;
   size_t ptrs_cap = 0;
   uint64_t wait_ticks = 0;
+  // wide loops: the partial words (device memory, allocated and zeroed at the
+  // first wide launch) and the stamp base of the next launch (+64 a launch:
+  // a launch covers at most 32 rounds, so no stamp repeats until the counter
+  // wraps, when the words are zeroed again)
+  void* wide_mem = nullptr;
+  uint32_t wide_stamp = 64;
+  uint32_t cus_log2 = 0;  // a wide loop's blocks must all be resident: at most one per CU
   static constexpr size_t kChunk = 64;
 };
 
@@ -1466,6 +1695,44 @@ uint64_t tail_wait_us() {
 // n = 20, degree 2, rounds 12-19 run in the loop at ~9 us each against
 // 21-29 us per launched round, the whole sumcheck 0.97 -> 0.88 ms); 0 turns
 // the loop off.
+// A single state whose rounds do not fit one block yet runs them in a wide
+// loop (one block per CU, then fewer) once a round has at most this many
+// (product, pair, point) items: HP_MI355X_SC_WIDE_ITEMS (0 turns it off), with
+// at most HP_MI355X_SC_WIDE_BLOCKS blocks (a power of two, <= 256).  Measured
+// at n = 20: at 2^19 the degree-2 loop starts at round 4 (round 3 in the loop
+// 71 us against 61 us launched), the degree-5 one at round 6.
+size_t wide_items() {
+  static const size_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_WIDE_ITEMS");
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(1) << 19;
+  }();
+  return v;
+}
+
+// A wide loop hands its rounds to one block once a round's factor products
+// (pairs x points x the products' total arity) are at most this many,
+// HP_MI355X_SC_WIDE_TAIL_WORK: one CU takes about 4 us per 1000 of them, a
+// wide round costs ~9 us more than a one-block round in exchanges.
+size_t wide_tail_work() {
+  static const size_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_WIDE_TAIL_WORK");
+    return e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : size_t(2048);
+  }();
+  return v;
+}
+
+uint32_t wide_blocks_log2() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("HP_MI355X_SC_WIDE_BLOCKS");
+    size_t b = e ? static_cast<size_t>(std::strtoull(e, nullptr, 0)) : sc::kWideMaxBlocks;
+    b = std::min<size_t>(std::max<size_t>(b, 2), sc::kWideMaxBlocks);
+    uint32_t l = 0;
+    while ((size_t(2) << l) <= b) ++l;
+    return l;
+  }();
+  return v;
+}
+
 size_t tail_items(bool batch) {
   static const size_t batch_v = [] {
     const char* e = std::getenv("HP_MI355X_SC_TAIL_ITEMS");
@@ -1480,20 +1747,33 @@ size_t tail_items(bool batch) {
 
 constexpr size_t kTailMaxStates = 128;
 
-bool tail_ok(const sc::State& s, bool batch) {
-  if (s.max_degree > kMaxJobDegree || s.product_count == 0 || s.round >= s.num_vars) return false;
-  if (s.product_count > sc::kTailMaxProducts || s.index_count > sc::kTailMaxEntries ||
-      s.distinct > sc::kTailMaxFactors)
-    return false;
-  const size_t pairs = size_t(1) << (s.num_vars - s.round - 1);
+// the state's metadata fits a loop block's LDS
+bool loop_meta_ok(const sc::State& s) {
+  return s.max_degree <= kMaxJobDegree && s.product_count > 0 && s.product_count <= sc::kTailMaxProducts &&
+         s.index_count <= sc::kTailMaxEntries && s.distinct <= sc::kTailMaxFactors;
+}
+
+// round k of the state fits the one-block loop
+bool tail_ok_at(const sc::State& s, size_t k, bool batch) {
+  if (!loop_meta_ok(s) || k >= s.num_vars) return false;
+  const size_t pairs = size_t(1) << (s.num_vars - k - 1);
   if (s.distinct * pairs > sc::kTailMaxLines) return false;
   return pairs * s.product_count * (s.max_degree + 1) <= tail_items(batch);
+}
+
+bool tail_ok(const sc::State& s, bool batch) { return tail_ok_at(s, s.round, batch); }
+
+// the state's current round starts (or continues) a wide loop
+bool wide_ok(const sc::State& s) {
+  if (!loop_meta_ok(s) || s.round >= s.num_vars || wide_items() == 0) return false;
+  const size_t pairs = size_t(1) << (s.num_vars - s.round - 1);
+  return pairs * s.product_count * (s.max_degree + 1) <= wide_items();
 }
 
 void quiesce_tails(rt::Context& c) {
   auto* pool = static_cast<TailPool*>(c.tails);
   if (pool == nullptr || pool->active.empty()) return;
-  for (sc::State* s : pool->active) __atomic_store_n(&s->box->abort, 1u, __ATOMIC_RELEASE);
+  for (sc::State* s : pool->active) __atomic_store_n(&s->box->post[sc::kTailAbortWord], 1u, __ATOMIC_RELEASE);
   for (sc::State* s : pool->active) s->tail_active = false;
   pool->active.clear();
   rt::hip_check(hipStreamSynchronize(c.stream));  // every block has seen its abort and exited
@@ -1504,6 +1784,7 @@ void release_tail_pool(rt::Context& c) {
   if (pool == nullptr) return;
   for (sc::TailMailbox* m : pool->chunks) (void)hipHostFree(m);
   if (pool->ptrs) (void)hipHostFree(pool->ptrs);
+  if (pool->wide_mem) (void)hipFree(pool->wide_mem);
   delete pool;
   c.tails = nullptr;
   c.quiesce = nullptr;
@@ -1519,6 +1800,12 @@ TailPool& tail_pool(rt::Context& c) {
       khz = 100000;  // 100 MHz
     }
     pool->wait_ticks = tail_wait_us() * static_cast<uint64_t>(khz) / 1000;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus <= 0) {
+      (void)hipGetLastError();
+      cus = 2;
+    }
+    while ((2 << pool->cus_log2) <= cus) ++pool->cus_log2;
     c.tails = pool;
     c.quiesce = &quiesce_tails;
     c.release_tails = &release_tail_pool;
@@ -1551,15 +1838,12 @@ void tail_return_mailbox(rt::Context& c, sc::State& s) {
   s.box = s.dbox = nullptr;
 }
 
-// One launch, one block per state, from each state's current round with its
-// challenge (stored form; ignored at round 0).  Precondition: no active loop.
-void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>& states,
-                  const std::vector<Fr>& challenges) {
-  if (pool.ptrs_cap < states.size()) {
+void tail_ptrs(TailPool& pool, size_t count) {
+  if (pool.ptrs_cap < count) {
     if (pool.ptrs) rt::hip_check(hipHostFree(pool.ptrs));
     pool.ptrs = nullptr;
     pool.ptrs_cap = 0;
-    const size_t cap = std::max<size_t>(states.size(), 64);
+    const size_t cap = std::max<size_t>(count, 64);
     void* raw = nullptr;
     rt::hip_check(hipHostMalloc(&raw, cap * sizeof(void*), hipHostMallocCoherent | hipHostMallocMapped));
     pool.ptrs = static_cast<sc::TailMailbox**>(raw);
@@ -1568,33 +1852,46 @@ void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>&
     pool.dptrs = static_cast<sc::TailMailbox**>(draw);
     pool.ptrs_cap = cap;
   }
+}
+
+// The state's loop job from its current round with the challenge (stored
+// form; ignored at round 0); the mailbox cleared for a new launch.
+void tail_job(TailPool& pool, sc::State& s, const Fr& challenge) {
+  tail_mailbox(pool, s);
+  sc::TailJob& j = s.box->job;
+  for (int t = 0; t < 3; ++t) j.tables[t] = s.table_ptrs[t];
+  j.coeffs29 = s.coeffs29;
+  j.offsets = s.prods.offsets;
+  j.slots = s.prods.slots;
+  j.owner = s.owner;
+  j.products = static_cast<uint32_t>(s.product_count);
+  j.degree = static_cast<uint32_t>(s.max_degree);
+  j.num_vars = static_cast<uint32_t>(s.num_vars);
+  j.first_round = static_cast<uint32_t>(s.round);
+  j.entries = static_cast<uint32_t>(s.index_count);
+  j.factors = static_cast<uint32_t>(s.distinct);
+  const f29::Fr29 c29 = f29::prescale_fr(challenge);
+  std::memcpy(j.challenge29, c29.l, sizeof j.challenge29);
+  for (uint32_t& w : s.box->post) w = 0;  // no challenge (tag 0), no abort
+  for (uint64_t& w : s.box->message) w = 0;  // no round's stamp
+  s.box->status = sc::kTailRunning;
+}
+
+constexpr uint32_t kTailSpinCap = 50000000u;
+
+// One launch, one block per state, from each state's current round with its
+// challenge.  Precondition: no active loop.
+void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>& states,
+                  const std::vector<Fr>& challenges) {
+  tail_ptrs(pool, states.size());
   for (size_t i = 0; i < states.size(); ++i) {
-    sc::State& s = *states[i];
-    tail_mailbox(pool, s);
-    sc::TailJob& j = s.box->job;
-    for (int t = 0; t < 3; ++t) j.tables[t] = s.table_ptrs[t];
-    j.coeffs29 = s.coeffs29;
-    j.offsets = s.prods.offsets;
-    j.slots = s.prods.slots;
-    j.owner = s.owner;
-    j.products = static_cast<uint32_t>(s.product_count);
-    j.degree = static_cast<uint32_t>(s.max_degree);
-    j.num_vars = static_cast<uint32_t>(s.num_vars);
-    j.first_round = static_cast<uint32_t>(s.round);
-    j.entries = static_cast<uint32_t>(s.index_count);
-    j.factors = static_cast<uint32_t>(s.distinct);
-    const f29::Fr29 c29 = f29::prescale_fr(challenges[i]);
-    std::memcpy(j.challenge29, c29.l, sizeof j.challenge29);
-    s.box->challenge_round = 0;
-    s.box->abort = 0;
-    s.box->message_round = 0;
-    s.box->status = sc::kTailRunning;
-    pool.ptrs[i] = s.dbox;
+    tail_job(pool, *states[i], challenges[i]);
+    pool.ptrs[i] = states[i]->dbox;
   }
   __atomic_thread_fence(__ATOMIC_SEQ_CST);
-  const uint32_t spin_cap = 50000000u;
   hipLaunchKernelGGL(sc::k_round_tail, dim3(static_cast<unsigned>(states.size())), dim3(sc::kTailThreads),
-                     0, c.stream, pool.dptrs, pool.wait_ticks, spin_cap);
+                     0, c.stream, pool.dptrs, pool.wait_ticks, kTailSpinCap, sc::TailStage{},
+                     sc::WideArea{nullptr, nullptr});
   rt::check_launch();
   for (sc::State* s : states) {
     s->tail_active = true;
@@ -1602,23 +1899,93 @@ void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>&
   }
 }
 
+// A wide launch of one state from its current round, in up to three stages:
+// G blocks (the most the round's pairs, the CUs and the maximum allow) while
+// every block keeps a pair, then, if that stops short of the one-block loop,
+// as many blocks as the round before the one-block loop has pairs, then one
+// block.  Precondition: no active loop, wide_ok(s), !tail_ok(s).
+void launch_wide(rt::Context& c, TailPool& pool, sc::State& s, const Fr& challenge) {
+  const size_t n = s.num_vars;
+  size_t tail_start = s.round;  // n: the one-block loop is switched off
+  auto work = [&](size_t k) { return (size_t(1) << (n - k - 1)) * s.index_count * (s.max_degree + 1); };
+  while (tail_start < n && !(tail_ok_at(s, tail_start, false) && work(tail_start) <= wide_tail_work()))
+    ++tail_start;
+  struct Stage {
+    uint32_t first, end, blocks_log2;
+  };
+  Stage st[3];
+  int count = 0;
+  const uint32_t b1 = std::min<uint32_t>(std::min(wide_blocks_log2(), pool.cus_log2),
+                                         static_cast<uint32_t>(n - s.round - 1));
+  const size_t end1 = std::min<size_t>(tail_start, n - b1);  // round k has 2^(n-k-1) pairs
+  st[count++] = {static_cast<uint32_t>(s.round), static_cast<uint32_t>(end1), b1};
+  if (end1 < tail_start)
+    st[count++] = {static_cast<uint32_t>(end1), static_cast<uint32_t>(tail_start),
+                   static_cast<uint32_t>(n - tail_start)};
+  if (tail_start < n) st[count++] = {static_cast<uint32_t>(tail_start), static_cast<uint32_t>(n), 0};
+  const size_t part_bytes = (size_t(sc::kWideMaxBlocks) * sc::kTailMaxWidth * 8 + 16) * sizeof(uint64_t);
+  bool zero = false;
+  if (pool.wide_mem == nullptr) {
+    rt::hip_check(hipMalloc(&pool.wide_mem, part_bytes));
+    zero = true;
+  }
+  if (pool.wide_stamp > UINT32_MAX - 64) {  // the stamps would wrap: start again from zeroed words
+    pool.wide_stamp = 64;
+    zero = true;
+  }
+  if (zero) rt::hip_check(hipMemsetAsync(pool.wide_mem, 0, part_bytes, c.stream));
+  const uint32_t stamp_base = pool.wide_stamp;
+  pool.wide_stamp += 64;
+  const sc::WideArea area{static_cast<uint64_t*>(pool.wide_mem),
+                          static_cast<uint64_t*>(pool.wide_mem) + size_t(sc::kWideMaxBlocks) * sc::kTailMaxWidth * 8};
+  tail_ptrs(pool, 1);
+  tail_job(pool, s, challenge);
+  pool.ptrs[0] = s.dbox;
+  sc::TailStage stage{};
+  for (int i = 0; i < 3; ++i) {  // unused stages are empty: [n, n)
+    stage.end[i] = i < count ? st[i].end : static_cast<uint32_t>(n);
+    stage.blocks_log2[i] = i < count ? st[i].blocks_log2 : 0;
+  }
+  stage.stamp_base = stamp_base;
+  stage.wide = 1;
+  __atomic_thread_fence(__ATOMIC_SEQ_CST);
+  hipLaunchKernelGGL(sc::k_round_tail, dim3(1u << st[0].blocks_log2), dim3(sc::kTailThreads), 0, c.stream,
+                     pool.dptrs, pool.wait_ticks, kTailSpinCap, stage, area);
+  rt::check_launch();
+  s.tail_active = true;
+  pool.active.push_back(&s);
+}
+
 void tail_post(sc::State& s, const Fr& challenge) {
   const f29::Fr29 c29 = f29::prescale_fr(challenge);
-  std::memcpy(s.box->challenge29, c29.l, sizeof s.box->challenge29);
-  __atomic_store_n(&s.box->challenge_round, static_cast<uint32_t>(s.round), __ATOMIC_RELEASE);
+  const uint32_t tag = sc::tail_tag(static_cast<uint32_t>(s.round)) << 29;
+  for (int i = 0; i < f29::FrP::L; ++i) __atomic_store_n(&s.box->post[i], c29.l[i] | tag, __ATOMIC_RELEASE);
 }
 
 // Wait for the message of the state's current round: true with it in `out`
 // (width canonical Fr), false if the block stopped first (wait limit or abort).
+// A word is taken once it carries the round's stamp; it is not written again
+// before the next challenge is posted.
 bool tail_wait(rt::Context& c, sc::State& s, uint8_t* out) {
-  const uint32_t want = static_cast<uint32_t>(s.round) + 1;
+  const uint64_t want = static_cast<uint64_t>(s.round + 1) << 32;
+  const size_t words = (s.max_degree + 1) * 8;
+  uint32_t w[sc::kTailMaxWidth * 8];
+  size_t got = 0;
+  auto take = [&] {
+    while (got < words) {
+      const uint64_t x = __atomic_load_n(&s.box->message[got], __ATOMIC_ACQUIRE);
+      if ((x & ~uint64_t(0xffffffff)) != want) return false;
+      w[got++] = static_cast<uint32_t>(x);
+    }
+    return true;
+  };
   const auto start = std::chrono::steady_clock::now();
   for (uint32_t spin = 0;; ++spin) {
-    if (__atomic_load_n(&s.box->message_round, __ATOMIC_ACQUIRE) == want) break;
+    if (take()) break;
     const uint32_t st = __atomic_load_n(&s.box->status, __ATOMIC_ACQUIRE);
     if (st == sc::kTailTimedOut || st == sc::kTailAborted) {
       // the block may have published this round just before stopping
-      if (__atomic_load_n(&s.box->message_round, __ATOMIC_ACQUIRE) == want) break;
+      if (take()) break;
       return false;
     }
     if ((spin & 1023u) == 1023u) {
@@ -1635,7 +2002,7 @@ bool tail_wait(rt::Context& c, sc::State& s, uint8_t* out) {
   // stored (Montgomery) form -> canonical
   for (size_t t = 0; t <= s.max_degree; ++t) {
     Fr v;
-    std::memcpy(v.v, s.box->message + 8 * t, rt::kFrBytes);
+    std::memcpy(v.v, w + 8 * t, rt::kFrBytes);
     v = from_mont(v);
     std::memcpy(out + rt::kFrBytes * t, v.v, rt::kFrBytes);
   }
@@ -1651,7 +2018,8 @@ void tail_advance(TailPool& pool, sc::State& s) {
   }
 }
 
-// Rounds of `states` (all tail_ok) through the loop: post or launch, wait.
+// Rounds of `states` (all tail_ok, or one wide_ok state) through the loop:
+// post or launch, wait.
 // Returns false (nothing advanced, no loop active) when a block stopped
 // before answering twice in a row; the caller then uses the launch path.
 bool tail_rounds(rt::Context& c, const std::vector<sc::State*>& states, const std::vector<Fr>& chs,
@@ -1662,7 +2030,10 @@ bool tail_rounds(rt::Context& c, const std::vector<sc::State*>& states, const st
   for (int attempt = 0; attempt < 2; ++attempt) {
     if (launch) {
       quiesce_tails(c);
-      launch_tails(c, pool, states, chs);
+      if (states.size() == 1 && !tail_ok(*states[0], false))
+        launch_wide(c, pool, *states[0], chs[0]);
+      else
+        launch_tails(c, pool, states, chs);
     } else {
       for (size_t i = 0; i < states.size(); ++i) tail_post(*states[i], chs[i]);
       // the runtime's error state, checked where a launched round checks its
@@ -2086,7 +2457,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
     try {
       const size_t width = s.max_degree + 1;
       const Fr ch = challenge_of(s, challenge_or_null);
-      if (tail_ok(s, false) && tail_rounds(c, {&s}, {ch}, {out_evaluations})) return rt::ok();
+      if ((tail_ok(s, false) || wide_ok(s)) && tail_rounds(c, {&s}, {ch}, {out_evaluations}))
+        return rt::ok();
       quiesce_tails(c);
       if (s.max_degree <= kMaxJobDegree && round_graphs_enabled()) {
         std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr, true)};

@@ -195,10 +195,11 @@ static int sumcheck_once(HpMi355xContext *ctx, HpMi355xDeviceAddress *f, int aud
       struct Counts c = end(what);
       /* launched rounds: challenge by value in the kernel arguments, message
        * written by the kernel into pinned host memory, one synchronisation;
-       * rounds in the persistent loop: challenge and message through the
-       * state's mailbox in pinned host memory, no synchronisation, the loop
-       * launched once.  Neither copies nor allocates. */
-      expect_online(&c, 0, 0, 2, what);
+       * rounds in the persistent loops: challenge and message through the
+       * state's mailbox in pinned host memory, no synchronisation, the loops
+       * launched once (a wide loop, one of fewer blocks, the one-block loop,
+       * queued together).  Neither copies nor allocates. */
+      expect_online(&c, 0, 0, 3, what);
       require(c.stream_syncs + c.event_syncs <= 1, "a round waits for its message at most once");
     }
   }

@@ -2,12 +2,13 @@
 batched MSM against the oracle, byte for byte.
 
 * the pair-per-thread kernels for D = 1..8 (processes with
-  HP_MI355X_SC_ITEMS_BELOW=0, so every round runs on them;
-  tests/sumcheck_pairs_check.py): k_round_classes<D> by default and
-  k_round_pairs<D> with HP_MI355X_SC_CLASSES=0, and the 14-table
-  degree-5 jellyfish-style gate of the n = 20 workload at num_vars 16 on the
-  default thresholds (round 0 on the pairs kernel, then the item and point
-  kernels), each round against oracle.SumCheckState
+  HP_MI355X_SC_ITEMS_BELOW=0 and the round loops off, so every round runs on
+  them; tests/sumcheck_pairs_check.py): k_round_classes<D> by default and
+  k_round_pairs<D> with HP_MI355X_SC_CLASSES=0; the same cases through the
+  persistent round loops (wide loops of 256, 16 and 2 blocks, then fewer
+  blocks, then one); and the 14-table degree-5 jellyfish-style gate of the
+  n = 20 workload at num_vars 16 on the default thresholds (round 0 on the
+  pairs kernel, then the loops), each round against oracle.SumCheckState
   (backend/src/cpu/sum_check.rs:74-180);
 * multilinear_kzg_batch and multi_commit of mixed sizes (num_vars 0..16, with
   repeats) entry by entry against the oracle's Pippenger and KZG open
@@ -36,7 +37,21 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 @pytest.mark.parametrize("classes", ["1", "0"])
 def test_pairs_kernel_every_degree_matches_oracle(classes):
-    env = dict(os.environ, HP_MI355X_SC_ITEMS_BELOW="0", HP_MI355X_SC_CLASSES=classes)
+    # round loops off: every round is a launched pair-per-thread round
+    env = dict(os.environ, HP_MI355X_SC_ITEMS_BELOW="0", HP_MI355X_SC_CLASSES=classes,
+               HP_MI355X_SC_WIDE_ITEMS="0", HP_MI355X_SC_TAIL_SINGLE_ITEMS="0")
+    res = subprocess.run([sys.executable, "-u", os.path.join(ROOT, "tests", "sumcheck_pairs_check.py")],
+                         capture_output=True, text=True, timeout=300, env=env, cwd=ROOT)
+    print(res.stdout[-3000:])
+    assert res.returncode == 0 and "PAIRS PASS" in res.stdout, res.stdout[-3000:] + res.stderr[-3000:]
+
+
+@pytest.mark.parametrize("blocks", ["256", "16", "2"])
+def test_round_loops_every_degree_match_oracle(blocks):
+    """The same cases through the persistent round loops (the defaults: a wide
+    loop of up to `blocks` blocks from round 0, then fewer blocks, then one):
+    with 2 blocks every block takes its pairs in many LDS chunks."""
+    env = dict(os.environ, HP_MI355X_SC_WIDE_BLOCKS=blocks)
     res = subprocess.run([sys.executable, "-u", os.path.join(ROOT, "tests", "sumcheck_pairs_check.py")],
                          capture_output=True, text=True, timeout=300, env=env, cwd=ROOT)
     print(res.stdout[-3000:])

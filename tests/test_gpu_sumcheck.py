@@ -348,11 +348,15 @@ def test_round_graph_replay_gives_same_messages(degree):
 @pytest.mark.parametrize("degree", ["2", "5"])
 def test_round_loops_give_the_same_messages(degree):
     """The persistent round loops change no message: whole n = 12 sumchecks
-    of one state with the loops on for rounds of at most 16384 items (the
-    default), off, on for rounds of at most 4096 items, taking every round
-    whose lines fit the block (a large item limit), and with a zero device
-    wait limit (every block stops right after its message, so each round
-    relaunches the loop from the state's round)."""
+    of one state with the defaults (a wide loop from round 0, then fewer
+    blocks, then the one-block loop for rounds of at most 16384 items), every
+    loop off, the wide loop off, the one-block loop off (a one-block stage
+    that takes its pairs in chunks), wide loops of 2 blocks (many chunks per
+    block), the one-block loop for rounds of at most 4096 items or for every
+    round whose lines fit the block (a large item limit), and with a zero
+    device wait limit (every block stops right after its message, so each
+    round relaunches the loop from the state's round and a wide launch finds
+    its ticket dirty)."""
     import json
     import os
     import subprocess
@@ -360,8 +364,12 @@ def test_round_loops_give_the_same_messages(degree):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     prog = os.path.join(root, "_build", "prove_bench")
     digests = []
-    for env_extra in ({}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "0"}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096"},
-                      {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "100000000"},
+    for env_extra in ({}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "0", "HP_MI355X_SC_WIDE_ITEMS": "0"},
+                      {"HP_MI355X_SC_WIDE_ITEMS": "0"}, {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "0"},
+                      {"HP_MI355X_SC_WIDE_BLOCKS": "2"},
+                      {"HP_MI355X_SC_WIDE_ITEMS": "0", "HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096"},
+                      {"HP_MI355X_SC_WIDE_ITEMS": "0", "HP_MI355X_SC_TAIL_SINGLE_ITEMS": "100000000"},
+                      {"HP_MI355X_SC_TAIL_WAIT_US": "0"},
                       {"HP_MI355X_SC_TAIL_SINGLE_ITEMS": "4096", "HP_MI355X_SC_TAIL_WAIT_US": "0"}):
         env = dict(os.environ, **env_extra)
         res = subprocess.run([prog, "sumcheck", "12", degree, "2", "3"], capture_output=True, text=True,
