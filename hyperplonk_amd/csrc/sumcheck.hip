@@ -127,6 +127,10 @@ struct RoundJob {
   const Fr* class_k;
   uint32_t class_count, class_width;
   uint32_t entries;  // flattened factor entries (offsets[products])
+  // class finish: the (class, t) sums as stamped words (kMaxClassWidth x 8,
+  // zeroed when the state is created) and this round's stamp (round + 1)
+  uint64_t* class_sums;
+  uint32_t class_stamp;
 };
 
 constexpr int kJobsPerLaunch = 16;
@@ -504,40 +508,89 @@ __global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pac
   }
 }
 
-// Second phase of a class launch: block j adds the block partials of job j
-// per (class, t), multiplies by the class coefficients, extends every class
-// from t = 0..m to t = 0..d and writes the canonical message.  Extension by
-// the trailing difference diagonal e_j = nabla^j f(m): nabla^m f is constant,
-// and f(t + 1) follows from e_j += e_(j+1) for j = m - 1 .. 0.
+// Second phase of a class launch, block (i, j) per job j and (class, t) sum
+// i: the block adds value i of every block partial of job j (four rows a
+// thread, loaded at once), stores the sum as stamped 64-bit words and takes the
+// job's ticket; the job's last block waits until every sum carries the stamp
+// (agent-scope loads; no fence, whose L2 write-back costs more than the whole
+// finish), multiplies each class's m + 1 sums by its coefficient, extends the
+// class from t = 0..m to t = 0..d and writes the canonical message.  Extension
+// by the trailing difference diagonal e_j = nabla^j f(m): nabla^m f is
+// constant, and f(t + 1) follows from e_j += e_(j+1) for j = m - 1 .. 0.  A
+// single block reading all partials took 13-40 us a round (up to ~100 partial
+// loads in a row per thread); this takes a few.
 __global__ void __launch_bounds__(kThreads) k_finish_classes(const JobPack pack) {
-  const RoundJob& job = pack.job[blockIdx.x];
-  const uint32_t width = job.class_width, groups = kThreads / width;
-  const uint32_t i = threadIdx.x % width, g = threadIdx.x / width;
+  const RoundJob& job = pack.job[blockIdx.y];
+  const uint32_t width = job.class_width, i = blockIdx.x;
+  if (i >= width) return;
   Fr s = Fr::zero();
-  if (g < groups) {
-    // eight independent loads in flight per step, added as a tree
-    constexpr uint32_t kU = 8;
-    for (uint32_t blk = g; blk < job.blocks; blk += kU * groups) {
-      Fr v[kU];
+  {
+    constexpr uint32_t kU = 4;  // kMaxBlocks / kThreads
+    Fr v[kU];
 #pragma unroll
-      for (uint32_t u = 0; u < kU; ++u) {
-        const uint32_t b = blk + u * groups;
-        v[u] = b < job.blocks ? load_fr(job.partial, (size_t)b * width + i) : Fr::zero();
-      }
-#pragma unroll
-      for (uint32_t h = kU / 2; h > 0; h >>= 1)
-#pragma unroll
-        for (uint32_t u = 0; u < h; ++u) v[u] = v[u] + v[u + h];
-      s = s + v[0];
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t b = threadIdx.x + u * kThreads;
+      v[u] = b < job.blocks ? load_fr(job.partial, (size_t)b * width + i) : Fr::zero();
     }
+    s = (v[0] + v[1]) + (v[2] + v[3]);
+    for (uint32_t b = threadIdx.x + kU * kThreads; b < job.blocks; b += kThreads)
+      s = s + load_fr(job.partial, (size_t)b * width + i);
   }
-  __shared__ Fr red[kThreads];
-  red[threadIdx.x] = s;
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) {
+    Fr o;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o.v[q] = __shfl_xor(s.v[q], off, 64);
+    s = s + o;
+  }
+  __shared__ Fr wsum[kThreads / 64];
+  __shared__ uint32_t last;
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (uint32_t hop = 1; hop < groups; hop <<= 1) {
-    if (g < groups && (g & (2 * hop - 1)) == 0 && g + hop < groups)
-      red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + hop * width];
-    __syncthreads();
+  const uint64_t stamp = static_cast<uint64_t>(job.class_stamp) << 32;
+  if (threadIdx.x < 64) {
+    Fr t = wsum[0];
+    for (uint32_t w = 1; w < kThreads / 64; ++w) t = t + wsum[w];
+    if (threadIdx.x < 8)
+      __hip_atomic_store(&job.class_sums[i * 8 + threadIdx.x], stamp | t.v[threadIdx.x], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0)
+      last = __hip_atomic_fetch_add(job.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == width - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  // the job's last block: every (class, t) sum, once stamped (each was
+  // stored before its block's ticket, so it arrives; the wait is bounded all
+  // the same, and a sum that never came leaves an all-ones word, which no
+  // canonical value has and the host rejects)
+  __shared__ Fr red[kMaxClassWidth];
+  __shared__ uint32_t missing;
+  if (threadIdx.x == 0) missing = 0;
+  __syncthreads();
+  for (uint32_t idx = threadIdx.x; idx < width * 8; idx += kThreads) {
+    uint64_t* w = &job.class_sums[idx];
+    uint64_t x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t spin = 0; (x & ~0xffffffffull) != stamp; ++spin) {
+      if (spin == (1u << 22)) {
+        missing = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    red[idx / 8].v[idx % 8] = static_cast<uint32_t>(x);
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(job.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (missing) {
+    if (threadIdx.x <= job.degree) {
+      Fr bad;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) bad.v[q] = 0xffffffffu;
+      store_fr(job.out, threadIdx.x, bad);
+      __threadfence_system();
+    }
+    return;
   }
   constexpr uint32_t kW = 9;  // degree <= 8
   __shared__ Fr val[kMaxClasses][kW], dif[kMaxClasses][kW], diag[kMaxClasses][kW];
@@ -1411,6 +1464,7 @@ struct State {
   const Fr* class_k = nullptr;
   uint32_t class_count = 0, class_width = 0;
   rt::BufferRef partial, result, ticket;
+  uint64_t* class_sums = nullptr;  // class finish sums (stamped words) in `ticket`'s block
   Fr* slot_base[2] = {nullptr, nullptr};
   // persistent round loop: this state's mailbox (pinned host memory, from the
   // context's pool) and whether its block may still wait on the host
@@ -1565,6 +1619,8 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out, bool graph = fa
   j.class_k = s.class_k;
   j.class_count = s.class_count;
   j.class_width = s.class_width;
+  j.class_sums = s.class_sums;
+  j.class_stamp = static_cast<uint32_t>(k) + 1;
   j.entries = static_cast<uint32_t>(s.index_count);
   return j;
 }
@@ -1640,8 +1696,10 @@ void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
                                   0, c.stream));
     if (l.mode == 3) {
       // class launches always finish in a second kernel
+      uint32_t width = 0;
+      for (uint32_t j = 0; j < l.pack.count; ++j) width = std::max(width, l.pack.job[j].class_width);
       rt::hip_check(hipLaunchKernel(reinterpret_cast<const void*>(&sc::k_finish_classes),
-                                    dim3(l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
+                                    dim3(width, l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
       continue;
     }
     bool multi = false;
@@ -2433,11 +2491,14 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_create(
     }
     s->partial = c.allocate(size_t(max_blocks) * std::max<size_t>(width, s->class_width) * rt::kFrBytes);
     s->result = c.allocate(width * rt::kFrBytes);
-    s->ticket = c.allocate(sizeof(unsigned));
+    // the ticket, then the class finish sums, cleared by one memset
+    constexpr size_t kSumsOffset = 256, kSumsBytes = sc::kMaxClassWidth * 8 * sizeof(uint64_t);
+    s->ticket = c.allocate(kSumsOffset + kSumsBytes);
+    s->class_sums = reinterpret_cast<uint64_t*>(static_cast<uint8_t*>(s->ticket->ptr) + kSumsOffset);
     hipLaunchKernelGGL(sc::k_zero_fr, dim3(1), dim3(64), 0, c.stream,
                        static_cast<Fr*>(s->result->ptr), width);
     rt::check_launch();
-    rt::hip_check(hipMemsetAsync(s->ticket->ptr, 0, sizeof(unsigned), c.stream));
+    rt::hip_check(hipMemsetAsync(s->ticket->ptr, 0, kSumsOffset + kSumsBytes, c.stream));
     c.sync();
     auto* handle = new HpMi355xSumCheckState{ctx, s};
     c.resources[handle] = {rt::Context::Kind::sumcheck_state, s};
@@ -2474,6 +2535,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
         std::vector<sc::RoundJob> jobs{make_job(s, ch, host)};
         launch_jobs(c, jobs);
         c.sync();
+        // a class finish whose sums never arrived writes non-canonical words
+        if (!rt::canonical_fr_array(reinterpret_cast<const uint8_t*>(host), width)) rt::fail_state(-4);
         std::memcpy(out_evaluations, host, width * rt::kFrBytes);
       } else {
         generic_round(c, s, ch);
@@ -2570,6 +2633,8 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
       }
       if (!jobs.empty()) launch_jobs(c, jobs);
       c.sync();
+      // a class finish whose sums never arrived writes non-canonical words
+      if (!rt::canonical_fr_array(reinterpret_cast<const uint8_t*>(out_base), total)) rt::fail_state(-4);
       for (size_t i = 0; i < count; ++i) {
         std::memcpy(out_evaluations[i], out_base + offset[i], (ss[i]->max_degree + 1) * rt::kFrBytes);
         ss[i]->round += 1;
