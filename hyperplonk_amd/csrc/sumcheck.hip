@@ -353,7 +353,7 @@ constexpr uint32_t kMaxClasses = 16, kMaxClassWidth = 64;
 // a one-factor term < 3r, a product term < 5.1r (17^2 / 70.7 + 1); eight
 // terms on a reduced accumulator (< 3r) stay below 44r < 2^261.
 template <int M>
-__device__ __forceinline__ void class_pass(const RoundJob& job, const Fr* const* tbl, uint32_t mb,
+__device__ __forceinline__ void class_pass_plain(const RoundJob& job, const Fr* const* tbl, uint32_t mb,
                                            uint32_t me, size_t b0, size_t stride,
                                            Fr29 (&out)[M + 1]) {
   Fr29 acc[M + 1];
@@ -399,6 +399,96 @@ __device__ __forceinline__ void class_pass(const RoundJob& job, const Fr* const*
   }
 #pragma unroll
   for (int t = 0; t <= M; ++t) out[t] = f29::reduce_est(acc[t]);
+}
+
+template <int M>
+__device__ __forceinline__ void class_pass_ahead(const RoundJob& job, const Fr* const* tbl, uint32_t mb,
+                                           uint32_t me, size_t b0, size_t stride,
+                                           Fr29 (&out)[M + 1]) {
+  Fr29 acc[M + 1];
+#pragma unroll
+  for (int t = 0; t <= M; ++t) acc[t] = f29::zero_t<f29::FrP>();
+  const uint32_t* members = job.classes + 4 * job.class_count;
+  uint32_t pending = 0;
+  // The thread's factor visits in order (pair b, member q, factor e), each
+  // visit's two table values loaded one visit ahead: the loads of the next
+  // factor are in flight while this one is multiplied.
+  auto at = [&](size_t b, uint32_t q, uint32_t e) {
+    return tbl[job.slots[job.offsets[members[q]] + e]] + 2 * b;
+  };
+  if (b0 < job.pairs && mb < me) {
+    size_t b = b0;
+    uint32_t q = mb, e = 0;
+    Fr cur_lo = load_fr(at(b, q, 0), 0), cur_hi = load_fr(at(b, q, 0), 1);
+    Fr29 term[M + 1];
+    for (;;) {
+      size_t nb = b;
+      uint32_t nq = q, ne = e + 1;
+      if (ne == M) {
+        ne = 0;
+        if (++nq == me) {
+          nq = mb;
+          nb = b + stride;
+        }
+      }
+      const bool more = nb < job.pairs;
+      Fr nxt_lo, nxt_hi;
+      if (more) {
+        const Fr* a = at(nb, nq, ne);
+        nxt_lo = load_fr(a, 0);
+        nxt_hi = load_fr(a, 1);
+      }
+      {
+        const Fr29 lo = f29::unpack_fr(cur_lo), hi = f29::unpack_fr(cur_hi);
+        const Fr29 d = f29::sub1p(hi, lo);
+        Fr29 line = lo;
+        if (e == 0) {
+#pragma unroll
+          for (int t = 0; t <= M; ++t) {
+            term[t] = line;
+            if (t < M) line = f29::add(line, d);
+          }
+        } else {
+#pragma unroll
+          for (int t = 0; t <= M; ++t) {
+            term[t] = f29::mul(term[t], line);
+            if (t < M) line = f29::add(line, d);
+          }
+        }
+      }
+      if (e + 1 == M) {
+#pragma unroll
+        for (int t = 0; t <= M; ++t) acc[t] = f29::add(acc[t], term[t]);
+        if (++pending == 8) {
+#pragma unroll
+          for (int t = 0; t <= M; ++t) acc[t] = f29::reduce_est(acc[t]);
+          pending = 0;
+        }
+      }
+      if (!more) break;
+      cur_lo = nxt_lo;
+      cur_hi = nxt_hi;
+      b = nb;
+      q = nq;
+      e = ne;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t <= M; ++t) out[t] = f29::reduce_est(acc[t]);
+}
+
+// A class's sums over the thread's pairs.  Degrees >= 5 load each factor one
+// visit ahead: that kernel then fits 228 VGPRs without spilling (it spilled
+// at 256) and its rounds run 1-2 % faster; at degree 2 the look-ahead's
+// registers cost a wavefront per SIMD and the rounds ran 3-12 % slower.
+template <int M, bool kAhead>
+__device__ __forceinline__ void class_pass(const RoundJob& job, const Fr* const* tbl, uint32_t mb,
+                                           uint32_t me, size_t b0, size_t stride,
+                                           Fr29 (&out)[M + 1]) {
+  if constexpr (kAhead)
+    class_pass_ahead<M>(job, tbl, mb, me, b0, stride, out);
+  else
+    class_pass_plain<M>(job, tbl, mb, me, b0, stride, out);
 }
 
 __device__ __forceinline__ Fr29 shfl_xor29(const Fr29& v, int offset) {
@@ -452,11 +542,11 @@ __device__ __forceinline__ void wave_reduce_store(const Fr29 (&v)[W], Fr* row) {
   if ((lane & ((64 >> LW) - 1)) == 0 && idx < W) row[idx] = f29::pack_fr_reduced(f29::reduce_est(x[0]));
 }
 
-template <int M>
+template <int M, bool kAhead>
 __device__ __forceinline__ void class_run(const RoundJob& job, const Fr* const* tbl, uint32_t c,
                                           size_t b0, size_t stride, Fr (*lds)[kMaxClassWidth]) {
   Fr29 v[M + 1];
-  class_pass<M>(job, tbl, job.classes[4 * c + 2], job.classes[4 * c + 3], b0, stride, v);
+  class_pass<M, kAhead>(job, tbl, job.classes[4 * c + 2], job.classes[4 * c + 3], b0, stride, v);
   wave_reduce_store<M + 1>(v, &lds[threadIdx.x >> 6][job.classes[4 * c + 1]]);
 }
 
@@ -472,7 +562,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pac
   const size_t stride = (size_t)job.blocks * kThreads;
   const size_t b0 = (size_t)lblock * kThreads + threadIdx.x;
   const Fr* const* tbl = job.fold ? job.dst : job.src;
-  if (job.fold) {
+  constexpr bool kAhead = D >= 5;
+  if (!kAhead && job.fold) {
     const Fr29 c29 = f29::prescale_fr(job.challenge);
     for (size_t b = b0; b < job.pairs; b += stride) {
       for (uint32_t f = 0; f < job.factors; ++f) {
@@ -485,17 +576,52 @@ __global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pac
     }
     // each thread reads back only the folded pairs it wrote itself
   }
+  if (kAhead && job.fold && b0 < job.pairs && job.factors > 0) {
+    const Fr29 c29 = f29::prescale_fr(job.challenge);
+    // visits (pair b, factor f) in order, each visit's four values loaded one
+    // visit ahead (as in class_pass)
+    size_t b = b0;
+    uint32_t f = 0;
+    Fr cur[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cur[i] = load_fr(job.src[0], 4 * b + i);
+    for (;;) {
+      size_t nb = b;
+      uint32_t nf = f + 1;
+      if (nf == job.factors) {
+        nf = 0;
+        nb = b + stride;
+      }
+      const bool more = nb < job.pairs;
+      Fr nxt[4];
+      if (more) {
+        const Fr* o = job.src[nf];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) nxt[i] = load_fr(o, 4 * nb + i);
+      }
+      const Fr29 a0 = f29::unpack_fr(cur[0]), a1 = f29::unpack_fr(cur[1]);
+      const Fr29 a2 = f29::unpack_fr(cur[2]), a3 = f29::unpack_fr(cur[3]);
+      store_fr(job.dst[f], 2 * b, f29::pack_fr_canonical(fold29(c29, a0, a1)));
+      store_fr(job.dst[f], 2 * b + 1, f29::pack_fr_canonical(fold29(c29, a2, a3)));
+      if (!more) break;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cur[i] = nxt[i];
+      b = nb;
+      f = nf;
+    }
+    // each thread reads back only the folded pairs it wrote itself
+  }
   __shared__ Fr lds[kThreads / 64][kMaxClassWidth];
   for (uint32_t c = 0; c < job.class_count; ++c) {
     switch (job.classes[4 * c]) {
-      case 1: class_run<1>(job, tbl, c, b0, stride, lds); break;
-      case 2: if constexpr (D >= 2) class_run<2>(job, tbl, c, b0, stride, lds); break;
-      case 3: if constexpr (D >= 3) class_run<3>(job, tbl, c, b0, stride, lds); break;
-      case 4: if constexpr (D >= 4) class_run<4>(job, tbl, c, b0, stride, lds); break;
-      case 5: if constexpr (D >= 5) class_run<5>(job, tbl, c, b0, stride, lds); break;
-      case 6: if constexpr (D >= 6) class_run<6>(job, tbl, c, b0, stride, lds); break;
-      case 7: if constexpr (D >= 7) class_run<7>(job, tbl, c, b0, stride, lds); break;
-      case 8: if constexpr (D >= 8) class_run<8>(job, tbl, c, b0, stride, lds); break;
+      case 1: class_run<1, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 2: if constexpr (D >= 2) class_run<2, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 3: if constexpr (D >= 3) class_run<3, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 4: if constexpr (D >= 4) class_run<4, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 5: if constexpr (D >= 5) class_run<5, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 6: if constexpr (D >= 6) class_run<6, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 7: if constexpr (D >= 7) class_run<7, kAhead>(job, tbl, c, b0, stride, lds); break;
+      case 8: if constexpr (D >= 8) class_run<8, kAhead>(job, tbl, c, b0, stride, lds); break;
       default: break;
     }
   }
