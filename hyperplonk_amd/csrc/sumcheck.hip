@@ -1437,22 +1437,25 @@ __global__ void __launch_bounds__(kTailThreads) k_round_tail(TailMailbox* const*
           const uint32_t tt = j >> blocks_log2, q = j & (G - 1);
           Fr v = Fr::zero();
           if (j < width * G) {
+            // all eight words again while any is unstamped: one memory round
+            // trip a poll (a word at a time cost up to eight in a row)
             uint64_t* w = &wide.part[(q * kTailMaxWidth + tt) * 8];
             uint64_t x[8];
+            for (uint32_t spin = 0;; ++spin) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) x[i] = agent_load64(w + i);
+              for (int i = 0; i < 8; ++i) x[i] = agent_load64(w + i);
+              bool ok = true;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-              for (uint32_t spin = 0; (x[i] & ~0xffffffffull) != stamp; ++spin) {
-                if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
-                  late = true;
-                  break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                x[i] = agent_load64(w + i);
+              for (int i = 0; i < 8; ++i) ok = ok && (x[i] & ~0xffffffffull) == stamp;
+              if (ok) break;
+              if (spin >= spin_cap || static_cast<uint64_t>(wall_clock64()) - start > wait_ticks) {
+                late = true;
+                break;
               }
-              v.v[i] = static_cast<uint32_t>(x[i]);
+              __builtin_amdgcn_s_sleep(1);
             }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v.v[i] = static_cast<uint32_t>(x[i]);
           }
 #pragma unroll
           for (uint32_t off = 32; off > 0; off >>= 1) {
