@@ -267,7 +267,7 @@ __device__ void finish_job(const RoundJob& job, uint32_t lblock, Fr (&acc)[D + 1
 // Second phase of a split launch: block j adds the block partials of job j of
 // the pack (strided shares of each point t, then one thread per t) and writes
 // its canonical message.  Single-block jobs finished in the round kernel.
-__global__ void __launch_bounds__(kThreads) k_finish_split(const JobPack pack) {
+__device__ __forceinline__ void finish_split_body(const JobPack& pack) {
   const RoundJob& job = pack.job[blockIdx.x];
   if (job.blocks <= 1) return;
   const uint32_t width = job.degree + 1, groups = kThreads / width;
@@ -290,11 +290,14 @@ __global__ void __launch_bounds__(kThreads) k_finish_split(const JobPack pack) {
     __threadfence_system();  // `out` may be pinned host memory read after the sync
   }
 }
+__global__ void __launch_bounds__(kThreads) k_finish_split(const JobPack pack) { finish_split_body(pack); }
+// the same kernel over a pack in device memory (replayed round graphs)
+__global__ void __launch_bounds__(kThreads) k_finish_split_dev(const JobPack* pack) { finish_split_body(*pack); }
 
 // Thread per pair (grid-strided): fold every distinct factor on the pair, then
 // every product.
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
+__device__ __forceinline__ void round_pairs_body(const JobPack& pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
   const size_t stride = (size_t)job.blocks * kThreads;
@@ -332,6 +335,11 @@ __global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) {
   to_stored<D>(acc, out);
   finish_job<D>(job, lblock, out, pack.split != 0);
 }
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_pairs(const JobPack pack) { round_pairs_body<D>(pack); }
+// the same kernel over a pack in device memory (replayed round graphs)
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_pairs_dev(const JobPack* pack) { round_pairs_body<D>(*pack); }
 
 // ---- class-major pairs decomposition (large tables) -------------------------
 //
@@ -556,7 +564,7 @@ __device__ __forceinline__ void class_run(const RoundJob& job, const Fr* const* 
 // At most 256 registers per lane (VGPRs and AGPRs together): two waves per
 // SIMD, which the degree-5 passes would otherwise lose by a few registers.
 template <int D>
-__global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pack) {
+__device__ __forceinline__ void round_classes_body(const JobPack& pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
   const size_t stride = (size_t)job.blocks * kThreads;
@@ -633,6 +641,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pac
     store_fr(job.partial, (size_t)lblock * job.class_width + threadIdx.x, s);
   }
 }
+template <int D>
+__global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pack) { round_classes_body<D>(pack); }
+// the same kernel over a pack in device memory (replayed round graphs)
+template <int D>
+__global__ void __launch_bounds__(kThreads, 2) k_round_classes_dev(const JobPack* pack) { round_classes_body<D>(*pack); }
 
 // Second phase of a class launch, block (i, j) per job j and (class, t) sum
 // i: the block adds value i of every block partial of job j (four rows a
@@ -645,7 +658,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_round_classes(const JobPack pac
 // constant, and f(t + 1) follows from e_j += e_(j+1) for j = m - 1 .. 0.  A
 // single block reading all partials took 13-40 us a round (up to ~100 partial
 // loads in a row per thread); this takes a few.
-__global__ void __launch_bounds__(kThreads) k_finish_classes(const JobPack pack) {
+__device__ __forceinline__ void finish_classes_body(const JobPack& pack) {
   const RoundJob& job = pack.job[blockIdx.y];
   const uint32_t width = job.class_width, i = blockIdx.x;
   if (i >= width) return;
@@ -747,6 +760,9 @@ __global__ void __launch_bounds__(kThreads) k_finish_classes(const JobPack pack)
     __threadfence_system();  // `out` may be pinned host memory read after the sync
   }
 }
+__global__ void __launch_bounds__(kThreads) k_finish_classes(const JobPack pack) { finish_classes_body(pack); }
+// the same kernel over a pack in device memory (replayed round graphs)
+__global__ void __launch_bounds__(kThreads) k_finish_classes_dev(const JobPack* pack) { finish_classes_body(*pack); }
 
 // ---- latency-bound decompositions (small tables) ----------------------------
 //
@@ -897,7 +913,7 @@ __device__ __forceinline__ void items_body(const JobPack& pack, const RoundJob& 
 }
 
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
+__device__ __forceinline__ void round_items_body(const JobPack& pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   __shared__ SmallMeta meta;
   const MetaView mv{job, meta, load_meta(job, meta)};
@@ -906,6 +922,11 @@ __global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) {
   else
     items_body<D, false>(pack, job, mv);
 }
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_items(const JobPack pack) { round_items_body<D>(pack); }
+// the same kernel over a pack in device memory (replayed round graphs)
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_items_dev(const JobPack* pack) { round_items_body<D>(*pack); }
 
 // ---- the smallest rounds: (pair, product, point) items in three phases ------
 //
@@ -949,7 +970,7 @@ __host__ __device__ inline PointsShape points_shape(uint32_t products, uint32_t 
 }
 
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
+__device__ __forceinline__ void round_points_body(const JobPack& pack) {
   const RoundJob& job = pack.job[job_of_block(pack)];
   const uint32_t lblock = blockIdx.x - job.block_begin;
   constexpr uint32_t width = D + 1;
@@ -1069,6 +1090,11 @@ __global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) {
   }
   finish_job<D>(job, lblock, out, false);
 }
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_points(const JobPack pack) { round_points_body<D>(pack); }
+// the same kernel over a pack in device memory (replayed round graphs)
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_round_points_dev(const JobPack* pack) { round_points_body<D>(*pack); }
 
 // ---- persistent round loops for the last (small) rounds -------------------
 //
@@ -1720,9 +1746,7 @@ void round_tables(const sc::State& s, size_t k, const Fr* const** src, Fr* const
   *dst = k == 0 ? nullptr : const_cast<Fr* const*>(s.table_ptrs[1 + ((k - 1) & 1)]);
 }
 
-// graph: the job runs in a replayed round graph, whose launches finish jobs
-// with tickets (no class-major decomposition there)
-sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out, bool graph = false) {
+sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out) {
   const size_t k = s.round;
   sc::RoundJob j{};
   round_tables(s, k, &j.src, &j.dst);
@@ -1742,7 +1766,6 @@ sc::RoundJob make_job(const sc::State& s, const Fr& ch, Fr* out, bool graph = fa
   j.fold = (k > 0 && s.product_count > 0) ? 1 : 0;
   uint32_t mode = 0;
   j.blocks = job_blocks(s, k, &mode);
-  if (mode == 3 && graph) mode = 0;
   j.items_mode = mode;
   j.classes = s.classes;
   j.class_k = s.class_k;
@@ -1789,28 +1812,50 @@ std::vector<PackLaunch> plan_packs(const std::vector<sc::RoundJob>& jobs) {
   return out;
 }
 
+// dev: the variant that reads its pack from device memory (round graphs)
 template <int D>
-const void* kernel_for(uint32_t mode) {
+const void* kernel_for(uint32_t mode, bool dev) {
+  auto f = [](auto by_value, auto by_pointer, bool d) {
+    return d ? reinterpret_cast<const void*>(by_pointer) : reinterpret_cast<const void*>(by_value);
+  };
   if constexpr (D <= kMaxClassDegree) {
-    if (mode == 3) return reinterpret_cast<const void*>(&sc::k_round_classes<D>);
+    if (mode == 3) return f(&sc::k_round_classes<D>, &sc::k_round_classes_dev<D>, dev);
   }
-  return mode == 2 ? reinterpret_cast<const void*>(&sc::k_round_points<D>)
-         : mode == 1 ? reinterpret_cast<const void*>(&sc::k_round_items<D>)
-                     : reinterpret_cast<const void*>(&sc::k_round_pairs<D>);
+  return mode == 2   ? f(&sc::k_round_points<D>, &sc::k_round_points_dev<D>, dev)
+         : mode == 1 ? f(&sc::k_round_items<D>, &sc::k_round_items_dev<D>, dev)
+                     : f(&sc::k_round_pairs<D>, &sc::k_round_pairs_dev<D>, dev);
 }
 
-const void* kernel_of(uint32_t d, uint32_t mode) {
+const void* kernel_of(uint32_t d, uint32_t mode, bool dev) {
   switch (d) {
-    case 1: return kernel_for<1>(mode);
-    case 2: return kernel_for<2>(mode);
-    case 3: return kernel_for<3>(mode);
-    case 4: return kernel_for<4>(mode);
-    case 5: return kernel_for<5>(mode);
-    case 6: return kernel_for<6>(mode);
-    case 7: return kernel_for<7>(mode);
-    case 8: return kernel_for<8>(mode);
+    case 1: return kernel_for<1>(mode, dev);
+    case 2: return kernel_for<2>(mode, dev);
+    case 3: return kernel_for<3>(mode, dev);
+    case 4: return kernel_for<4>(mode, dev);
+    case 5: return kernel_for<5>(mode, dev);
+    case 6: return kernel_for<6>(mode, dev);
+    case 7: return kernel_for<7>(mode, dev);
+    case 8: return kernel_for<8>(mode, dev);
     default: rt::fail_state(-3);
   }
+}
+
+// The second kernel of a launch (0 if none): the class finish, or the split
+// finish when a job has several blocks; its grid in *grid.
+const void* finish_of(const PackLaunch& l, bool dev, dim3* grid) {
+  if (l.mode == 3) {
+    uint32_t width = 0;
+    for (uint32_t j = 0; j < l.pack.count; ++j) width = std::max(width, l.pack.job[j].class_width);
+    *grid = dim3(width, l.pack.count);
+    return dev ? reinterpret_cast<const void*>(&sc::k_finish_classes_dev)
+               : reinterpret_cast<const void*>(&sc::k_finish_classes);
+  }
+  bool multi = false;
+  for (uint32_t j = 0; j < l.pack.count; ++j) multi = multi || l.pack.job[j].blocks > 1;
+  if (!multi) return nullptr;
+  *grid = dim3(l.pack.count);
+  return dev ? reinterpret_cast<const void*>(&sc::k_finish_split_dev)
+             : reinterpret_cast<const void*>(&sc::k_finish_split);
 }
 
 // Split launches: a round kernel whose multi-block jobs only store their
@@ -1821,22 +1866,11 @@ void launch_jobs(rt::Context& c, std::vector<sc::RoundJob>& jobs) {
   for (PackLaunch& l : plan_packs(jobs)) {
     l.pack.split = 1;
     void* args[] = {&l.pack};
-    rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.mode), dim3(l.grid), dim3(sc::kThreads), args,
-                                  0, c.stream));
-    if (l.mode == 3) {
-      // class launches always finish in a second kernel
-      uint32_t width = 0;
-      for (uint32_t j = 0; j < l.pack.count; ++j) width = std::max(width, l.pack.job[j].class_width);
-      rt::hip_check(hipLaunchKernel(reinterpret_cast<const void*>(&sc::k_finish_classes),
-                                    dim3(width, l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
-      continue;
-    }
-    bool multi = false;
-    for (uint32_t j = 0; j < l.pack.count; ++j) multi = multi || l.pack.job[j].blocks > 1;
-    if (multi) {
-      rt::hip_check(hipLaunchKernel(reinterpret_cast<const void*>(&sc::k_finish_split),
-                                    dim3(l.pack.count), dim3(sc::kThreads), args, 0, c.stream));
-    }
+    rt::hip_check(hipLaunchKernel(kernel_of(l.degree, l.mode, false), dim3(l.grid), dim3(sc::kThreads),
+                                  args, 0, c.stream));
+    dim3 fgrid;
+    if (const void* fin = finish_of(l, false, &fgrid))
+      rt::hip_check(hipLaunchKernel(fin, fgrid, dim3(sc::kThreads), args, 0, c.stream));
   }
 }
 
@@ -2244,21 +2278,27 @@ bool tail_rounds(rt::Context& c, const std::vector<sc::State*>& states, const st
 // ---- round launches as hipGraphs, replayed with new challenges ----------
 //
 // A batch round whose launch shapes (per launch: degree class, decomposition,
-// grid, job count) were seen before replays the graph built for them: its
-// kernel nodes get the new job packs (tables, challenges) as parameters and
-// the graph's copy node brings the messages back into its pinned buffer.
-// Graphs hold no memset nodes; the per-job tickets are reset by the kernels.
+// grid, jobs and their block counts and class widths) were seen before
+// replays the graph built for them, with no node updates: the graph's first
+// node copies this round's job packs (tables, challenges, stamps) from its
+// pinned staging buffer into device memory, its kernels (the *_dev variants)
+// read their pack there, the same launches as launch_jobs (round kernel, then
+// the class or split finish), and the finishes write the messages straight
+// into the graph's pinned buffer.  Graphs hold no memset nodes; the per-job
+// tickets are reset by the kernels.
 struct RoundGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<hipGraphNode_t> kernels;
-  rt::BufferRef out;
-  uint8_t* host = nullptr;
+  uint8_t* host = nullptr;  // pinned: the messages
   size_t bytes = 0;
+  rt::BufferRef packs;               // the launches' packs, device
+  sc::JobPack* packs_host = nullptr;  // pinned staging, copied by the first node
+  std::vector<const sc::JobPack*> pack_ptrs;  // kernel arguments (device addresses)
   ~RoundGraph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (host) (void)hipHostFree(host);
+    if (packs_host) (void)hipHostFree(packs_host);
   }
 };
 
@@ -2270,10 +2310,9 @@ struct RoundGraphCache {
 
 int g_round_graph_key;
 
-// Off by default: measured on MI355X, a graph launch plus node-parameter
-// updates costs more than the one to a few kernel launches of a round
-// (n = 20 degree-5 proof 6.34 vs 6.02 ms; nv-12 proof 8.24 vs 7.80 ms).
-// HP_MI355X_ROUND_GRAPHS=1 turns it on.
+// Off by default (HP_MI355X_ROUND_GRAPHS=1 turns it on): the launched rounds
+// are the large ones, whose kernels outlast their launches, and the small
+// rounds run in the persistent round loops.
 bool round_graphs_enabled() {
   static const bool on = [] {
     const char* e = std::getenv("HP_MI355X_ROUND_GRAPHS");
@@ -2293,17 +2332,6 @@ RoundGraphCache& graph_cache(rt::Context& c) {
   return *static_cast<RoundGraphCache*>(it->second.object.get());
 }
 
-hipKernelNodeParams node_params(PackLaunch& l, void** args) {
-  hipKernelNodeParams p{};
-  p.func = const_cast<void*>(kernel_of(l.degree, l.mode));
-  p.gridDim = dim3(l.grid);
-  p.blockDim = dim3(sc::kThreads);
-  p.sharedMemBytes = 0;
-  p.kernelParams = args;
-  p.extra = nullptr;
-  return p;
-}
-
 // Run the jobs (outputs at offsets out_offset[i] of a total_bytes buffer)
 // through a cached or new graph and copy the messages to `host_out`.
 void run_jobs_graph(rt::Context& c, std::vector<sc::RoundJob>& jobs,
@@ -2317,7 +2345,11 @@ void run_jobs_graph(rt::Context& c, std::vector<sc::RoundJob>& jobs,
       key.push_back(l.mode);
       key.push_back(l.grid);
       key.push_back(l.pack.count);
-      for (uint32_t j = 0; j < l.pack.count; ++j) key.push_back(l.pack.job[j].degree);
+      for (uint32_t j = 0; j < l.pack.count; ++j) {
+        key.push_back(l.pack.job[j].degree);
+        key.push_back(l.pack.job[j].blocks);
+        key.push_back(l.pack.job[j].class_width);
+      }
     }
     key.push_back(static_cast<uint32_t>(total_bytes));
   }
@@ -2332,38 +2364,50 @@ void run_jobs_graph(rt::Context& c, std::vector<sc::RoundJob>& jobs,
     }
     auto ng = std::make_unique<RoundGraph>();
     ng->bytes = total_bytes;
-    ng->out = c.allocate(total_bytes);
     rt::hip_check(hipHostMalloc(reinterpret_cast<void**>(&ng->host), total_bytes, hipHostMallocDefault));
     g = ng.get();
     cache.graphs.emplace(key, std::move(ng));
     cache.order.push_back(key);
     fresh = true;
   }
-  Fr* out_base = static_cast<Fr*>(g->out->ptr);
+  Fr* out_base = reinterpret_cast<Fr*>(g->host);
   for (size_t i = 0; i < jobs.size(); ++i) jobs[i].out = out_base + out_offset[i];
   std::vector<PackLaunch> packs = plan_packs(jobs);
+  for (PackLaunch& l : packs) l.pack.split = 1;
+  const size_t pack_bytes = packs.size() * sizeof(sc::JobPack);
   if (fresh) {
+    g->packs = c.allocate(pack_bytes);
+    rt::hip_check(hipHostMalloc(reinterpret_cast<void**>(&g->packs_host), pack_bytes, hipHostMallocDefault));
+    const sc::JobPack* dev = static_cast<const sc::JobPack*>(g->packs->ptr);
+    for (size_t k = 0; k < packs.size(); ++k) g->pack_ptrs.push_back(dev + k);
     rt::hip_check(hipGraphCreate(&g->graph, 0));
-    hipGraphNode_t prev = nullptr;
-    for (PackLaunch& l : packs) {
-      void* args[] = {&l.pack};
-      hipKernelNodeParams p = node_params(l, args);
+    hipGraphNode_t prev;
+    rt::hip_check(hipGraphAddMemcpyNode1D(&prev, g->graph, nullptr, 0, g->packs->ptr, g->packs_host,
+                                          pack_bytes, hipMemcpyHostToDevice));
+    auto add_kernel = [&](const void* f, dim3 grid, const sc::JobPack** arg) {
+      void* args[] = {arg};
+      hipKernelNodeParams p{};
+      p.func = const_cast<void*>(f);
+      p.gridDim = grid;
+      p.blockDim = dim3(sc::kThreads);
+      p.sharedMemBytes = 0;
+      p.kernelParams = args;
+      p.extra = nullptr;
       hipGraphNode_t node;
-      rt::hip_check(hipGraphAddKernelNode(&node, g->graph, prev ? &prev : nullptr, prev ? 1 : 0, &p));
-      g->kernels.push_back(node);
+      rt::hip_check(hipGraphAddKernelNode(&node, g->graph, &prev, 1, &p));
       prev = node;
-    }
-    hipGraphNode_t copy;
-    rt::hip_check(hipGraphAddMemcpyNode1D(&copy, g->graph, &prev, 1, g->host, g->out->ptr, total_bytes,
-                                          hipMemcpyDeviceToHost));
-    rt::hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
-  } else {
+    };
     for (size_t k = 0; k < packs.size(); ++k) {
-      void* args[] = {&packs[k].pack};
-      hipKernelNodeParams p = node_params(packs[k], args);
-      rt::hip_check(hipGraphExecKernelNodeSetParams(g->exec, g->kernels[k], &p));
+      const PackLaunch& l = packs[k];
+      add_kernel(kernel_of(l.degree, l.mode, true), dim3(l.grid), &g->pack_ptrs[k]);
+      dim3 fgrid;
+      if (const void* fin = finish_of(l, true, &fgrid)) add_kernel(fin, fgrid, &g->pack_ptrs[k]);
     }
+    rt::hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   }
+  // the previous replay of this graph has ended (each call synchronises), so
+  // its staging buffer is free
+  for (size_t k = 0; k < packs.size(); ++k) g->packs_host[k] = packs[k].pack;
   rt::hip_check(hipGraphLaunch(g->exec, c.stream));
   rt::hip_check(hipStreamSynchronize(c.stream));
   std::memcpy(host_out, g->host, total_bytes);
@@ -2651,9 +2695,10 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round(
         return rt::ok();
       quiesce_tails(c);
       if (s.max_degree <= kMaxJobDegree && round_graphs_enabled()) {
-        std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr, true)};
+        std::vector<sc::RoundJob> jobs{make_job(s, ch, nullptr)};
         std::vector<uint8_t> out(width * rt::kFrBytes);
         run_jobs_graph(c, jobs, {0}, out.size(), out.data());
+        if (!rt::canonical_fr_array(out.data(), width)) rt::fail_state(-4);
         std::memcpy(out_evaluations, out.data(), out.size());
         s.round += 1;
         return rt::ok();
@@ -2732,11 +2777,13 @@ extern "C" HpMi355xStatus hp_mi355x_v1_sumcheck_prover_state_round_batch(
         for (size_t i = 0; i < count; ++i) {
           sc::State& s = *ss[i];
           const Fr ch = challenge_of(s, challenges_or_null ? challenges_or_null[i] : nullptr);
-          jobs.push_back(make_job(s, ch, nullptr, true));
+          jobs.push_back(make_job(s, ch, nullptr));
           offs.push_back(offset[i]);
         }
         std::vector<uint8_t> host(total * rt::kFrBytes);
         run_jobs_graph(c, jobs, offs, host.size(), host.data());
+        // a class finish whose sums never arrived writes non-canonical words
+        if (!rt::canonical_fr_array(host.data(), total)) rt::fail_state(-4);
         for (size_t i = 0; i < count; ++i) {
           std::memcpy(out_evaluations[i], host.data() + offset[i] * rt::kFrBytes,
                       (ss[i]->max_degree + 1) * rt::kFrBytes);

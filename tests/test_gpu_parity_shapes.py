@@ -4,7 +4,8 @@ batched MSM against the oracle, byte for byte.
 * the pair-per-thread kernels for D = 1..8 (processes with
   HP_MI355X_SC_ITEMS_BELOW=0 and the round loops off, so every round runs on
   them; tests/sumcheck_pairs_check.py): k_round_classes<D> by default and
-  k_round_pairs<D> with HP_MI355X_SC_CLASSES=0; the same cases through the
+  k_round_pairs<D> with HP_MI355X_SC_CLASSES=0, and the class kernels replayed
+  from round graphs (HP_MI355X_ROUND_GRAPHS=1); the same cases through the
   persistent round loops (wide loops of 256, 16 and 2 blocks, then fewer
   blocks, then one); and the 14-table degree-5 jellyfish-style gate of the
   n = 20 workload at num_vars 16 on the default thresholds (round 0 on the
@@ -35,11 +36,13 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.mark.parametrize("classes", ["1", "0"])
-def test_pairs_kernel_every_degree_matches_oracle(classes):
-    # round loops off: every round is a launched pair-per-thread round
+@pytest.mark.parametrize("classes,graphs", [("1", "0"), ("0", "0"), ("1", "1")])
+def test_pairs_kernel_every_degree_matches_oracle(classes, graphs):
+    # round loops off: every round is a launched pair-per-thread round (with
+    # graphs "1", replayed from hipGraphs that read their packs from device memory)
     env = dict(os.environ, HP_MI355X_SC_ITEMS_BELOW="0", HP_MI355X_SC_CLASSES=classes,
-               HP_MI355X_SC_WIDE_ITEMS="0", HP_MI355X_SC_TAIL_SINGLE_ITEMS="0")
+               HP_MI355X_SC_WIDE_ITEMS="0", HP_MI355X_SC_TAIL_SINGLE_ITEMS="0",
+               HP_MI355X_ROUND_GRAPHS=graphs)
     res = subprocess.run([sys.executable, "-u", os.path.join(ROOT, "tests", "sumcheck_pairs_check.py")],
                          capture_output=True, text=True, timeout=300, env=env, cwd=ROOT)
     print(res.stdout[-3000:])

@@ -326,23 +326,31 @@ def test_round_batch_checks_before_work(backend):
 
 @pytest.mark.parametrize("degree", ["2", "5"])
 def test_round_graph_replay_gives_same_messages(degree):
-    """Rounds replayed from hipGraphs (HP_MI355X_ROUND_GRAPHS=1) produce the
-    same messages as direct launches: whole n = 12 sumchecks, 1 + 2 runs each
-    (the later runs replay the graphs built by the first)."""
+    """Rounds replayed from hipGraphs (HP_MI355X_ROUND_GRAPHS=1, the round
+    loops off so that every round is a launched one) produce the same messages
+    as the defaults: whole n = 12 sumchecks, 1 + 2 runs each (the later runs
+    replay the graphs built by the first, with the packs copied in by the
+    graph's first node), on the default decompositions, on the class kernels
+    and their finish for every round, and on the pair kernels and the split
+    finish for every round."""
     import json
     import os
     import subprocess
 
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     prog = os.path.join(root, "_build", "prove_bench")
+    loops_off = {"HP_MI355X_SC_WIDE_ITEMS": "0", "HP_MI355X_SC_TAIL_SINGLE_ITEMS": "0"}
     digests = []
-    for graphs in ("0", "1"):
-        env = dict(os.environ, HP_MI355X_ROUND_GRAPHS=graphs)
+    for env_extra in ({}, dict(loops_off, HP_MI355X_ROUND_GRAPHS="1"),
+                      dict(loops_off, HP_MI355X_ROUND_GRAPHS="1", HP_MI355X_SC_ITEMS_BELOW="0"),
+                      dict(loops_off, HP_MI355X_ROUND_GRAPHS="1", HP_MI355X_SC_ITEMS_BELOW="0",
+                           HP_MI355X_SC_CLASSES="0")):
+        env = dict(os.environ, **env_extra)
         res = subprocess.run([prog, "sumcheck", "12", degree, "2", "3"], capture_output=True, text=True,
                              timeout=120, env=env)
-        assert res.returncode == 0, res.stderr
+        assert res.returncode == 0, (env_extra, res.stderr)
         digests.append(json.loads(res.stdout.strip().splitlines()[-1])["messages_sha3_8"])
-    assert digests[0] == digests[1]
+    assert all(d == digests[0] for d in digests), digests
 
 
 @pytest.mark.parametrize("degree", ["2", "5"])
