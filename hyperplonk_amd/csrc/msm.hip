@@ -1105,6 +1105,24 @@ bool quad_lanes() {
   return on;
 }
 
+// At most 2^sum_lanes_cap() lanes per bucket in the bucket sums of MSMs with
+// at least 16384 buckets (HP_MI355X_MSM_SUM_LANES, default 2).  Fewer lanes
+// mean longer sequential chains but fewer tree levels over the lanes, and the
+// narrower launch then admits two or four threads per addition: measured on
+// MI355X, bucket sums at 2^16 (c = 15, ~9.5 partials a bucket) 0.120 ms with 8
+// lanes of one thread, 0.094 with 4 lanes of four (whole MSM 0.645 -> 0.615
+// ms); 2^18 0.150 against 0.131; 2^20 keeps its 2 lanes.  Applied to the 4096-
+// bucket MSMs of an nv-12 proof the cap cost 0.15 ms a proof, and one lane of
+// four threads sends too many buckets to the heavy path at 2^12 (0.054 ->
+// 0.17 ms), hence the bucket-count condition and the cap of 2.
+uint32_t sum_lanes_cap() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("HP_MI355X_MSM_SUM_LANES");
+    return e ? static_cast<uint32_t>(std::min(6, std::max(0, std::atoi(e)))) : 2u;
+  }();
+  return v;
+}
+
 // Entries per lane task by the largest MSM of a batch (autotuner sweep; for
 // batches of 2^12-point MSMs, 4 beats 2 and 8 in whole proofs: the shorter
 // bucket-sum chains outweigh the longer lane tasks).
@@ -1261,12 +1279,14 @@ void Batch::enqueue(rt::Context& c, G1Xyzz* result, StageEvents* ev,
                p.c, p.windows, p.buckets, e.bucket_base, e.chunks, (uint32_t)e.cc_base,
                (uint32_t)fold_total, 0u, 0u, 0u, 0u};
     fold_total += e.layer->n;
-    // bucket-sum shape of this MSM as if alone: lanes per bucket while each
-    // lane keeps >= 2 partials and the job's threads stay <= 131072; heavy
-    // threshold three times its mean partials per lane
+    // bucket-sum shape of this MSM as if alone: lanes per bucket (at most
+    // 2^sum_lanes_cap() from 16384 buckets) while each lane keeps >= 2
+    // partials and the job's threads stay <= 131072; heavy threshold three
+    // times its mean partials per lane
     per_bucket[j] = ((double)e.layer->n * p.windows / cap + 1.0) / p.buckets;
+    const uint32_t max_lanes = p.buckets >= 16384 ? sum_lanes_cap() : 6u;
     uint32_t l = 0;
-    while (l < 6 && ((size_t)p.buckets << (l + 1)) <= 131072 && (double)(2u << l) < per_bucket[j])
+    while (l < max_lanes && ((size_t)p.buckets << (l + 1)) <= 131072 && (double)(2u << l) < per_bucket[j])
       ++l;
     jobs[j].lanes_log2 = l;
     jobs[j].padded_buckets = (p.buckets + kRed1 - 1) / kRed1 * kRed1;
