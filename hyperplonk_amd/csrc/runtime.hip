@@ -7,6 +7,7 @@
 // freed while work may still read them, and context_destroy drains the stream
 // before returning memory to the driver.
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <atomic>
 #include <condition_variable>
@@ -283,6 +284,18 @@ void download_fr(Context& c, const Fr* src, size_t count, uint8_t* host) {
 
 namespace {
 // A batch of items: workers and the caller claim indices until none is left.
+// Polls `ready` for at most `limit` (a pause between polls); returns whether
+// it became true.
+template <class F>
+bool spin_until(F ready, std::chrono::microseconds limit) {
+  const auto until = std::chrono::steady_clock::now() + limit;
+  for (uint32_t k = 0;; ++k) {
+    if (ready()) return true;
+    __builtin_ia32_pause();
+    if ((k & 63) == 63 && std::chrono::steady_clock::now() >= until) return false;
+  }
+}
+
 struct ForkJoin {
   const std::function<void(size_t)>* body;
   size_t n;
@@ -311,6 +324,7 @@ class Pool {
     {
       std::lock_guard<std::mutex> g(mu_);
       for (size_t k = 0; k < helpers; ++k) queue_.push_back(job);
+      pending_.fetch_add(helpers, std::memory_order_release);
     }
     cv_.notify_all();
   }
@@ -327,15 +341,22 @@ class Pool {
   void loop() {
     for (;;) {
       std::shared_ptr<ForkJoin> job;
+      // fork-joins come in bursts (an MSM finish runs three in a row): look
+      // for the next one for a short while before sleeping, so it does not
+      // wait for a wake-up
+      spin_until([&] { return pending_.load(std::memory_order_acquire) != 0; }, kWorkerSpin);
       {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&] { return !queue_.empty(); });
         job = std::move(queue_.front());
         queue_.pop_front();
+        pending_.fetch_sub(1, std::memory_order_relaxed);
       }
       job->run();
     }
   }
+  static constexpr std::chrono::microseconds kWorkerSpin{200};
+  std::atomic<size_t> pending_{0};  // queued entries, read without the lock
   std::mutex mu_;
   std::condition_variable cv_;
   std::deque<std::shared_ptr<ForkJoin>> queue_;
@@ -355,6 +376,11 @@ void parallel_for(size_t n, const std::function<void(size_t)>& body) {
   Pool& pool = Pool::get();
   pool.submit(job, std::min(n - 1, pool.workers()));
   job->run();
+  // the helpers finish their last items within microseconds of the caller:
+  // poll before sleeping on the condition variable
+  if (spin_until([&] { return job->done.load(std::memory_order_acquire) == n; },
+                 std::chrono::microseconds(2000)))
+    return;
   std::unique_lock<std::mutex> lk(job->mu);
   job->cv.wait(lk, [&] { return job->done.load() == n; });
 }
