@@ -217,13 +217,31 @@ __global__ void k_zero_u32(unsigned* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
-// Per-thread Montgomery batch inversion of prod_k g_k over a strided set of
-// entries; out = prod_k f_k / prod_k g_k.  A zero denominator raises *zero.
-__global__ void k_frac(const Fr* const* nums, const Fr* const* dens, size_t count, Fr* out,
-                       size_t n, unsigned* zero) {
+// Montgomery batch inversion of d_j = prod_k g_k[j] over all entries, out =
+// prod_k f_k / prod_k g_k, in four launches and one host inversion:
+//   k_frac_prefix  thread t (of T, a power of two) walks its strided entries:
+//                  out[j] := the exclusive prefix product of its d_j, tree[t]
+//                  := its total; a zero d_j raises aux->zero;
+//   k_frac_up      one block: the product tree over the T totals (levels
+//                  stored one after another, the root last, copied to aux);
+//   host           inverts the root (one inversion instead of one Fermat
+//                  chain per thread, whose ~340 dependent products set the
+//                  kernel's time: 154 us at nv 12);
+//   k_frac_down    one block: the inverses down the tree, in place
+//                  (inv(left) = inv(parent) right, inv(right) = inv(parent)
+//                  left), leaving 1 / total_t in tree[t];
+//   k_frac_back    thread t walks its entries backwards as before.
+struct FracAux {
+  unsigned zero;
+  unsigned pad[7];
+  Fr root;
+};
+
+__global__ void k_frac_prefix(const Fr* const* dens, size_t count, Fr* out, size_t n, Fr* tree,
+                              FracAux* aux) {
   const size_t threads = (size_t)gridDim.x * blockDim.x;
   const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  // pass 1: out[j] := prefix product of denominators (exclusive), acc = total
+  if (tid >= n) return;
   Fr acc = Fr::one();
   bool saw_zero = false;
   for (size_t j = tid; j < n; j += threads) {
@@ -233,10 +251,50 @@ __global__ void k_frac(const Fr* const* nums, const Fr* const* dens, size_t coun
     store_fr(out, j, acc);
     acc = acc * d;
   }
-  if (saw_zero) atomicOr(zero, 1u);
+  if (saw_zero) atomicOr(&aux->zero, 1u);
+  store_fr(tree, tid, acc);
+}
+
+__global__ void __launch_bounds__(1024) k_frac_up(Fr* tree, size_t T, FracAux* aux) {
+  Fr* level = tree;
+  for (size_t len = T; len > 1; len /= 2) {
+    for (size_t i = threadIdx.x; i < len / 2; i += blockDim.x)
+      store_fr(level + len, i, load_fr(level, 2 * i) * load_fr(level, 2 * i + 1));
+    __syncthreads();
+    level += len;
+  }
+  if (threadIdx.x == 0) {
+    store_fr(&aux->root, 0, load_fr(level, 0));
+    __threadfence_system();
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_frac_down(Fr* tree, size_t T, Fr root_inverse) {
+  // level offsets: level l starts at T + T/2 + ... (l terms) = 2T - 2T/2^l
+  size_t levels = 0;
+  while ((size_t(1) << levels) < T) ++levels;
+  if (threadIdx.x == 0) store_fr(tree, 2 * T - 2, root_inverse);
+  __syncthreads();
+  for (size_t l = levels; l-- > 0;) {
+    const size_t len = T >> l, off = 2 * T - 2 * len;  // level l: len entries at off
+    Fr* level = tree + off;
+    const Fr* parent = tree + off + len;
+    for (size_t p = threadIdx.x; p < len / 2; p += blockDim.x) {
+      const Fr a = load_fr(level, 2 * p), b = load_fr(level, 2 * p + 1), ip = load_fr(parent, p);
+      store_fr(level, 2 * p, ip * b);
+      store_fr(level, 2 * p + 1, ip * a);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_frac_back(const Fr* const* nums, const Fr* const* dens, size_t count, Fr* out,
+                            size_t n, const Fr* tree) {
+  const size_t threads = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (tid >= n) return;
-  Fr inv = f29::inverse_fr(acc);
-  // pass 2 (backwards): inv_j = inv * prefix_j; inv *= d_j
+  Fr inv = load_fr(tree, tid);
+  // backwards: inv_j = inv * prefix_j; inv *= d_j
   size_t last = tid + ((n - 1 - tid) / threads) * threads;
   for (size_t j = last;; j -= threads) {
     Fr d = load_fr(dens[0], j);
@@ -735,6 +793,93 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_build_eq_x_r(HpMi355xContext* ctx, co
   });
 }
 
+// a R -> 1/a R (Montgomery form, R = 2^256; a != 0) on the host: the binary
+// extended Euclidean algorithm on the integer A = a R gives A^-1 = 1/a R^-1,
+// and one Montgomery product by R^3 mod r returns 1/a R.
+Fr host_inverse_fr(const Fr& a) {
+  using u64 = uint64_t;
+  using u128 = unsigned __int128;
+  auto limbs = [](const uint32_t* w, u64* x) {
+    for (int i = 0; i < 4; ++i) x[i] = (u64)w[2 * i] | ((u64)w[2 * i + 1] << 32);
+  };
+  u64 r[4], u[4], v[4], x1[4] = {1, 0, 0, 0}, x2[4] = {0, 0, 0, 0};
+  limbs(FrParams::MOD, r);
+  limbs(a.v, u);
+  std::memcpy(v, r, sizeof v);
+  auto is_one = [](const u64* x) { return x[0] == 1 && (x[1] | x[2] | x[3]) == 0; };
+  auto geq = [](const u64* x, const u64* y) {
+    for (int i = 3; i >= 0; --i)
+      if (x[i] != y[i]) return x[i] > y[i];
+    return true;
+  };
+  auto sub = [](u64* x, const u64* y) {  // x -= y, no underflow
+    u64 borrow = 0;
+    for (int i = 0; i < 4; ++i) {
+      const u128 d = (u128)x[i] - y[i] - borrow;
+      x[i] = (u64)d;
+      borrow = (u64)(d >> 64) & 1;
+    }
+  };
+  auto half_mod = [&](u64* x) {  // x / 2 mod r for x < r (x + r < 2^256)
+    if (x[0] & 1) {
+      u64 carry = 0;
+      for (int i = 0; i < 4; ++i) {
+        const u128 t = (u128)x[i] + r[i] + carry;
+        x[i] = (u64)t;
+        carry = (u64)(t >> 64);
+      }
+    }
+    for (int i = 0; i < 3; ++i) x[i] = (x[i] >> 1) | (x[i + 1] << 63);
+    x[3] >>= 1;
+  };
+  auto half = [](u64* x) {
+    for (int i = 0; i < 3; ++i) x[i] = (x[i] >> 1) | (x[i + 1] << 63);
+    x[3] >>= 1;
+  };
+  auto sub_mod = [&](u64* x, const u64* y) {  // x - y mod r, both < r
+    if (geq(x, y)) {
+      sub(x, y);
+    } else {
+      u64 t[4];
+      std::memcpy(t, r, sizeof t);
+      sub(t, y);
+      u64 carry = 0;
+      for (int i = 0; i < 4; ++i) {
+        const u128 s = (u128)x[i] + t[i] + carry;
+        x[i] = (u64)s;
+        carry = (u64)(s >> 64);
+      }
+    }
+  };
+  // invariants: x1 A = u, x2 A = v (mod r); u, v > 0 (A != 0 mod r)
+  while (!is_one(u) && !is_one(v)) {
+    while ((u[0] & 1) == 0) {
+      half(u);
+      half_mod(x1);
+    }
+    while ((v[0] & 1) == 0) {
+      half(v);
+      half_mod(x2);
+    }
+    if (geq(u, v)) {
+      sub(u, v);
+      sub_mod(x1, x2);
+    } else {
+      sub(v, u);
+      sub_mod(x2, x1);
+    }
+  }
+  const u64* x = is_one(u) ? x1 : x2;
+  Fr ainv, r2;
+  for (int i = 0; i < 4; ++i) {
+    ainv.v[2 * i] = (uint32_t)x[i];
+    ainv.v[2 * i + 1] = (uint32_t)(x[i] >> 32);
+  }
+  std::memcpy(r2.v, FrParams::R2, sizeof r2.v);
+  const Fr r3 = r2 * r2;  // R^4 / R
+  return ainv * r3;
+}
+
 extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_frac_poly(
     HpMi355xContext* ctx, size_t num_vars, const HpMi355xDeviceAddress* numerators,
     const HpMi355xDeviceAddress* denominators, size_t count, HpMi355xDeviceAddress* out) {
@@ -758,22 +903,34 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_compute_frac_poly(
     rt::BufferRef result = c.allocate(n * rt::kFrBytes, num_vars);
     rt::BufferRef dn = upload_pointers(c, nums);
     rt::BufferRef dd = upload_pointers(c, dens);
-    rt::BufferRef flag = c.allocate(sizeof(unsigned));
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.stream, static_cast<unsigned*>(flag->ptr));
+    rt::BufferRef aux_buf = c.allocate(sizeof(FracAux));
+    FracAux* aux = static_cast<FracAux*>(aux_buf->ptr);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.stream, &aux->zero);
     rt::check_launch();
-    // one Fermat inversion per thread: small tables take one entry per thread
-    // (the chain is the inversion alone), large ones ~16 entries per thread
-    // with at least 16384 threads
-    const size_t threads = std::min<size_t>(
-        n, std::max<size_t>(16384, std::min<size_t>(n / 16 + 1, 65536)));
-    hipLaunchKernelGGL(k_frac, dim3(rt::blocks_for(threads, 64)), dim3(64), 0, c.stream,
-                       static_cast<const Fr* const*>(dn->ptr),
-                       static_cast<const Fr* const*>(dd->ptr), count,
-                       static_cast<Fr*>(result->ptr), n, static_cast<unsigned*>(flag->ptr));
+    // T threads (a power of two): small tables one entry per thread, large
+    // ones ~16 entries per thread with 16384 to 65536 threads
+    size_t T = 1;
+    while (T < std::min<size_t>(n, std::max<size_t>(16384, std::min<size_t>(n / 16, 65536)))) T *= 2;
+    T = std::min(T, n);
+    rt::BufferRef tree_buf = c.allocate((2 * T - 1) * rt::kFrBytes);
+    Fr* tree = static_cast<Fr*>(tree_buf->ptr);
+    const auto* dnp = static_cast<const Fr* const*>(dn->ptr);
+    const auto* ddp = static_cast<const Fr* const*>(dd->ptr);
+    Fr* res = static_cast<Fr*>(result->ptr);
+    hipLaunchKernelGGL(k_frac_prefix, dim3(rt::blocks_for(T, 64)), dim3(64), 0, c.stream, ddp, count,
+                       res, n, tree, aux);
     rt::check_launch();
-    unsigned zero = 0;
-    rt::download_bytes(c, flag->ptr, sizeof(unsigned), reinterpret_cast<uint8_t*>(&zero));
-    if (zero) return rt::invalid_parameters();  // result buffer is released unpublished
+    hipLaunchKernelGGL(k_frac_up, dim3(1), dim3(1024), 0, c.stream, tree, T, aux);
+    rt::check_launch();
+    FracAux host{};
+    rt::download_bytes(c, aux, sizeof(FracAux), reinterpret_cast<uint8_t*>(&host));
+    if (host.zero) return rt::invalid_parameters();  // result buffer is released unpublished
+    hipLaunchKernelGGL(k_frac_down, dim3(1), dim3(1024), 0, c.stream, tree, T,
+                       host_inverse_fr(host.root));
+    rt::check_launch();
+    hipLaunchKernelGGL(k_frac_back, dim3(rt::blocks_for(T, 64)), dim3(64), 0, c.stream, dnp, ddp,
+                       count, res, n, static_cast<const Fr*>(tree));
+    rt::check_launch();
     *out = c.publish(std::move(result));
     return rt::ok();
   });

@@ -133,7 +133,7 @@ def test_product_factors_known_answers(backend):
         backend.compute_product_factors(mle(backend, 0, [2]), mle(backend, 0, [2]))
 
 
-@pytest.mark.parametrize("nv", [1, 3, 10, 11, 14])
+@pytest.mark.parametrize("nv", [1, 3, 10, 11, 14, 16])
 def test_random_tables_match_oracle(backend, oracle, rng_factory, nv):
     rng = rng_factory(1000 + nv)
     n = 1 << nv
