@@ -148,18 +148,35 @@ BufferRef Context::lookup(void* handle, size_t nv) const {
   return it->second;
 }
 
-uint8_t* Context::staging(size_t bytes) {
-  if (pinned_free) {
-    // the previous copy out of the staging buffer has usually completed (its
-    // call synchronised): wait only when it has not
-    const hipError_t q = hipEventQuery(pinned_free);
-    if (q == hipErrorNotReady) {
-      (void)hipGetLastError();
-      hip_check(hipEventSynchronize(pinned_free));
-    } else {
-      hip_check(q);
-    }
+namespace {
+// Waits until the copy that last used a staging area has completed.
+void wait_staging(hipEvent_t e) {
+  if (e == nullptr) return;
+  const hipError_t q = hipEventQuery(e);
+  if (q == hipErrorNotReady) {
+    (void)hipGetLastError();
+    hip_check(hipEventSynchronize(e));
+  } else {
+    hip_check(q);
   }
+}
+}  // namespace
+
+uint8_t* Context::staging(size_t bytes) {
+  if (bytes <= kRingSlotBytes) {
+    if (ring == nullptr)
+      hip_check(hipHostMalloc(reinterpret_cast<void**>(&ring), kRingSlots * kRingSlotBytes,
+                              hipHostMallocDefault));
+    const size_t s = ring_next;
+    ring_next = (ring_next + 1) % kRingSlots;
+    wait_staging(ring_free[s]);
+    staging_slot = static_cast<int>(s);
+    return ring + s * kRingSlotBytes;
+  }
+  staging_slot = -1;
+  // the previous copy out of the staging buffer has usually completed (its
+  // call synchronised): wait only when it has not
+  wait_staging(pinned_free);
   if (bytes > pinned_bytes) {
     if (pinned) hip_check(hipHostFree(pinned));
     pinned = nullptr;
@@ -185,8 +202,9 @@ uint8_t* Context::results(size_t bytes) {
 }
 
 void Context::staging_used() {
-  if (!pinned_free) hip_check(hipEventCreateWithFlags(&pinned_free, hipEventDisableTiming));
-  hip_check(hipEventRecord(pinned_free, stream));
+  hipEvent_t& e = staging_slot >= 0 ? ring_free[staging_slot] : pinned_free;
+  if (!e) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hip_check(hipEventRecord(e, stream));
 }
 
 void Context::sync() { hip_check(hipStreamSynchronize(stream)); }
@@ -456,6 +474,9 @@ extern "C" HpMi355xStatus hp_mi355x_v1_context_destroy(HpMi355xContext* ctx) {
   if (c.pinned) (void)hipHostFree(c.pinned);
   if (c.results_host) (void)hipHostFree(c.results_host);
   if (c.pinned_free) (void)hipEventDestroy(c.pinned_free);
+  if (c.ring) (void)hipHostFree(c.ring);
+  for (hipEvent_t e : c.ring_free)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c.stream);
   if (c.side != nullptr) (void)hipStreamDestroy(c.side);
   const bool poisoned = c.poisoned;

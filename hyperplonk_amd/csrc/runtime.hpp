@@ -135,6 +135,14 @@ struct Context {
   uint8_t* pinned = nullptr;
   size_t pinned_bytes = 0;
   hipEvent_t pinned_free = nullptr;  // last copy that read/wrote `pinned`
+  // Uses of at most kRingSlotBytes take the slots of a ring in turn, each with
+  // its own event, so the small uploads of one call do not wait for each
+  // other's copies (a copy waits behind all earlier work on the stream).
+  static constexpr size_t kRingSlots = 16, kRingSlotBytes = 64 * 1024;
+  uint8_t* ring = nullptr;
+  hipEvent_t ring_free[kRingSlots] = {};
+  size_t ring_next = 0;
+  int staging_slot = -1;  // slot of the current staging use; -1: `pinned`
 
   // Allocate a new device buffer (not registered).
   BufferRef allocate(size_t bytes, size_t num_vars = 0);
