@@ -91,10 +91,7 @@ struct EqChunks {
 // t[x] = prod_{i<k} (bit_i(x) ? r_i : 1 - r_i), built in LDS by prepending one
 // coordinate per level from the highest to the lowest (eq.rs:67-95 order).
 // The chunks are independent: one launch, one block each.
-__global__ void k_eq_chunk(const Fr* r_all, EqChunks chunks) {
-  const Fr* r = r_all + chunks.shift[blockIdx.x];
-  const size_t k = chunks.bits[blockIdx.x];
-  Fr* table = const_cast<Fr*>(chunks.table[blockIdx.x]);
+__device__ __forceinline__ void eq_chunk_body(const Fr* r, size_t k, Fr* table) {
   __shared__ Fr buf[1024];
   if (threadIdx.x == 0) buf[0] = Fr::one();
   __syncthreads();
@@ -121,6 +118,44 @@ __global__ void k_eq_chunk(const Fr* r_all, EqChunks chunks) {
     len *= 2;
   }
   for (size_t i = threadIdx.x; i < len; i += blockDim.x) store_fr(table, i, buf[i]);
+}
+
+__global__ void k_eq_chunk(const Fr* r_all, EqChunks chunks) {
+  eq_chunk_body(r_all + chunks.shift[blockIdx.x], chunks.bits[blockIdx.x],
+                const_cast<Fr*>(chunks.table[blockIdx.x]));
+}
+
+// Several eq tables in two launches: one block per (table, chunk), then the
+// outer products, each table over its own range of blocks.
+struct EqJob {
+  const Fr* r;  // the table's point (device, Montgomery)
+  Fr* out;
+  uint32_t nv, block_begin, blocks;
+  EqChunks chunks;
+};
+struct EqChunkRef {
+  uint32_t job, chunk;
+};
+
+__global__ void k_eq_chunks_batch(const EqJob* jobs, const EqChunkRef* refs) {
+  const EqChunkRef ref = refs[blockIdx.x];
+  const EqJob& job = jobs[ref.job];
+  eq_chunk_body(job.r + job.chunks.shift[ref.chunk], job.chunks.bits[ref.chunk],
+                const_cast<Fr*>(job.chunks.table[ref.chunk]));
+}
+
+__global__ void k_eq_outer_batch(const EqJob* jobs, uint32_t count) {
+  uint32_t j = 0;
+  while (j + 1 < count && blockIdx.x >= jobs[j + 1].block_begin) ++j;
+  const EqJob& job = jobs[j];
+  const EqChunks& ch = job.chunks;
+  const size_t n = size_t(1) << job.nv, stride = (size_t)job.blocks * blockDim.x;
+  for (size_t x = (size_t)(blockIdx.x - job.block_begin) * blockDim.x + threadIdx.x; x < n; x += stride) {
+    Fr v = load_fr(ch.table[0], x & ((size_t(1) << ch.bits[0]) - 1));
+    for (uint32_t c = 1; c < ch.count; ++c)
+      v = v * load_fr(ch.table[c], (x >> ch.shift[c]) & ((size_t(1) << ch.bits[c]) - 1));
+    store_fr(job.out, x, v);
+  }
 }
 
 // out[x] = prod_c chunk_c[(x >> shift_c) & mask_c]
@@ -390,6 +425,91 @@ void evaluate(rt::Context& c, const Fr* table, size_t nv, const Fr* point, Fr* o
   hipLaunchKernelGGL(k_fold_tail, dim3(1), dim3(256), 0, c.stream, src, point + level, nv - level,
                      out);
   rt::check_launch();
+}
+
+// Chunk split of an eq table over nv >= 1 coordinates: <= 4 chunks of <= 10
+// bits, balanced, low bits first; the chunk tables at `t` (returns its end).
+Fr* eq_chunks(size_t nv, Fr* t, EqChunks* eq) {
+  const size_t chunks = (nv + 9) / 10;
+  eq->count = static_cast<uint32_t>(chunks);
+  size_t off = 0;
+  for (size_t k = 0; k < chunks; ++k) {
+    const size_t bits = (nv - off + (chunks - k) - 1) / (chunks - k);
+    eq->table[k] = t;
+    eq->shift[k] = static_cast<uint32_t>(off);
+    eq->bits[k] = static_cast<uint32_t>(bits);
+    t += size_t(1) << bits;
+    off += bits;
+  }
+  return t;
+}
+
+size_t eq_chunk_entries(size_t nv) {
+  const size_t chunks = (nv + 9) / 10;
+  size_t off = 0, total = 0;
+  for (size_t k = 0; k < chunks; ++k) {
+    const size_t bits = (nv - off + (chunks - k) - 1) / (chunks - k);
+    total += size_t(1) << bits;
+    off += bits;
+  }
+  return total;
+}
+
+// eq tables of several points (canonical host bytes, nv[i] >= 1 each) into
+// outs[i]: the points in one upload, the chunks in one launch, the outer
+// products in another (one launch pair per table cost ~14 us of device time
+// each at nv 12, most of it fixed).  `keep` receives the scratch buffers.
+void build_eq_batch(rt::Context& c, const std::vector<const uint8_t*>& points,
+                    const std::vector<size_t>& nv, const std::vector<Fr*>& outs,
+                    std::vector<rt::BufferRef>& keep) {
+  const size_t m = points.size();
+  if (m == 0) return;
+  size_t coords = 0, entries = 0;
+  for (size_t i = 0; i < m; ++i) {
+    if (nv[i] == 0) rt::fail_params();
+    coords += nv[i];
+    entries += eq_chunk_entries(nv[i]);
+  }
+  std::vector<uint8_t> bytes(coords * rt::kFrBytes);
+  for (size_t i = 0, at = 0; i < m; at += nv[i], ++i)
+    std::memcpy(bytes.data() + at * rt::kFrBytes, points[i], nv[i] * rt::kFrBytes);
+  rt::BufferRef dpts = c.allocate(coords * rt::kFrBytes);
+  rt::upload_fr_values(c, static_cast<Fr*>(dpts->ptr), bytes.data(), coords);
+  rt::BufferRef scratch = c.allocate(entries * rt::kFrBytes);
+  std::vector<EqJob> jobs(m);
+  std::vector<EqChunkRef> refs;
+  Fr* t = static_cast<Fr*>(scratch->ptr);
+  uint32_t blocks = 0;
+  for (size_t i = 0, at = 0; i < m; at += nv[i], ++i) {
+    EqJob& j = jobs[i];
+    j.r = static_cast<const Fr*>(dpts->ptr) + at;
+    j.out = outs[i];
+    j.nv = static_cast<uint32_t>(nv[i]);
+    t = eq_chunks(nv[i], t, &j.chunks);
+    for (uint32_t k = 0; k < j.chunks.count; ++k) refs.push_back({static_cast<uint32_t>(i), k});
+    j.block_begin = blocks;
+    j.blocks = grid(size_t(1) << nv[i]);
+    blocks += j.blocks;
+  }
+  const size_t jb = m * sizeof(EqJob), rb = refs.size() * sizeof(EqChunkRef);
+  rt::BufferRef desc = c.allocate(jb + rb);
+  uint8_t* stage = c.staging(jb + rb);
+  std::memcpy(stage, jobs.data(), jb);
+  std::memcpy(stage + jb, refs.data(), rb);
+  rt::hip_check(hipMemcpyAsync(desc->ptr, stage, jb + rb, hipMemcpyHostToDevice, c.stream));
+  c.staging_used();
+  const EqJob* djobs = static_cast<const EqJob*>(desc->ptr);
+  const EqChunkRef* drefs =
+      reinterpret_cast<const EqChunkRef*>(static_cast<const uint8_t*>(desc->ptr) + jb);
+  hipLaunchKernelGGL(k_eq_chunks_batch, dim3((unsigned)refs.size()), dim3(256), 0, c.stream, djobs,
+                     drefs);
+  rt::check_launch();
+  hipLaunchKernelGGL(k_eq_outer_batch, dim3(blocks), dim3(kThreads), 0, c.stream, djobs,
+                     static_cast<uint32_t>(m));
+  rt::check_launch();
+  keep.push_back(std::move(dpts));
+  keep.push_back(std::move(scratch));
+  keep.push_back(std::move(desc));
 }
 
 void build_eq(rt::Context& c, const Fr* r, size_t nv, Fr* out) {
@@ -694,13 +814,20 @@ extern "C" HpMi355xStatus hp_mi355x_v1_mle_evaluate_batch(
       if (d == firsts.size()) firsts.push_back(i);
       point_of[i] = d;
     }
-    std::vector<rt::BufferRef> eqs(firsts.size());
-    for (size_t d = 0; d < firsts.size(); ++d) {
-      const size_t j = firsts[d];
-      if (num_vars[j] == 0) continue;
-      rt::BufferRef dp = upload_small(c, points[j], num_vars[j]);
-      eqs[d] = c.allocate((size_t(1) << num_vars[j]) * rt::kFrBytes);
-      mle::build_eq(c, static_cast<const Fr*>(dp->ptr), num_vars[j], static_cast<Fr*>(eqs[d]->ptr));
+    std::vector<rt::BufferRef> eqs(firsts.size()), keep;
+    {
+      std::vector<const uint8_t*> pts;
+      std::vector<size_t> nvs;
+      std::vector<Fr*> outs;
+      for (size_t d = 0; d < firsts.size(); ++d) {
+        const size_t j = firsts[d];
+        if (num_vars[j] == 0) continue;
+        eqs[d] = c.allocate((size_t(1) << num_vars[j]) * rt::kFrBytes);
+        pts.push_back(points[j]);
+        nvs.push_back(num_vars[j]);
+        outs.push_back(static_cast<Fr*>(eqs[d]->ptr));
+      }
+      mle::build_eq_batch(c, pts, nvs, outs, keep);
     }
     std::vector<DotJob> jobs(count);
     uint32_t grid_blocks = 0;

@@ -197,9 +197,10 @@ def test_evaluate_large_table_matches_eq_weighted_sum(backend, rng_factory):
 
 def test_evaluate_batch_matches_oracle(backend, oracle, rng_factory):
     """Batched evaluations: mixed sizes (including a constant and a table above
-    one dot block), repeated and shared points, byte-exact against the oracle."""
+    one dot block), repeated and shared points, eq tables of one to three
+    chunks built in one batch, byte-exact against the oracle."""
     rng = rng_factory(77)
-    nvs = [0, 1, 3, 12, 12, 12, 15, 17, 5]
+    nvs = [0, 1, 3, 12, 12, 12, 15, 17, 5, 21]
     tabs = [rng.fr_bytes(1 << nv) for nv in nvs]
     mles = [backend.mle_from_evaluations(nv, t) for nv, t in zip(nvs, tabs)]
     p12 = fr_from_bytes(rng.fr_bytes(12).tobytes())
