@@ -240,6 +240,21 @@ constexpr size_t kStagingChunk = size_t(64) << 20;
 }  // namespace
 
 void upload_fr_values(Context& c, Fr* dst, const uint8_t* host, size_t count) {
+  // a handful of values (points, challenges, coefficients): converted on the
+  // host (~0.1 us each), which saves the conversion launch
+  constexpr size_t kHostConvert = 64;
+  if (count > 0 && count <= kHostConvert) {
+    uint8_t* stage = c.staging(count * kFrBytes);
+    for (size_t i = 0; i < count; ++i) {
+      Fr v;
+      std::memcpy(v.v, host + i * kFrBytes, kFrBytes);
+      v = to_mont(v);
+      std::memcpy(stage + i * kFrBytes, v.v, kFrBytes);
+    }
+    hip_check(hipMemcpyAsync(dst, stage, count * kFrBytes, hipMemcpyHostToDevice, c.stream));
+    c.staging_used();
+    return;
+  }
   size_t done = 0;
   while (done < count) {
     const size_t n = std::min(count - done, kStagingChunk / kFrBytes);
