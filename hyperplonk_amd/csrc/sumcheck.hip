@@ -2121,8 +2121,8 @@ void launch_tails(rt::Context& c, TailPool& pool, const std::vector<sc::State*>&
 }
 
 // A wide launch of one state from its current round, in up to three stages:
-// G blocks (the most the round's pairs, the CUs and the maximum allow) while
-// every block keeps a pair, then, if that stops short of the one-block loop,
+// G blocks (the most the CUs, the maximum and 32 pairs a block in the first
+// round allow) while every block keeps a pair, then, if that stops short of the one-block loop,
 // as many blocks as the round before the one-block loop has pairs, then one
 // block.  Precondition: no active loop, wide_ok(s), !tail_ok(s).
 void launch_wide(rt::Context& c, TailPool& pool, sc::State& s, const Fr& challenge) {
@@ -2136,8 +2136,15 @@ void launch_wide(rt::Context& c, TailPool& pool, sc::State& s, const Fr& challen
   };
   Stage st[3];
   int count = 0;
-  const uint32_t b1 = std::min<uint32_t>(std::min(wide_blocks_log2(), pool.cus_log2),
-                                         static_cast<uint32_t>(n - s.round - 1));
+  // at least 2^kWideMinPairsLog2 pairs a block in the first round: with fewer,
+  // the blocks' exchanges outweigh their work (nv-12 proofs: 64 blocks from
+  // round 0, 2.92-2.96 ms, against 256 blocks of 8 pairs, 2.98-3.01 ms; the
+  // n = 20 loops start with 2^15 pairs or more and keep 256 blocks)
+  constexpr uint32_t kWideMinPairsLog2 = 5;
+  const uint32_t pairs_log2 = static_cast<uint32_t>(n - s.round - 1);
+  const uint32_t b1 = std::min<uint32_t>(
+      std::min(wide_blocks_log2(), pool.cus_log2),
+      pairs_log2 > kWideMinPairsLog2 ? pairs_log2 - kWideMinPairsLog2 : std::min<uint32_t>(pairs_log2, 1));
   const size_t end1 = std::min<size_t>(tail_start, n - b1);  // round k has 2^(n-k-1) pairs
   st[count++] = {static_cast<uint32_t>(s.round), static_cast<uint32_t>(end1), b1};
   if (end1 < tail_start)
