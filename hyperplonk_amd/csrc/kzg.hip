@@ -40,6 +40,36 @@ __global__ void k_fold_quotient(const Fr* src, Fr* dst, Fr* q, const Fr* x, size
   }
 }
 
+// The last `levels` fold levels of an open in one block, from a table of
+// 2^levels <= 1024 entries: each level's quotients to q (consecutive, as the
+// level launches write them) and the final value to out[0]; the same field
+// operations as k_fold_quotient, without a launch per level.
+constexpr uint32_t kFoldTailMax = 10;
+__global__ void __launch_bounds__(512) k_fold_quotient_tail(const Fr* src, Fr* q, const Fr* x,
+                                                            uint32_t levels, Fr* out) {
+  __shared__ Fr buf[size_t(1) << kFoldTailMax];
+  size_t len = size_t(1) << levels;
+  for (size_t i = threadIdx.x; i < len; i += blockDim.x) buf[i] = load_fr(src, i);
+  __syncthreads();
+  for (uint32_t l = 0; l < levels; ++l) {
+    const Fr c = load_fr(x, l);
+    const size_t half = len / 2;  // <= 512 = blockDim.x
+    Fr v = Fr::zero();
+    if (threadIdx.x < half) {
+      const Fr lo = buf[2 * threadIdx.x], hi = buf[2 * threadIdx.x + 1];
+      const Fr d = hi - lo;
+      store_fr(q, threadIdx.x, d);
+      v = lo + c * d;
+    }
+    __syncthreads();
+    if (threadIdx.x < half) buf[threadIdx.x] = v;
+    __syncthreads();
+    q += half;
+    len = half;
+  }
+  if (threadIdx.x == 0) store_fr(out, 0, buf[0]);
+}
+
 unsigned grid_for(size_t n) {
   size_t b = (n + 255) / 256;
   return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(b, 65535)));
@@ -312,6 +342,20 @@ void enqueue_open(rt::Context& c, const kzg::Params& p, const Fr* table, size_t 
   size_t qoff = 0;
   for (size_t j = 0; j < num_vars; ++j) {
     const size_t half = n >> (j + 1);
+    if (num_vars - j <= kzg::kFoldTailMax) {
+      // the remaining levels in one block; the value lands where the level
+      // launches would have left it
+      Fr* last = ((num_vars - 1) & 1) ? fb : fa;
+      hipLaunchKernelGGL(kzg::k_fold_quotient_tail, dim3(1), dim3(512), 0, c.stream, src, q + qoff,
+                         static_cast<const Fr*>(dpoint->ptr) + j, static_cast<uint32_t>(num_vars - j), last);
+      rt::check_launch();
+      for (size_t jj = j; jj < num_vars; ++jj) {
+        batch.add(p.layers[N - num_vars + 1 + jj], q + qoff);
+        qoff += n >> (jj + 1);
+      }
+      src = last;
+      break;
+    }
     Fr* dst = (j & 1) ? fb : fa;
     hipLaunchKernelGGL(kzg::k_fold_quotient, dim3(kzg::grid_for(half)), dim3(256), 0, c.stream,
                        src, dst, q + qoff, static_cast<const Fr*>(dpoint->ptr) + j, half);

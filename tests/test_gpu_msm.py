@@ -137,7 +137,7 @@ def test_multi_commit_uneven_matches_single(backend, rng_factory, srs20):
     assert backend.multi_commit(pp, []) == []
 
 
-@pytest.mark.parametrize("nv", [0, 1, 4, 9, 14])
+@pytest.mark.parametrize("nv", [0, 1, 4, 9, 10, 11, 14])
 def test_open_matches_oracle(backend, oracle, rng_factory, srs20, nv):
     N, t, pp = srs20
     rng = rng_factory(800 + nv)
