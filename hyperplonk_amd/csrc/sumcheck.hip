@@ -734,14 +734,18 @@ __device__ __forceinline__ void finish_classes_body(const JobPack& pack) {
   constexpr uint32_t kW = 9;  // degree <= 8
   __shared__ Fr val[kMaxClasses][kW], dif[kMaxClasses][kW], diag[kMaxClasses][kW];
   const uint32_t deg = job.degree;
+  // the class coefficient times each sum: one thread per (class, t) value
+  // (one product of latency instead of m + 1 in a row)
+  static_assert(kMaxClasses * kW <= kThreads, "one value a thread");
+  if (threadIdx.x < job.class_count * kW) {
+    const uint32_t c = threadIdx.x / kW, j = threadIdx.x % kW;
+    if (j <= job.classes[4 * c]) val[c][j] = red[job.classes[4 * c + 1] + j] * load_fr(job.class_k, c);
+  }
+  __syncthreads();
   if (threadIdx.x < job.class_count) {
     const uint32_t c = threadIdx.x;
-    const uint32_t m = job.classes[4 * c], off = job.classes[4 * c + 1];
-    const Fr k = load_fr(job.class_k, c);
-    for (uint32_t j = 0; j <= m; ++j) {
-      val[c][j] = red[off + j] * k;
-      dif[c][j] = val[c][j];
-    }
+    const uint32_t m = job.classes[4 * c];
+    for (uint32_t j = 0; j <= m; ++j) dif[c][j] = val[c][j];
     diag[c][0] = val[c][m];
     for (uint32_t j = 1; j <= m; ++j) {
       for (uint32_t q = 0; q + j <= m; ++q) dif[c][q] = dif[c][q + 1] - dif[c][q];
